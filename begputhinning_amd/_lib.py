@@ -1,0 +1,97 @@
+"""ctypes binding of the C-ABI in include/bgt.h (libbgt.so, built in-tree).
+
+Loading fails loudly when the HIP library has not been built: there is no CPU
+fallback anywhere on the product path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbgt.so")
+
+BGT_OK = 0
+BGT_ERR_INVALID = 1
+BGT_ERR_HIP = 2
+BGT_ERR_NOMEM = 3
+BGT_ERR_UNSUPPORTED = 4
+BGT_ERR_NOT_CONVERGED = 5
+BGT_ERR_ADMISSION = 6
+
+DTYPES = {"uint8": 0, "int8": 1, "uint16": 2, "int16": 3, "uint32": 4, "int32": 5,
+          "float32": 6, "float64": 7, "bool": 8}
+
+
+class BGTError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bgt error {code}: {msg}")
+        self.code = code
+
+
+class Segments(ctypes.Structure):
+    _fields_ = [("nseg", ctypes.c_int), ("npts", ctypes.c_int),
+                ("x", ctypes.POINTER(ctypes.c_int16)), ("y", ctypes.POINTER(ctypes.c_int16)),
+                ("z", ctypes.POINTER(ctypes.c_int16)),
+                ("seg_start", ctypes.POINTER(ctypes.c_uint32)),
+                ("is_end", ctypes.POINTER(ctypes.c_uint8))]
+
+
+_lib = None
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_sz = ctypes.c_size_t
+_ip = ctypes.POINTER(ctypes.c_int)
+_fpp = ctypes.POINTER(ctypes.POINTER(ctypes.c_float))
+_upp = ctypes.POINTER(ctypes.POINTER(ctypes.c_uint32))
+_up = ctypes.POINTER(ctypes.c_uint32)
+
+SIGNATURES = {
+    "bgt_last_error": (ctypes.c_char_p, []),
+    "bgt_version": (_i, []),
+    "bgt_thinning2d": (_i, [_i, _i, _i, _vp, _vp]),
+    "bgt_thinning3d": (_i, [_i, _i, _i, _i, _vp, _vp]),
+    "bgt_thin2d_device": (_i, [_vp, _sz, _i, _i, _vp, _sz, _vp, _ip]),
+    "bgt_thin3d_device": (_i, [_vp, _i, _i, _i, _vp, _vp, _ip]),
+    "bgt_extract_segments2d": (_i, [_i, _vp, _i, _i, ctypes.POINTER(Segments)]),
+    "bgt_extract_segments3d": (_i, [_i, _vp, _i, _i, _i, ctypes.POINTER(Segments)]),
+    "bgt_segments_free": (None, [ctypes.POINTER(Segments)]),
+    "bgt_centerline2d": (_i, [_i, _vp, _i, _i, _i, _i, _fpp, _fpp, _upp, _up]),
+    "bgt_centerline3d": (_i, [_i, _vp, _i, _i, _i, _i, _i, _fpp, _fpp, _fpp, _upp, _up]),
+    "bgt_free": (None, [_vp]),
+    "bgt_synth_item_seed": (ctypes.c_uint64, [_i, _i]),
+    "bgt_synth_polylines": (_i, [_vp, _i, _i, ctypes.c_uint64, _i, ctypes.c_double]),
+    "bgt_synth_tree2d": (_i, [_vp, _i, _i, ctypes.c_uint64, _i, ctypes.c_double, ctypes.c_double]),
+    "bgt_synth_tree3d": (_i, [_vp, _i, _i, _i, ctypes.c_uint64, _i, _i, ctypes.c_double]),
+    "bgt_set_config2d": (_i, [_i, _i, _i]),
+    "bgt_set_config3d": (_i, [_i, _i]),
+    "bgt_synth_batch_dims": (_i, [ctypes.c_uint64, _i, _i, _i, _ip, _i, _i, _i, _ip]),
+    "bgt_synth_arrivals": (_i, [ctypes.c_uint64, _i, ctypes.c_double,
+                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+}
+
+# entry points added by later subsystems register their signatures here
+EXTRA_SIGNATURES = {}
+
+
+def lib():
+    """The loaded HIP library; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: the HIP library must be built first "
+                "(python -m begputhinning_amd.build); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in list(SIGNATURES.items()) + list(EXTRA_SIGNATURES.items()):
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != BGT_OK:
+        msg = lib().bgt_last_error()
+        raise BGTError(code, msg.decode() if msg else "")
+    return code

@@ -1,0 +1,89 @@
+"""Build the HIP library (libbgt.so, gfx950) in-tree with hipcc.
+
+    python -m begputhinning_amd.build          # library only
+    python -m begputhinning_amd.build --all    # library + oracle (+ reference build)
+
+The oracle and the reference build are test infrastructure (oracle/Makefile);
+the library never links them.
+"""
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libbgt.so")
+ARCH = os.environ.get("BGT_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+SOURCES = ["thin2d.hip", "thin3d.hip", "batch.hip", "capi.cpp", "segments.cpp", "synth.cpp",
+           "server.cpp"]
+
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
+          "-I" + os.path.join(ROOT, "include")]
+# packed-FP32 instructions are disabled for all device code (no float math on
+# this path, and the compiler must not form them)
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+
+def _compile(src):
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(OBJ, src + ".o")
+    if src.endswith(".hip"):
+        cmd = [HIPCC, "-c", path, "-o", obj] + COMMON + DEVICE
+    else:
+        # host-only translation units: hipcc as the host compiler, no device code
+        cmd = [HIPCC, "-c", "-x", "c++", path, "-o", obj, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+               "-ffp-contract=off"] + COMMON
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    out = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
+                    if "packed-fp32-ops" not in l)
+    return src, r.returncode, out, cmd
+
+
+def build_library(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")))
+    failed = []
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        for src, rc, out, cmd in ex.map(_compile, srcs):
+            if out.strip() and (verbose or rc):
+                print(f"[{src}]\n{out}", file=sys.stderr)
+            if rc:
+                failed.append(src)
+    if failed:
+        raise RuntimeError("hipcc failed for: " + ", ".join(failed))
+    objs = [os.path.join(OBJ, s + ".o") for s in srcs]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-fPIC"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    make = shutil.which("make")
+    if not make:
+        raise RuntimeError("make not found")
+    subprocess.run([make, "-s", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=True)
+    if os.path.isdir("/root/reference"):
+        subprocess.run([make, "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+
+
+def main(argv):
+    build_library(verbose="-v" in argv)
+    if "--all" in argv:
+        build_oracle()
+    print(LIB)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -1,0 +1,231 @@
+// capi.cpp -- the extern "C" entry points declared in include/bgt.h, error
+// reporting and the per-thread device workspace.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "segments.h"
+#include "thin2d.h"
+#include "thin3d.h"
+
+namespace bgt {
+
+static thread_local std::string g_error;
+static thread_local int g_error_code = BGT_OK;
+
+void set_error(int code, const std::string &msg)
+{
+    g_error_code = code;
+    g_error = msg;
+}
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    set_error(code, buf);
+    return code;
+}
+
+int workspace_reserve(Workspace &ws, size_t bytes)
+{
+    int dev = 0;
+    BGT_HIP(hipGetDevice(&dev));
+    if (ws.ptr && ws.bytes >= bytes && ws.device == dev) return BGT_OK;
+    if (ws.ptr) {
+        int cur = dev;
+        if (ws.device != dev) BGT_HIP(hipSetDevice(ws.device));
+        (void)hipFree(ws.ptr);
+        if (ws.device != dev) BGT_HIP(hipSetDevice(cur));
+        ws.ptr = nullptr;
+        ws.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4 + 4096;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(BGT_ERR_NOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    ws.ptr = p;
+    ws.bytes = want;
+    ws.device = dev;
+    return BGT_OK;
+}
+
+Workspace &thread_workspace(int slot)
+{
+    thread_local Workspace ws[4];
+    return ws[slot & 3];
+}
+
+static Config2D &config2d_storage()
+{
+    static Config2D c = [] {
+        Config2D v;
+        if (const char *e = std::getenv("BGT_2D_WAVES")) v.waves = std::atoi(e);
+        if (const char *e = std::getenv("BGT_2D_ROWS")) v.rows = std::atoi(e);
+        if (const char *e = std::getenv("BGT_2D_K")) v.iters_per_pass = std::atoi(e);
+        return v;
+    }();
+    return c;
+}
+
+const Config2D &default_config2d() { return config2d_storage(); }
+
+static Config3D &config3d_storage()
+{
+    static Config3D c = [] {
+        Config3D v;
+        if (const char *e = std::getenv("BGT_3D_WAVES")) v.waves = std::atoi(e);
+        if (const char *e = std::getenv("BGT_3D_K")) v.iters_per_pass = std::atoi(e);
+        return v;
+    }();
+    return c;
+}
+
+const Config3D &default_config3d() { return config3d_storage(); }
+
+}  // namespace bgt
+
+using namespace bgt;
+
+extern "C" {
+
+const char *bgt_last_error(void) { return g_error.c_str(); }
+
+int bgt_version(void) { return 1; }
+
+int bgt_set_config2d(int waves, int rows, int iters_per_pass)
+{
+    Config2D c;
+    c.waves = waves;
+    c.rows = rows;
+    c.iters_per_pass = iters_per_pass;
+    BGT_CHECK(check_config2d(c));
+    config2d_storage() = c;
+    return BGT_OK;
+}
+
+int bgt_set_config3d(int waves, int iters_per_pass)
+{
+    if (waves < 1 || waves > 8 || iters_per_pass < 1 || iters_per_pass > 5)
+        return fail(BGT_ERR_INVALID, "bad 3D config (waves=%d k=%d)", waves, iters_per_pass);
+    Config3D c;
+    c.waves = waves;
+    c.iters_per_pass = iters_per_pass;
+    config3d_storage() = c;
+    return BGT_OK;
+}
+
+int bgt_thinning2d(int dtype, int width, int height, const void *in, void *out)
+{
+    return thin2d_host(dtype, width, height, in, out, default_config2d());
+}
+
+int bgt_thin2d_device(const uint8_t *d_in, size_t in_pitch, int width, int height, uint8_t *d_out,
+                      size_t out_pitch, void *stream, int *iterations)
+{
+    return thin2d_device_u8(d_in, in_pitch, width, height, d_out, out_pitch,
+                            (hipStream_t)stream, iterations, default_config2d());
+}
+
+int bgt_thinning3d(int dtype, int width, int height, int depth, const void *in, void *out)
+{
+    return thin3d_host(dtype, width, height, depth, in, out, default_config3d());
+}
+
+int bgt_thin3d_device(const uint8_t *d_in, int width, int height, int depth, uint8_t *d_out,
+                      void *stream, int *iterations)
+{
+    return thin3d_device_u8(d_in, width, height, depth, d_out, (hipStream_t)stream, iterations,
+                            default_config3d());
+}
+
+int bgt_extract_segments2d(int dtype, const void *img, int width, int height, bgt_segments *out)
+{
+    return extract_segments(dtype, img, width, height, 1, false, out);
+}
+
+int bgt_extract_segments3d(int dtype, const void *vol, int width, int height, int depth,
+                           bgt_segments *out)
+{
+    return extract_segments(dtype, vol, width, height, depth, true, out);
+}
+
+void bgt_segments_free(bgt_segments *s)
+{
+    if (!s) return;
+    std::free(s->x); std::free(s->y); std::free(s->z);
+    std::free(s->seg_start); std::free(s->is_end);
+    std::memset(s, 0, sizeof(*s));
+}
+
+void bgt_free(void *p) { std::free(p); }
+
+// 2D: pad to pw = ceil32(w), ph = max(1, (h+63)/128)*128 + 64
+// (2D/CenterlineExtraction.cpp:27-28); thinning does not depend on the zero
+// padding, tracing does (it skips the padded frame's border).
+int bgt_centerline2d(int dtype, const void *img, int width, int height, int max_segment_length,
+                     int smoothing_span, float **out_x, float **out_y, uint32_t **out_idx,
+                     uint32_t *nsegs)
+{
+    if (!img || !out_x || !out_y || !out_idx || !nsegs) return fail(BGT_ERR_INVALID, "null pointer");
+    if (width <= 0 || height <= 0) return fail(BGT_ERR_INVALID, "bad size");
+    const int pw = (width + 31) / 32 * 32;
+    const int q = (height + 63) / 128;
+    const int ph = (q > 1 ? q : 1) * 128 + 64;
+    if (pw > 32767 || ph > 32767) return fail(BGT_ERR_UNSUPPORTED, "padded size exceeds int16");
+    std::vector<uint8_t> skel((size_t)width * height);
+    BGT_CHECK(thin2d_host_mask(dtype, width, height, img, skel.data(), default_config2d()));
+    std::vector<uint8_t> frame((size_t)pw * ph, 0);
+    for (int y = 0; y < height; ++y)
+        std::memcpy(frame.data() + (size_t)y * pw, skel.data() + (size_t)y * width, width);
+    bgt_segments s;
+    BGT_CHECK(extract_segments_mask(frame.data(), pw, ph, 1, false, &s));
+    int r = centerline_from_segments(s, false, max_segment_length, smoothing_span, out_x, out_y,
+                                     nullptr, out_idx, nsegs);
+    bgt_segments_free(&s);
+    return r;
+}
+
+// 3D: pad each side to a multiple of 32 with zeros (3D/CenterlineExtraction.cpp:27-47,
+// whose padding the reference leaves uninitialised), foreground iff int(v) > 0.
+int bgt_centerline3d(int dtype, const void *vol, int width, int height, int depth,
+                     int max_segment_length, int smoothing_span, float **out_x, float **out_y,
+                     float **out_z, uint32_t **out_idx, uint32_t *nsegs)
+{
+    if (!vol || !out_x || !out_y || !out_z || !out_idx || !nsegs)
+        return fail(BGT_ERR_INVALID, "null pointer");
+    if (width <= 0 || height <= 0 || depth <= 0) return fail(BGT_ERR_INVALID, "bad size");
+    const int pw = (width + 31) / 32 * 32, ph = (height + 31) / 32 * 32, pd = (depth + 31) / 32 * 32;
+    if (pw > 32767 || ph > 32767 || pd > 32767)
+        return fail(BGT_ERR_UNSUPPORTED, "padded size exceeds int16");
+    const size_t n = (size_t)width * height * depth;
+    std::vector<uint8_t> mask(n), skel(n);
+    BGT_CHECK(threshold_int_positive(dtype, vol, n, mask.data()));
+    BGT_CHECK(thin3d_host(BGT_U8, width, height, depth, mask.data(), skel.data(),
+                          default_config3d()));
+    std::vector<uint8_t> frame((size_t)pw * ph * pd, 0);
+    for (int z = 0; z < depth; ++z)
+        for (int y = 0; y < height; ++y)
+            std::memcpy(frame.data() + ((size_t)z * ph + y) * pw,
+                        skel.data() + ((size_t)z * height + y) * width, width);
+    bgt_segments s;
+    BGT_CHECK(extract_segments_mask(frame.data(), pw, ph, pd, true, &s));
+    int r = centerline_from_segments(s, true, max_segment_length, smoothing_span, out_x, out_y,
+                                     out_z, out_idx, nsegs);
+    bgt_segments_free(&s);
+    return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// common.h -- shared host/device definitions of the MI355X thinning library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/bgt.h"
+
+namespace bgt {
+
+// ---- errors ----------------------------------------------------------------
+void set_error(int code, const std::string &msg);
+int fail(int code, const char *fmt, ...);
+
+#define BGT_HIP(expr)                                                                \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return ::bgt::fail(BGT_ERR_HIP, "%s failed: %s (%s:%d)", #expr,         \
+                               hipGetErrorString(e_), __FILE__, __LINE__);           \
+    } while (0)
+
+#define BGT_CHECK(expr)                                                              \
+    do {                                                                             \
+        int s_ = (expr);                                                             \
+        if (s_ != BGT_OK) return s_;                                                 \
+    } while (0)
+
+// ---- device memory -------------------------------------------------------
+// Grow-only per-thread workspace used by the host-pointer drop-in entry points
+// so that repeated calls do not hipMalloc/hipFree every time.
+struct Workspace {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int device = -1;
+};
+int workspace_reserve(Workspace &ws, size_t bytes);
+Workspace &thread_workspace(int slot);
+
+// ---- pixel types ---------------------------------------------------------
+inline size_t dtype_size(int dt)
+{
+    switch (dt) {
+    case BGT_U8: case BGT_I8: case BGT_BOOL: return 1;
+    case BGT_U16: case BGT_I16: return 2;
+    case BGT_U32: case BGT_I32: case BGT_F32: return 4;
+    case BGT_F64: return 8;
+    default: return 0;
+    }
+}
+
+inline size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+inline int pow2_ceil(int v)
+{
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+}  // namespace bgt

@@ -1,0 +1,46 @@
+// thin2d.h -- host-side interface of the 2D thinning engine (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace bgt {
+
+struct Item2D;
+struct Tile2D;
+
+// Launch shape of the fused 2D pass kernel.
+struct Config2D {
+    int waves = 4;           // wave64s per workgroup
+    int rows = 16;           // register rows per lane (8, 16 or 32)
+    int iters_per_pass = 2;  // K: iterations fused per HBM pass (halo 4K rows)
+};
+
+struct Plan2D {
+    int w = 0, h = 0, L = 1, lgL = 0;
+    int band_rows = 0, nbands = 0;
+    size_t state_words = 0;
+};
+
+int plan2d(int w, int h, const Config2D &cfg, Plan2D &pl);
+int check_config2d(const Config2D &cfg);
+int band_rows_for(int L, const Config2D &cfg);
+size_t workspace_bytes2d(const Plan2D &pl);
+int setup_item2d(const Plan2D &pl, void *ws, hipStream_t s, Item2D **d_item, unsigned **d_ctr,
+                 Item2D *host_item);
+int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned *d_ctr,
+                 const int *d_state, hipStream_t s, int *iterations);
+
+int thin2d_device_u8(const uint8_t *d_in, size_t in_pitch, int w, int h, uint8_t *d_out,
+                     size_t out_pitch, hipStream_t s, int *iterations, const Config2D &cfg);
+int thin2d_host(int dtype, int w, int h, const void *in, void *out, const Config2D &cfg);
+
+const Config2D &default_config2d();
+
+}  // namespace bgt
+
+namespace bgt {
+int thin2d_host_mask(int dtype, int w, int h, const void *in, uint8_t *out, const Config2D &cfg);
+}

@@ -1,0 +1,537 @@
+// thin2d.hip -- 2D thinning on gfx950: pack/unpack kernels, the fused pass
+// kernel and the host drivers (per-image drop-in and device-resident).
+//
+// Reference path replaced: 2D/LibThinning2D.cu:39-130 (host driver),
+// 2D/CUDA_ThinningKernelMultiBlock2D.cuh:68-170 (driver kernel),
+// 2D/CUDA_SubiterationKernels2D.cuh (rules), 2D/CUDA_DeviceHelperFunctions.cuh
+// (pack/unpack, lane shifts, reductions).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
+#include "common.h"
+#include "thin2d.h"
+#include "thin2d_kernels.h"
+
+namespace bgt {
+
+// ---------------------------------------------------------------------------
+// pixel thresholds of the reference
+// ---------------------------------------------------------------------------
+
+// 2D rule: (unsigned)v > 0 with the GPU's saturating float->unsigned
+// conversion (2D/CUDA_DeviceHelperFunctions.cuh:101): integers v != 0,
+// floating point v >= 1 (NaN and negatives are background).
+template <typename T>
+__device__ __forceinline__ bool fg2d(T v)
+{
+    return v != 0;
+}
+template <>
+__device__ __forceinline__ bool fg2d<float>(float v)
+{
+    return v >= 1.0f;
+}
+template <>
+__device__ __forceinline__ bool fg2d<double>(double v)
+{
+    return v >= 1.0;
+}
+
+template <typename T>
+__device__ __forceinline__ T one_value()
+{
+    return (T)1;
+}
+
+// 4 packed bytes -> 4 bits (bit k set iff byte k is nonzero)
+__device__ __forceinline__ uint32_t nz4(uint32_t x)
+{
+    uint32_t t = x | (x >> 4);
+    t |= t >> 2;
+    t |= t >> 1;
+    t &= 0x01010101u;
+    return (t | (t >> 7) | (t >> 14) | (t >> 21)) & 0xfu;
+}
+// 4 bits -> 4 bytes of 0/1
+__device__ __forceinline__ uint32_t spread4(uint32_t b)
+{
+    return ((b & 0xfu) * 0x00204081u) & 0x01010101u;
+}
+
+// ---------------------------------------------------------------------------
+// pack / unpack
+// ---------------------------------------------------------------------------
+
+// uint8 rows (pitch bytes) -> packed words; one thread per 64-bit word
+__global__ void pack2d_u8(const uint8_t *__restrict__ in, size_t pitch, int w, int h, int L,
+                          uint64_t *__restrict__ out)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)L * h) return;
+    const int y = (int)(idx / L), c = (int)(idx % L);
+    const uint8_t *row = in + (size_t)y * pitch;
+    const int x0 = c * 64;
+    uint32_t lo = 0, hi = 0;
+    if (x0 + 64 <= w && ((((uintptr_t)(row + x0)) & 15) == 0)) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(row + x0);
+        uint4 a = p[0], b = p[1], cc = p[2], d = p[3];
+        lo = nz4(a.x) | nz4(a.y) << 4 | nz4(a.z) << 8 | nz4(a.w) << 12 | nz4(b.x) << 16 |
+             nz4(b.y) << 20 | nz4(b.z) << 24 | nz4(b.w) << 28;
+        hi = nz4(cc.x) | nz4(cc.y) << 4 | nz4(cc.z) << 8 | nz4(cc.w) << 12 | nz4(d.x) << 16 |
+             nz4(d.y) << 20 | nz4(d.z) << 24 | nz4(d.w) << 28;
+    } else {
+        for (int k = 0; k < 64; ++k) {
+            int x = x0 + k;
+            if (x < w && row[x]) {
+                if (k < 32) lo |= 1u << k;
+                else hi |= 1u << (k - 32);
+            }
+        }
+    }
+    out[idx] = ((uint64_t)hi << 32) | lo;
+}
+
+template <typename T>
+__global__ void pack2d_t(const T *__restrict__ in, int w, int h, int L, uint64_t *__restrict__ out)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)L * h) return;
+    const int y = (int)(idx / L), c = (int)(idx % L);
+    const T *row = in + (size_t)y * w;
+    uint64_t v = 0;
+    for (int k = 0; k < 64; ++k) {
+        int x = c * 64 + k;
+        if (x < w && fg2d<T>(row[x])) v |= 1ull << k;
+    }
+    out[idx] = v;
+}
+
+// packed words (from the buffer the item's final parity selects) -> uint8 0/1
+__global__ void unpack2d_u8(const Item2D *__restrict__ item, uint8_t *__restrict__ out,
+                            size_t pitch)
+{
+    const Item2D it = *item;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)it.L * it.h) return;
+    const int y = (int)(idx / it.L), c = (int)(idx % it.L);
+    const int x0 = c * 64;
+    if (x0 >= it.w) return;
+    const uint64_t v = (it.st[ST_PARITY] ? it.buf[1] : it.buf[0])[idx];
+    uint8_t *row = out + (size_t)y * pitch;
+    if (x0 + 64 <= it.w && ((((uintptr_t)(row + x0)) & 15) == 0)) {
+        uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+        uint4 *p = reinterpret_cast<uint4 *>(row + x0);
+        p[0] = make_uint4(spread4(lo), spread4(lo >> 4), spread4(lo >> 8), spread4(lo >> 12));
+        p[1] = make_uint4(spread4(lo >> 16), spread4(lo >> 20), spread4(lo >> 24), spread4(lo >> 28));
+        p[2] = make_uint4(spread4(hi), spread4(hi >> 4), spread4(hi >> 8), spread4(hi >> 12));
+        p[3] = make_uint4(spread4(hi >> 16), spread4(hi >> 20), spread4(hi >> 24), spread4(hi >> 28));
+    } else {
+        for (int k = 0; k < 64 && x0 + k < it.w; ++k) row[x0 + k] = (uint8_t)((v >> k) & 1);
+    }
+}
+
+template <typename T>
+__global__ void unpack2d_t(const Item2D *__restrict__ item, T *__restrict__ out)
+{
+    const Item2D it = *item;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)it.L * it.h) return;
+    const int y = (int)(idx / it.L), c = (int)(idx % it.L);
+    const uint64_t v = (it.st[ST_PARITY] ? it.buf[1] : it.buf[0])[idx];
+    T *row = out + (size_t)y * it.w;
+    for (int k = 0; k < 64; ++k) {
+        int x = c * 64 + k;
+        if (x < it.w) row[x] = ((v >> k) & 1) ? one_value<T>() : (T)0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// item set-up (kernel arguments only: no host staging, no memset node)
+// ---------------------------------------------------------------------------
+__global__ void init_item2d(Item2D *item, Item2D val, unsigned *ctr)
+{
+    const int t = threadIdx.x;
+    if (t == 0) *item = val;
+    if (t < ST_INTS) val.st[t] = 0;
+    if (t == 0 && ctr) *ctr = 0;
+}
+
+// ---------------------------------------------------------------------------
+// the fused pass kernel
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void finalize_item2d(const Item2D &it, int K)
+{
+    int *st = it.st;
+    if (st[ST_DONE]) return;
+    const int p = st[ST_PASS];
+    const unsigned mask = __hip_atomic_load((unsigned *)&st[ST_MASK], __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+    const int m = __popc(mask);  // iterations of this pass that deleted something
+    if (m < K) {
+        st[ST_ITERS] = p * K + m + 1;
+        st[ST_PARITY] = (p + 1) & 1;
+        st[ST_DONE] = 1;
+    } else {
+        st[ST_PASS] = p + 1;
+    }
+    st[ST_MASK] = 0;
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void thin2d_pass(const Item2D *__restrict__ items,
+                                                    const Tile2D *__restrict__ tiles, int nitems,
+                                                    int K, unsigned *__restrict__ ctr)
+{
+    extern __shared__ uint64_t edges[];
+    __shared__ unsigned s_mask;
+    __shared__ int s_last;
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int NW = blockDim.x >> 6;
+    const Tile2D tl = tiles ? tiles[blockIdx.x] : Tile2D{0, (int)blockIdx.x};
+    const Item2D it = items[tl.item];
+    const int band = tl.band;
+
+    if (tid == 0) s_mask = 0;
+    bool work = !it.st[ST_DONE];
+    int p = 0, par = 0;
+    if (work) {
+        p = it.st[ST_PASS];
+        par = p & 1;
+        if (p > 0) {
+            const uint8_t *c = it.chg + (par ^ 1) * it.nbands;
+            bool act = c[band] || (band > 0 && c[band - 1]) ||
+                       (band + 1 < it.nbands && c[band + 1]);
+            if (!act) {
+                if (tid == 0) it.chg[par * it.nbands + band] = 0;
+                work = false;
+            }
+        }
+    }
+
+    if (work) {
+        const int L = it.L, lgL = it.lgL, G = 64 >> lgL, S = NW * G, SL = NW * 64;
+        const int g = lane >> lgL, c = lane & (L - 1), q = wv * G + g;
+        const int H = 4 * K;
+        const int y_reg0 = band * it.band_rows - H + q * R;  // image row of register row 0
+        const uint32_t mw = c == 0 ? 0u : 0xffffffffu;
+        const uint32_t me = c == L - 1 ? 0u : 0xffffffffu;
+        // rows of this segment inside the band interior, as a bit mask
+        const int ra = max(0, H - q * R), rb = min(R, H + it.band_rows - q * R);
+        const uint32_t interior = rb > ra ? (uint32_t)(((1ull << rb) - 1ull) & ~((1ull << ra) - 1ull)) : 0u;
+        const uint64_t *src = par ? it.buf[1] : it.buf[0];
+        uint64_t *dst = par ? it.buf[0] : it.buf[1];
+
+        uint32_t lo[R], hi[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int y = y_reg0 + r;
+            uint64_t v = 0;
+            if (y >= 0 && y < it.h) v = src[(size_t)y * L + c];
+            lo[r] = (uint32_t)v;
+            hi[r] = (uint32_t)(v >> 32);
+        }
+        // edges written after subiteration s live in buffer (s + 1) & 1;
+        // buffer 0 holds the initial rows.  Layout [buf][top/bot][S*L]; the
+        // entry of lane (g, c) of wave v is v*64 + lane.
+        const int me_idx = tid;
+        const int up_idx = q > 0 ? me_idx - L : me_idx;
+        const int dn_idx = q + 1 < S ? me_idx + L : me_idx;
+        const uint32_t up_ok = q > 0 ? 0xffffffffu : 0u;
+        const uint32_t dn_ok = q + 1 < S ? 0xffffffffu : 0u;
+        edges[(0 * 2 + 0) * SL + me_idx] = ((uint64_t)hi[0] << 32) | lo[0];
+        edges[(0 * 2 + 1) * SL + me_idx] = ((uint64_t)hi[R - 1] << 32) | lo[R - 1];
+        __syncthreads();
+
+        unsigned bits = 0;
+        int s = 0;
+#pragma unroll 1
+        for (int itn = 0; itn < K; ++itn) {
+            uint32_t rows_changed = 0;
+#define BGT_SUB(D)                                                                           \
+    {                                                                                        \
+        const int rb_ = s & 1, wb_ = (s + 1) & 1;                                            \
+        const uint64_t ue = edges[(rb_ * 2 + 1) * SL + up_idx];                              \
+        const uint64_t de = edges[(rb_ * 2 + 0) * SL + dn_idx];                              \
+        subiteration<D, R>(lo, hi, W64{(uint32_t)ue & up_ok, (uint32_t)(ue >> 32) & up_ok},  \
+                           W64{(uint32_t)de & dn_ok, (uint32_t)(de >> 32) & dn_ok}, mw, me,  \
+                           rows_changed);                                                    \
+        edges[(wb_ * 2 + 0) * SL + me_idx] = ((uint64_t)hi[0] << 32) | lo[0];                \
+        edges[(wb_ * 2 + 1) * SL + me_idx] = ((uint64_t)hi[R - 1] << 32) | lo[R - 1];        \
+        __syncthreads();                                                                     \
+        ++s;                                                                                 \
+    }
+            BGT_SUB(0) BGT_SUB(1) BGT_SUB(2) BGT_SUB(3)
+#undef BGT_SUB
+            if (rows_changed & interior) bits |= 1u << itn;
+        }
+
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int y = y_reg0 + r;
+            if (((interior >> r) & 1u) && y >= 0 && y < it.h)
+                dst[(size_t)y * L + c] = ((uint64_t)hi[r] << 32) | lo[r];
+        }
+        if (bits) atomicOr(&s_mask, bits);
+        __syncthreads();
+        if (tid == 0) {
+            it.chg[par * it.nbands + band] = s_mask != 0;
+            if (s_mask) atomicOr((unsigned *)&it.st[ST_MASK], s_mask);
+        }
+    }
+
+    // the last workgroup of the launch advances every item's pass state
+    if (tid == 0) {
+        __threadfence();
+        s_last = atomicAdd(ctr, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        for (int i = tid; i < nitems; i += blockDim.x) finalize_item2d(items[i], K);
+        __threadfence();
+        if (tid == 0) atomicExch(ctr, 0u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+int band_rows_for(int L, const Config2D &cfg)
+{
+    const int G = 64 / L;
+    return cfg.waves * G * cfg.rows - 8 * cfg.iters_per_pass;
+}
+
+int check_config2d(const Config2D &cfg)
+{
+    if (cfg.waves < 1 || cfg.waves > 4 || (cfg.rows != 8 && cfg.rows != 16 && cfg.rows != 32) ||
+        cfg.iters_per_pass < 1 || cfg.iters_per_pass > 16)
+        return fail(BGT_ERR_INVALID, "bad 2D config (waves=%d rows=%d k=%d)", cfg.waves, cfg.rows,
+                    cfg.iters_per_pass);
+    return BGT_OK;
+}
+
+static int launch_pass2d(const Config2D &cfg, int nblocks, const Item2D *items, const Tile2D *tiles,
+                         int nitems, unsigned *ctr, hipStream_t s)
+{
+    const size_t lds = (size_t)2 * 2 * cfg.waves * 64 * sizeof(uint64_t);
+    dim3 blk(cfg.waves * 64);
+    switch (cfg.rows) {
+    case 8:
+        hipLaunchKernelGGL(thin2d_pass<8>, dim3(nblocks), blk, lds, s, items, tiles, nitems,
+                           cfg.iters_per_pass, ctr);
+        break;
+    case 16:
+        hipLaunchKernelGGL(thin2d_pass<16>, dim3(nblocks), blk, lds, s, items, tiles, nitems,
+                           cfg.iters_per_pass, ctr);
+        break;
+    default:
+        hipLaunchKernelGGL(thin2d_pass<32>, dim3(nblocks), blk, lds, s, items, tiles, nitems,
+                           cfg.iters_per_pass, ctr);
+        break;
+    }
+    BGT_HIP(hipGetLastError());
+    return BGT_OK;
+}
+
+int plan2d(int w, int h, const Config2D &cfg, Plan2D &pl)
+{
+    if (w <= 0 || h <= 0) return fail(BGT_ERR_INVALID, "2D size must be >= 1 (got %d x %d)", w, h);
+    if (w > 64 * 64)
+        return fail(BGT_ERR_UNSUPPORTED, "2D width %d exceeds 4096 (one wave64 of 64-bit words)", w);
+    BGT_CHECK(check_config2d(cfg));
+    pl.w = w;
+    pl.h = h;
+    pl.L = pow2_ceil((w + 63) / 64);
+    pl.lgL = 0;
+    while ((1 << pl.lgL) < pl.L) pl.lgL++;
+    pl.band_rows = band_rows_for(pl.L, cfg);
+    if (pl.band_rows < 4 * cfg.iters_per_pass)
+        return fail(BGT_ERR_INVALID, "2D band of %d rows is thinner than its halo %d",
+                    pl.band_rows, 4 * cfg.iters_per_pass);
+    pl.nbands = (h + pl.band_rows - 1) / pl.band_rows;
+    pl.state_words = (size_t)pl.L * h;
+    return BGT_OK;
+}
+
+size_t workspace_bytes2d(const Plan2D &pl)
+{
+    size_t b = 0;
+    b += round_up(sizeof(Item2D), 256);
+    b += round_up(ST_INTS * sizeof(int), 256);
+    b += round_up(sizeof(unsigned), 256);
+    b += 2 * round_up(pl.state_words * sizeof(uint64_t), 256);
+    b += round_up((size_t)2 * pl.nbands, 256);
+    return b;
+}
+
+struct Carve {
+    char *p;
+    template <typename T>
+    T *take(size_t bytes)
+    {
+        T *r = reinterpret_cast<T *>(p);
+        p += round_up(bytes, 256);
+        return r;
+    }
+};
+
+int setup_item2d(const Plan2D &pl, void *ws, hipStream_t s, Item2D **d_item, unsigned **d_ctr,
+                 Item2D *host_item)
+{
+    Carve cv{reinterpret_cast<char *>(ws)};
+    Item2D *item = cv.take<Item2D>(sizeof(Item2D));
+    int *st = cv.take<int>(ST_INTS * sizeof(int));
+    unsigned *ctr = cv.take<unsigned>(sizeof(unsigned));
+    Item2D v{};
+    v.buf[0] = cv.take<uint64_t>(pl.state_words * sizeof(uint64_t));
+    v.buf[1] = cv.take<uint64_t>(pl.state_words * sizeof(uint64_t));
+    v.chg = cv.take<uint8_t>((size_t)2 * pl.nbands);
+    v.st = st;
+    v.w = pl.w; v.h = pl.h; v.L = pl.L; v.lgL = pl.lgL;
+    v.nbands = pl.nbands;
+    v.band_rows = pl.band_rows;
+    hipLaunchKernelGGL(init_item2d, dim3(1), dim3(64), 0, s, item, v, ctr);
+    BGT_HIP(hipGetLastError());
+    *d_item = item;
+    *d_ctr = ctr;
+    if (host_item) *host_item = v;
+    return BGT_OK;
+}
+
+// Per-thread pinned word for the convergence poll.
+static int *poll_buffer()
+{
+    thread_local int *p = nullptr;
+    if (!p) {
+        if (hipHostMalloc((void **)&p, 64, hipHostMallocDefault) != hipSuccess) p = nullptr;
+    }
+    return p;
+}
+
+int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned *d_ctr,
+                 const int *d_state, hipStream_t s, int *iterations)
+{
+    int *hp = poll_buffer();
+    if (!hp) return fail(BGT_ERR_NOMEM, "pinned poll buffer allocation failed");
+    const long max_pass = (long)pl.w * pl.h / cfg.iters_per_pass + 2;  // each changing iteration deletes >= 1 px
+    long launched = 0;
+    int chunk = 2;
+    for (;;) {
+        for (int i = 0; i < chunk; ++i)
+            BGT_CHECK(launch_pass2d(cfg, pl.nbands, d_item, nullptr, 1, d_ctr, s));
+        launched += chunk;
+        BGT_HIP(hipMemcpyAsync(hp, d_state, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
+        BGT_HIP(hipStreamSynchronize(s));
+        if (hp[ST_DONE]) break;
+        if (launched > max_pass) return fail(BGT_ERR_NOT_CONVERGED, "2D thinning did not converge");
+        chunk = std::min(chunk * 2, 16);
+    }
+    if (iterations) *iterations = hp[ST_ITERS];
+    return BGT_OK;
+}
+
+int thin2d_device_u8(const uint8_t *d_in, size_t in_pitch, int w, int h, uint8_t *d_out,
+                     size_t out_pitch, hipStream_t s, int *iterations, const Config2D &cfg)
+{
+    if (!d_in || !d_out) return fail(BGT_ERR_INVALID, "null device pointer");
+    if (in_pitch < (size_t)w || out_pitch < (size_t)w)
+        return fail(BGT_ERR_INVALID, "pitch smaller than width");
+    Plan2D pl;
+    BGT_CHECK(plan2d(w, h, cfg, pl));
+    Workspace &ws = thread_workspace(0);
+    BGT_CHECK(workspace_reserve(ws, workspace_bytes2d(pl)));
+    Item2D *d_item;
+    unsigned *d_ctr;
+    Item2D hv;
+    BGT_CHECK(setup_item2d(pl, ws.ptr, s, &d_item, &d_ctr, &hv));
+    const long nw = (long)pl.L * h;
+    hipLaunchKernelGGL(pack2d_u8, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, d_in,
+                       in_pitch, w, h, pl.L, hv.buf[0]);
+    BGT_HIP(hipGetLastError());
+    BGT_CHECK(run_passes2d(cfg, pl, d_item, d_ctr, hv.st, s, iterations));
+    hipLaunchKernelGGL(unpack2d_u8, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, d_item,
+                       d_out, out_pitch);
+    BGT_HIP(hipGetLastError());
+    BGT_HIP(hipStreamSynchronize(s));
+    return BGT_OK;
+}
+
+// OutT is T (drop-in Thinning2D) or uint8_t (0/1 mask for the tracer)
+template <typename T, typename OutT = T>
+static int thin2d_host_t(int w, int h, const T *in, OutT *out, const Config2D &cfg)
+{
+    Plan2D pl;
+    BGT_CHECK(plan2d(w, h, cfg, pl));
+    const size_t img_bytes = (size_t)w * h * sizeof(T);
+    Workspace &ws = thread_workspace(0);
+    Workspace &io = thread_workspace(1);
+    BGT_CHECK(workspace_reserve(ws, workspace_bytes2d(pl)));
+    BGT_CHECK(workspace_reserve(io, img_bytes));
+    hipStream_t s = nullptr;
+    BGT_HIP(hipMemcpyAsync(io.ptr, in, img_bytes, hipMemcpyHostToDevice, s));
+    Item2D *d_item;
+    unsigned *d_ctr;
+    Item2D hv;
+    BGT_CHECK(setup_item2d(pl, ws.ptr, s, &d_item, &d_ctr, &hv));
+    const long nw = (long)pl.L * h;
+    const dim3 g((unsigned)((nw + 255) / 256)), b(256);
+    if (sizeof(T) == 1 && std::is_same<T, uint8_t>::value)
+        hipLaunchKernelGGL(pack2d_u8, g, b, 0, s, (const uint8_t *)io.ptr, (size_t)w, w, h, pl.L,
+                           hv.buf[0]);
+    else
+        hipLaunchKernelGGL(pack2d_t<T>, g, b, 0, s, (const T *)io.ptr, w, h, pl.L, hv.buf[0]);
+    BGT_HIP(hipGetLastError());
+    BGT_CHECK(run_passes2d(cfg, pl, d_item, d_ctr, hv.st, s, nullptr));
+    if (std::is_same<OutT, uint8_t>::value)
+        hipLaunchKernelGGL(unpack2d_u8, g, b, 0, s, d_item, (uint8_t *)io.ptr, (size_t)w);
+    else
+        hipLaunchKernelGGL(unpack2d_t<OutT>, g, b, 0, s, d_item, (OutT *)io.ptr);
+    BGT_HIP(hipGetLastError());
+    BGT_HIP(hipMemcpyAsync(out, io.ptr, (size_t)w * h * sizeof(OutT), hipMemcpyDeviceToHost, s));
+    BGT_HIP(hipStreamSynchronize(s));
+    return BGT_OK;
+}
+
+int thin2d_host(int dtype, int w, int h, const void *in, void *out, const Config2D &cfg)
+{
+    if (!in || !out) return fail(BGT_ERR_INVALID, "null host pointer");
+    switch (dtype) {
+    case BGT_U8: return thin2d_host_t<uint8_t>(w, h, (const uint8_t *)in, (uint8_t *)out, cfg);
+    case BGT_I8: return thin2d_host_t<int8_t>(w, h, (const int8_t *)in, (int8_t *)out, cfg);
+    case BGT_U16: return thin2d_host_t<uint16_t>(w, h, (const uint16_t *)in, (uint16_t *)out, cfg);
+    case BGT_I16: return thin2d_host_t<int16_t>(w, h, (const int16_t *)in, (int16_t *)out, cfg);
+    case BGT_U32: return thin2d_host_t<uint32_t>(w, h, (const uint32_t *)in, (uint32_t *)out, cfg);
+    case BGT_I32: return thin2d_host_t<int32_t>(w, h, (const int32_t *)in, (int32_t *)out, cfg);
+    case BGT_F32: return thin2d_host_t<float>(w, h, (const float *)in, (float *)out, cfg);
+    case BGT_F64: return thin2d_host_t<double>(w, h, (const double *)in, (double *)out, cfg);
+    case BGT_BOOL: return thin2d_host_t<bool>(w, h, (const bool *)in, (bool *)out, cfg);
+    default: return fail(BGT_ERR_INVALID, "unknown pixel type %d", dtype);
+    }
+}
+
+int thin2d_host_mask(int dtype, int w, int h, const void *in, uint8_t *out, const Config2D &cfg)
+{
+    if (!in || !out) return fail(BGT_ERR_INVALID, "null host pointer");
+    switch (dtype) {
+    case BGT_U8: return thin2d_host_t<uint8_t, uint8_t>(w, h, (const uint8_t *)in, out, cfg);
+    case BGT_I8: return thin2d_host_t<int8_t, uint8_t>(w, h, (const int8_t *)in, out, cfg);
+    case BGT_U16: return thin2d_host_t<uint16_t, uint8_t>(w, h, (const uint16_t *)in, out, cfg);
+    case BGT_I16: return thin2d_host_t<int16_t, uint8_t>(w, h, (const int16_t *)in, out, cfg);
+    case BGT_U32: return thin2d_host_t<uint32_t, uint8_t>(w, h, (const uint32_t *)in, out, cfg);
+    case BGT_I32: return thin2d_host_t<int32_t, uint8_t>(w, h, (const int32_t *)in, out, cfg);
+    case BGT_F32: return thin2d_host_t<float, uint8_t>(w, h, (const float *)in, out, cfg);
+    case BGT_F64: return thin2d_host_t<double, uint8_t>(w, h, (const double *)in, out, cfg);
+    case BGT_BOOL: return thin2d_host_t<bool, uint8_t>(w, h, (const bool *)in, out, cfg);
+    default: return fail(BGT_ERR_INVALID, "unknown pixel type %d", dtype);
+    }
+}
+
+}  // namespace bgt

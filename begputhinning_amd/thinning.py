@@ -1,0 +1,181 @@
+"""Host-side mirror of the reference's C++ API for the thinning path.
+
+Names, argument meaning and output layout follow the reference
+(2D/LibThinning2D.h:13-15, 3D/LibThinning3D.h:13-15, 2D/ExtractSegments.h:16-17,
+3D/ExtractSegments.h:21-22, 2D/CenterlineExtraction.h:25-28,
+3D/CenterlineExctraction.h:26-29); arrays are numpy (host) or torch tensors on
+the GPU (device-resident variants).  Every call goes through the HIP C-ABI;
+errors raise BGTError instead of the reference's printf/exit.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import DTYPES, Segments, check, lib
+
+
+def _dtype_code(a):
+    name = np.dtype(a.dtype).name
+    if name not in DTYPES:
+        raise TypeError(f"unsupported pixel type {a.dtype} (reference instantiations: "
+                        f"{', '.join(DTYPES)})")
+    return DTYPES[name]
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def Thinning2D(data):
+    """Thinning2D<T>(width, height, data, out): returns the thinned image (same
+    dtype, 0/1) of a (height, width) array."""
+    a = np.ascontiguousarray(data)
+    if a.ndim != 2:
+        raise ValueError("Thinning2D expects a (height, width) array")
+    out = np.empty_like(a)
+    check(lib().bgt_thinning2d(_dtype_code(a), a.shape[1], a.shape[0], _ptr(a), _ptr(out)))
+    return out
+
+
+def Thinning3D(data):
+    """Thinning3D<T>(width, height, depth, data, out) of a (depth, height,
+    width) array."""
+    a = np.ascontiguousarray(data)
+    if a.ndim != 3:
+        raise ValueError("Thinning3D expects a (depth, height, width) array")
+    out = np.empty_like(a)
+    check(lib().bgt_thinning3d(_dtype_code(a), a.shape[2], a.shape[1], a.shape[0], _ptr(a),
+                               _ptr(out)))
+    return out
+
+
+class SSegment:
+    """One traced segment: points (n, 2) [x, y] or (n, 3) [x, y, z] int16 and
+    the is_end_segment flag (2D/Structs.h:27-34)."""
+    __slots__ = ("points", "is_end_segment", "pruned")
+
+    def __init__(self, points, is_end):
+        self.points = points
+        self.is_end_segment = bool(is_end)
+        self.pruned = False
+
+    def __repr__(self):
+        return f"SSegment(n={len(self.points)}, is_end={self.is_end_segment})"
+
+
+def _take_segments(s, three_d):
+    n, m = s.nseg, s.npts
+    x = np.ctypeslib.as_array(s.x, (m,)).copy() if m else np.zeros(0, np.int16)
+    y = np.ctypeslib.as_array(s.y, (m,)).copy() if m else np.zeros(0, np.int16)
+    z = (np.ctypeslib.as_array(s.z, (m,)).copy() if m else np.zeros(0, np.int16)) if three_d else None
+    st = np.ctypeslib.as_array(s.seg_start, (n + 1,)).copy()
+    ie = np.ctypeslib.as_array(s.is_end, (n,)).copy() if n else np.zeros(0, np.uint8)
+    lib().bgt_segments_free(ctypes.byref(s))
+    return {"x": x, "y": y, "z": z, "start": st, "is_end": ie}
+
+
+def extract_segments_flat(data):
+    """ExtractSegmentsV as flat arrays: x, y (, z), start (nseg+1), is_end."""
+    a = np.ascontiguousarray(data)
+    s = Segments()
+    if a.ndim == 2:
+        check(lib().bgt_extract_segments2d(_dtype_code(a), _ptr(a), a.shape[1], a.shape[0],
+                                           ctypes.byref(s)))
+        return _take_segments(s, False)
+    if a.ndim == 3:
+        check(lib().bgt_extract_segments3d(_dtype_code(a), _ptr(a), a.shape[2], a.shape[1],
+                                           a.shape[0], ctypes.byref(s)))
+        return _take_segments(s, True)
+    raise ValueError("ExtractSegmentsV expects a 2D or 3D array")
+
+
+def ExtractSegmentsV(data):
+    """ExtractSegmentsV<T>(data, width, height[, depth]) -> list of SSegment."""
+    f = extract_segments_flat(data)
+    cols = [f["x"], f["y"]] + ([f["z"]] if f["z"] is not None else [])
+    pts = np.stack(cols, axis=1) if len(f["x"]) else np.zeros((0, len(cols)), np.int16)
+    st = f["start"]
+    return [SSegment(pts[st[i]:st[i + 1]], f["is_end"][i]) for i in range(len(st) - 1)]
+
+
+def CenterlineExtraction(data, max_segment_length, smoothing_span):
+    """CenterlineExtraction<T>: returns (out_x, out_y, out_idx) for a 2D image
+    or (out_x, out_y, out_z, out_idx) for a volume; out_idx has nsegs + 1
+    entries with out_idx[nsegs] = number of points."""
+    a = np.ascontiguousarray(data)
+    px = ctypes.POINTER(ctypes.c_float)()
+    py = ctypes.POINTER(ctypes.c_float)()
+    pz = ctypes.POINTER(ctypes.c_float)()
+    pi = ctypes.POINTER(ctypes.c_uint32)()
+    ns = ctypes.c_uint32(0)
+    if a.ndim == 2:
+        check(lib().bgt_centerline2d(_dtype_code(a), _ptr(a), a.shape[1], a.shape[0],
+                                     int(max_segment_length), int(smoothing_span),
+                                     ctypes.byref(px), ctypes.byref(py), ctypes.byref(pi),
+                                     ctypes.byref(ns)))
+    elif a.ndim == 3:
+        check(lib().bgt_centerline3d(_dtype_code(a), _ptr(a), a.shape[2], a.shape[1], a.shape[0],
+                                     int(max_segment_length), int(smoothing_span),
+                                     ctypes.byref(px), ctypes.byref(py), ctypes.byref(pz),
+                                     ctypes.byref(pi), ctypes.byref(ns)))
+    else:
+        raise ValueError("CenterlineExtraction expects a 2D or 3D array")
+    n = ns.value
+    idx = np.ctypeslib.as_array(pi, (n + 1,)).copy()
+    m = int(idx[n])
+    take = lambda p: np.ctypeslib.as_array(p, (m,)).copy() if m else np.zeros(0, np.float32)
+    x, y = take(px), take(py)
+    z = take(pz) if a.ndim == 3 else None
+    for p in (px, py, pi) + ((pz,) if a.ndim == 3 else ()):
+        lib().bgt_free(ctypes.cast(p, ctypes.c_void_p))
+    return (x, y, idx) if a.ndim == 2 else (x, y, z, idx)
+
+
+# ---- device-resident variants (torch tensors on the GPU) --------------------
+
+def _stream_handle(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def thin2d_device(img, out=None, stream=None):
+    """Thin a (h, w) uint8 CUDA tensor; returns (out, iterations)."""
+    import torch
+    if img.dtype != torch.uint8 or img.dim() != 2 or not img.is_cuda:
+        raise TypeError("thin2d_device expects a 2D uint8 GPU tensor")
+    if img.stride(1) != 1:
+        raise ValueError("rows must be contiguous")
+    h, w = img.shape
+    if out is None:
+        out = torch.empty_like(img)
+    it = ctypes.c_int(0)
+    check(lib().bgt_thin2d_device(ctypes.c_void_p(img.data_ptr()), img.stride(0), w, h,
+                                  ctypes.c_void_p(out.data_ptr()), out.stride(0),
+                                  _stream_handle(stream), ctypes.byref(it)))
+    return out, it.value
+
+
+def thin3d_device(vol, out=None, stream=None):
+    """Thin a contiguous (d, h, w) uint8 CUDA tensor; returns (out, iterations)."""
+    import torch
+    if vol.dtype != torch.uint8 or vol.dim() != 3 or not vol.is_cuda or not vol.is_contiguous():
+        raise TypeError("thin3d_device expects a contiguous 3D uint8 GPU tensor")
+    d, h, w = vol.shape
+    if out is None:
+        out = torch.empty_like(vol)
+    it = ctypes.c_int(0)
+    check(lib().bgt_thin3d_device(ctypes.c_void_p(vol.data_ptr()), w, h, d,
+                                  ctypes.c_void_p(out.data_ptr()), _stream_handle(stream),
+                                  ctypes.byref(it)))
+    return out, it.value
+
+
+def set_config2d(waves=4, rows=16, iters_per_pass=2):
+    """Launch shape of the 2D pass kernel for later calls (see bgt.h)."""
+    check(lib().bgt_set_config2d(waves, rows, iters_per_pass))
+
+
+def set_config3d(waves=8, iters_per_pass=1):
+    """Launch shape of the 3D pass kernel for later calls (see bgt.h)."""
+    check(lib().bgt_set_config3d(waves, iters_per_pass))

@@ -1,0 +1,136 @@
+/*
+ * bgt.h -- C-ABI of the MI355X-native BEGPUThinning thinning path.
+ *
+ * Plain C: pointers, sizes and status codes; no HIP or torch types (streams are
+ * passed as `void *`, a hipStream_t; NULL means the default stream).  Every
+ * function returns BGT_OK (0) or a BGT_ERR_* code; bgt_last_error() gives the
+ * message of the calling thread's last failure.
+ *
+ * The drop-in C++ API of the reference (CenterlineExtraction.h,
+ * LibThinning2D.h, LibThinning3D.h, ExtractSegments.h) is provided header-only
+ * in include/2D and include/3D on top of these entry points.
+ */
+#ifndef BGT_H
+#define BGT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BGT_OK = 0,
+    BGT_ERR_INVALID = 1,       /* bad argument (size, pointer, type) */
+    BGT_ERR_HIP = 2,           /* HIP runtime error */
+    BGT_ERR_NOMEM = 3,         /* device or host allocation failed */
+    BGT_ERR_UNSUPPORTED = 4,   /* shape outside what this build handles */
+    BGT_ERR_NOT_CONVERGED = 5, /* iteration cap reached */
+    BGT_ERR_ADMISSION = 6      /* serving: request does not fit the memory budget */
+};
+
+/* Pixel types of the reference's template instantiations
+ * (2D/LibThinning2D.cu:134-143, 2D/CenterlineExtraction.cpp:146-155).  `char`
+ * is BGT_I8 on targets where char is signed. */
+enum {
+    BGT_U8 = 0, BGT_I8 = 1, BGT_U16 = 2, BGT_I16 = 3, BGT_U32 = 4,
+    BGT_I32 = 5, BGT_F32 = 6, BGT_F64 = 7, BGT_BOOL = 8
+};
+
+const char *bgt_last_error(void);
+int bgt_version(void);
+
+/* Launch shapes (no reference counterpart: the reference hard-codes them,
+ * 2D/LibThinning2D.cu:59-60, 3D/LibThinning3D.cu:50).  2D: waves per
+ * workgroup (1-4), register rows per lane (8 or 16), iterations fused per
+ * HBM pass (1-16).  3D: waves per workgroup (1-8), iterations per pass (1-5).
+ * Applies to later calls of the calling process. */
+int bgt_set_config2d(int waves, int rows, int iters_per_pass);
+int bgt_set_config3d(int waves, int iters_per_pass);
+
+/* ------------------------------------------------------------------------- */
+/* Drop-in thinning on host buffers.                                          */
+/* ------------------------------------------------------------------------- */
+
+/* Replaces template<class T> void Thinning2D(int width, int height, T *data,
+ * T *out) (2D/LibThinning2D.h:13-15, 2D/LibThinning2D.cu:39-130).
+ * Foreground iff (unsigned)v > 0 with saturating conversion of floats
+ * (2D/CUDA_DeviceHelperFunctions.cuh:101); out holds T(0) / T(1).  Any
+ * width/height >= 1 (the reference needs padded widths <= 2048). */
+int bgt_thinning2d(int dtype, int width, int height, const void *in, void *out);
+
+/* Replaces template<class T> void Thinning3D(int width, int height, int depth,
+ * T *data, T *out) (3D/LibThinning3D.h:13-15, 3D/LibThinning3D.cu:34-90).
+ * Foreground iff v > 0 in T (3D/CUDA_DeviceHelperFunctions.cuh:162).  Any
+ * dimensions >= 1 (the reference needs multiples of 32, >= 64). */
+int bgt_thinning3d(int dtype, int width, int height, int depth, const void *in, void *out);
+
+/* ------------------------------------------------------------------------- */
+/* Device-resident thinning (inputs already in HBM).  uint8 masks, nonzero =  */
+/* foreground; out receives 0/1.  *iterations (may be NULL) receives the      */
+/* number of iterations executed including the final no-change one.           */
+/* ------------------------------------------------------------------------- */
+int bgt_thin2d_device(const uint8_t *d_in, size_t in_pitch, int width, int height,
+                      uint8_t *d_out, size_t out_pitch, void *stream, int *iterations);
+int bgt_thin3d_device(const uint8_t *d_in, int width, int height, int depth, uint8_t *d_out,
+                      void *stream, int *iterations);
+
+/* ------------------------------------------------------------------------- */
+/* Segment tracing (host), ExtractSegmentsV.                                  */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int nseg;
+    int npts;
+    int16_t *x, *y, *z;   /* z is NULL for 2D */
+    uint32_t *seg_start;  /* nseg + 1 entries, seg_start[nseg] = npts */
+    uint8_t *is_end;      /* nseg entries */
+} bgt_segments;
+
+/* Replaces template<class T> std::vector<SSegment*> ExtractSegmentsV(T *data,
+ * int width, int height) (2D/ExtractSegments.h:16-17, 2D/ExtractSegments.cpp:16-127). */
+int bgt_extract_segments2d(int dtype, const void *img, int width, int height, bgt_segments *out);
+/* Replaces ExtractSegmentsV(T *data, int width, int height, int depth)
+ * (3D/ExtractSegments.h:21-22, 3D/ExtractSegments.cpp:135-251). */
+int bgt_extract_segments3d(int dtype, const void *vol, int width, int height, int depth,
+                           bgt_segments *out);
+void bgt_segments_free(bgt_segments *s);
+
+/* ------------------------------------------------------------------------- */
+/* Centerline extraction (pad, thin on the GPU, trace, prune, smooth,         */
+/* flatten).  Outputs are malloc'd by the library: release with bgt_free.     */
+/* ------------------------------------------------------------------------- */
+
+/* Replaces template<class T> unsigned CenterlineExtraction(T *data, int width,
+ * int height, int max_segment_length, int smoothing_span, float **out_x,
+ * float **out_y, unsigned **out_idx) (2D/CenterlineExtraction.h:25-28,
+ * 2D/CenterlineExtraction.cpp:22-142).  A negative smoothing_span is treated
+ * as 0 (the reference indexes out of bounds there). */
+int bgt_centerline2d(int dtype, const void *img, int width, int height, int max_segment_length,
+                     int smoothing_span, float **out_x, float **out_y, uint32_t **out_idx,
+                     uint32_t *nsegs);
+/* Replaces CenterlineExtraction(T *data, int width, int height, int depth, ...)
+ * (3D/CenterlineExctraction.h:26-29, 3D/CenterlineExtraction.cpp:22-155);
+ * foreground iff int(v) > 0 (3D/CenterlineExtraction.cpp:44). */
+int bgt_centerline3d(int dtype, const void *vol, int width, int height, int depth,
+                     int max_segment_length, int smoothing_span, float **out_x, float **out_y,
+                     float **out_z, uint32_t **out_idx, uint32_t *nsegs);
+void bgt_free(void *p);
+
+/* ------------------------------------------------------------------------- */
+/* Synthetic inputs (seeded, deterministic) for tests and benchmarks.         */
+/* ------------------------------------------------------------------------- */
+uint64_t bgt_synth_item_seed(int config, int item);
+int bgt_synth_polylines(uint8_t *img, int w, int h, uint64_t seed, int n_lines, double p_salt);
+int bgt_synth_tree2d(uint8_t *img, int w, int h, uint64_t seed, int depth, double p_salt,
+                     double target_fraction);
+int bgt_synth_tree3d(uint8_t *vol, int w, int h, int d, uint64_t seed, int depth,
+                     int n_ellipsoids, double target_fraction);
+int bgt_synth_batch_dims(uint64_t seed, int n2, int lo2, int hi2, int *dims2, int n3, int lo3,
+                         int hi3, int *dims3);
+int bgt_synth_arrivals(uint64_t seed, int n, double qps, double *gaps, double *mix);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
