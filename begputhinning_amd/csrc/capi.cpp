@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.h"
 #include "common.h"
 #include "segments.h"
 #include "thin2d.h"
@@ -171,6 +172,79 @@ void bgt_segments_free(bgt_segments *s)
 }
 
 void bgt_free(void *p) { std::free(p); }
+
+struct bgt_batch_s {
+    bgt::Batch *b;
+};
+
+int bgt_batch_create(bgt_batch *out)
+{
+    if (!out) return fail(BGT_ERR_INVALID, "null pointer");
+    *out = new bgt_batch_s{batch_new()};
+    return batch_set_config(*(*out)->b, default_config2d(), default_config3d());
+}
+
+int bgt_batch_use_current_config(bgt_batch b)
+{
+    if (!b) return fail(BGT_ERR_INVALID, "null batch");
+    return batch_set_config(*b->b, default_config2d(), default_config3d());
+}
+
+int bgt_batch_add2d(bgt_batch b, const uint8_t *d_in, size_t in_pitch, int width, int height,
+                    uint8_t *d_out, size_t out_pitch)
+{
+    if (!b) return fail(BGT_ERR_INVALID, "null batch");
+    return batch_add2d(*b->b, d_in, in_pitch, width, height, d_out, out_pitch);
+}
+
+int bgt_batch_add3d(bgt_batch b, const uint8_t *d_in, int width, int height, int depth,
+                    uint8_t *d_out)
+{
+    if (!b) return fail(BGT_ERR_INVALID, "null batch");
+    return batch_add3d(*b->b, d_in, width, height, depth, d_out);
+}
+
+int bgt_batch_run(bgt_batch b, void *stream, int use_graph)
+{
+    if (!b) return fail(BGT_ERR_INVALID, "null batch");
+    return batch_run(*b->b, (hipStream_t)stream, use_graph != 0);
+}
+
+int bgt_batch_iterations(bgt_batch b, int *iterations)
+{
+    if (!b || !iterations) return fail(BGT_ERR_INVALID, "null pointer");
+    return batch_iterations(*b->b, iterations);
+}
+
+int bgt_batch_device_bytes(bgt_batch b, size_t *bytes)
+{
+    if (!b || !bytes) return fail(BGT_ERR_INVALID, "null pointer");
+    *bytes = batch_device_bytes(*b->b);
+    return BGT_OK;
+}
+
+int bgt_batch_set_timing(bgt_batch b, int on)
+{
+    if (!b) return fail(BGT_ERR_INVALID, "null batch");
+    batch_set_timing(*b->b, on != 0);
+    return BGT_OK;
+}
+
+int bgt_batch_kernel_time(bgt_batch b, double *ms2d, int *launches2d, double *ms3d,
+                          int *launches3d)
+{
+    if (!b || !ms2d || !launches2d || !ms3d || !launches3d) return fail(BGT_ERR_INVALID, "null pointer");
+    batch_kernel_time(*b->b, ms2d, launches2d, ms3d, launches3d);
+    return BGT_OK;
+}
+
+int bgt_batch_destroy(bgt_batch b)
+{
+    if (!b) return BGT_OK;
+    batch_delete(b->b);
+    delete b;
+    return BGT_OK;
+}
 
 // 2D: pad to pw = ceil32(w), ph = max(1, (h+63)/128)*128 + 64
 // (2D/CenterlineExtraction.cpp:27-28); thinning does not depend on the zero

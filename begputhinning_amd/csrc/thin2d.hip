@@ -164,7 +164,7 @@ __global__ void init_item2d(Item2D *item, Item2D val, unsigned *ctr)
 // the fused pass kernel
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void finalize_item2d(const Item2D &it, int K)
+__device__ __forceinline__ void finalize_item2d(const Item2D &it, int K, int *live)
 {
     int *st = it.st;
     if (st[ST_DONE]) return;
@@ -176,6 +176,7 @@ __device__ __forceinline__ void finalize_item2d(const Item2D &it, int K)
         st[ST_ITERS] = p * K + m + 1;
         st[ST_PARITY] = (p + 1) & 1;
         st[ST_DONE] = 1;
+        if (live) atomicSub(live, 1);
     } else {
         st[ST_PASS] = p + 1;
     }
@@ -185,7 +186,8 @@ __device__ __forceinline__ void finalize_item2d(const Item2D &it, int K)
 template <int R>
 __global__ __launch_bounds__(256) void thin2d_pass(const Item2D *__restrict__ items,
                                                     const Tile2D *__restrict__ tiles, int nitems,
-                                                    int K, unsigned *__restrict__ ctr)
+                                                    int K, unsigned *__restrict__ ctr,
+                                                    int *__restrict__ live)
 {
     extern __shared__ uint64_t edges[];
     __shared__ unsigned s_mask;
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(256) void thin2d_pass(const Item2D *__restrict__ it
     __syncthreads();
     if (s_last) {
         __threadfence();
-        for (int i = tid; i < nitems; i += blockDim.x) finalize_item2d(items[i], K);
+        for (int i = tid; i < nitems; i += blockDim.x) finalize_item2d(items[i], K, live);
         __threadfence();
         if (tid == 0) atomicExch(ctr, 0u);
     }
@@ -318,23 +320,23 @@ int check_config2d(const Config2D &cfg)
     return BGT_OK;
 }
 
-static int launch_pass2d(const Config2D &cfg, int nblocks, const Item2D *items, const Tile2D *tiles,
-                         int nitems, unsigned *ctr, hipStream_t s)
+int launch_pass2d(const Config2D &cfg, int nblocks, const Item2D *items, const Tile2D *tiles,
+                  int nitems, unsigned *ctr, int *live, hipStream_t s)
 {
     const size_t lds = (size_t)2 * 2 * cfg.waves * 64 * sizeof(uint64_t);
     dim3 blk(cfg.waves * 64);
     switch (cfg.rows) {
     case 8:
         hipLaunchKernelGGL(thin2d_pass<8>, dim3(nblocks), blk, lds, s, items, tiles, nitems,
-                           cfg.iters_per_pass, ctr);
+                           cfg.iters_per_pass, ctr, live);
         break;
     case 16:
         hipLaunchKernelGGL(thin2d_pass<16>, dim3(nblocks), blk, lds, s, items, tiles, nitems,
-                           cfg.iters_per_pass, ctr);
+                           cfg.iters_per_pass, ctr, live);
         break;
     default:
         hipLaunchKernelGGL(thin2d_pass<32>, dim3(nblocks), blk, lds, s, items, tiles, nitems,
-                           cfg.iters_per_pass, ctr);
+                           cfg.iters_per_pass, ctr, live);
         break;
     }
     BGT_HIP(hipGetLastError());
@@ -426,7 +428,7 @@ int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned
     int chunk = 2;
     for (;;) {
         for (int i = 0; i < chunk; ++i)
-            BGT_CHECK(launch_pass2d(cfg, pl.nbands, d_item, nullptr, 1, d_ctr, s));
+            BGT_CHECK(launch_pass2d(cfg, pl.nbands, d_item, nullptr, 1, d_ctr, nullptr, s));
         launched += chunk;
         BGT_HIP(hipMemcpyAsync(hp, d_state, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
         BGT_HIP(hipStreamSynchronize(s));

@@ -31,6 +31,10 @@ int setup_item3d(const Plan3D &pl, void *ws, hipStream_t s, Item3D **d_item, uns
                  Item3D *host_item);
 int launch_pass3d(const Config3D &cfg, const Plan3D &pl, int nblocks, const Item3D *items,
                   const Tile3D *tiles, int nitems, unsigned *ctr, hipStream_t s);
+int launch_pass3d_nh(const Config3D &cfg, int NH, int nblocks, const Item3D *items,
+                     const Tile3D *tiles, int nitems, unsigned *ctr, int *live, hipStream_t s);
+size_t lds_bytes3d(const Config3D &cfg, int NH);
+int rows_for_nh(int nh);
 int run_passes3d(const Config3D &cfg, const Plan3D &pl, Item3D *d_item, unsigned *d_ctr,
                  const int *d_state, hipStream_t s, int *iterations);
 

@@ -120,7 +120,7 @@ __global__ void init_item3d(Item3D *item, Item3D val, unsigned *ctr)
 // pass kernel
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void finalize_state(int *st, int K)
+__device__ __forceinline__ void finalize_state(int *st, int K, int *live)
 {
     if (st[ST_DONE]) return;
     const int p = st[ST_PASS];
@@ -131,6 +131,7 @@ __device__ __forceinline__ void finalize_state(int *st, int K)
         st[ST_ITERS] = p * K + m + 1;
         st[ST_PARITY] = (p + 1) & 1;
         st[ST_DONE] = 1;
+        if (live) atomicSub(live, 1);
     } else {
         st[ST_PASS] = p + 1;
     }
@@ -192,7 +193,8 @@ __device__ __forceinline__ void step3d(uint32_t (&s)[R][NH], uint32_t *P,
 template <int NH, int R>
 __global__ __launch_bounds__(512) void thin3d_pass(const Item3D *__restrict__ items,
                                                     const Tile3D *__restrict__ tiles, int nitems,
-                                                    int K, unsigned *__restrict__ ctr)
+                                                    int K, unsigned *__restrict__ ctr,
+                                                    int *__restrict__ live)
 {
     extern __shared__ uint32_t edges3[];  // [buf 2][top/bot 2][NW][NH][64]
     __shared__ unsigned s_mask;
@@ -323,7 +325,7 @@ __global__ __launch_bounds__(512) void thin3d_pass(const Item3D *__restrict__ it
     __syncthreads();
     if (s_last) {
         __threadfence();
-        for (int i = tid; i < nitems; i += blockDim.x) finalize_state(items[i].st, K);
+        for (int i = tid; i < nitems; i += blockDim.x) finalize_state(items[i].st, K, live);
         __threadfence();
         if (tid == 0) atomicExch(ctr, 0u);
     }
@@ -419,29 +421,43 @@ int setup_item3d(const Plan3D &pl, void *ws, hipStream_t s, Item3D **d_item, uns
     return BGT_OK;
 }
 
-int launch_pass3d(const Config3D &cfg, const Plan3D &pl, int nblocks, const Item3D *items,
-                  const Tile3D *tiles, int nitems, unsigned *ctr, hipStream_t s)
+size_t lds_bytes3d(const Config3D &cfg, int NH)
 {
     // z-edge exchange buffers + per-lane protection masks
-    const size_t lds = ((size_t)2 * 2 * cfg.waves * pl.NH * 64 +
-                        (size_t)(pl.R * pl.NH + 1) * cfg.waves * 64) * sizeof(uint32_t);
+    const int R = rows_for_nh(NH);
+    return ((size_t)2 * 2 * cfg.waves * NH * 64 + (size_t)(R * NH + 1) * cfg.waves * 64) *
+           sizeof(uint32_t);
+}
+
+int launch_pass3d_nh(const Config3D &cfg, int NH, int nblocks, const Item3D *items,
+                     const Tile3D *tiles, int nitems, unsigned *ctr, int *live, hipStream_t s)
+{
+    const size_t lds = lds_bytes3d(cfg, NH);
     const dim3 g(nblocks), b(cfg.waves * 64);
     const int K = cfg.iters_per_pass;
-    int dev = 0;
-    BGT_HIP(hipGetDevice(&dev));
-    int max_lds = 0;
-    BGT_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    static thread_local int max_lds = -1;
+    if (max_lds < 0) {
+        int dev = 0;
+        BGT_HIP(hipGetDevice(&dev));
+        BGT_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    }
     if (lds + 64 > (size_t)max_lds)
         return fail(BGT_ERR_UNSUPPORTED, "3D pass needs %zu B of LDS, device allows %d", lds, max_lds);
-    switch (pl.NH) {
-    case 2: hipLaunchKernelGGL((thin3d_pass<2, 8>), g, b, lds, s, items, tiles, nitems, K, ctr); break;
-    case 4: hipLaunchKernelGGL((thin3d_pass<4, 6>), g, b, lds, s, items, tiles, nitems, K, ctr); break;
-    case 6: hipLaunchKernelGGL((thin3d_pass<6, 4>), g, b, lds, s, items, tiles, nitems, K, ctr); break;
-    case 8: hipLaunchKernelGGL((thin3d_pass<8, 3>), g, b, lds, s, items, tiles, nitems, K, ctr); break;
-    default: return fail(BGT_ERR_UNSUPPORTED, "no 3D kernel for %d words per line", pl.NH);
+    switch (NH) {
+    case 2: hipLaunchKernelGGL((thin3d_pass<2, 8>), g, b, lds, s, items, tiles, nitems, K, ctr, live); break;
+    case 4: hipLaunchKernelGGL((thin3d_pass<4, 6>), g, b, lds, s, items, tiles, nitems, K, ctr, live); break;
+    case 6: hipLaunchKernelGGL((thin3d_pass<6, 4>), g, b, lds, s, items, tiles, nitems, K, ctr, live); break;
+    case 8: hipLaunchKernelGGL((thin3d_pass<8, 3>), g, b, lds, s, items, tiles, nitems, K, ctr, live); break;
+    default: return fail(BGT_ERR_UNSUPPORTED, "no 3D kernel for %d words per line", NH);
     }
     BGT_HIP(hipGetLastError());
     return BGT_OK;
+}
+
+int launch_pass3d(const Config3D &cfg, const Plan3D &pl, int nblocks, const Item3D *items,
+                  const Tile3D *tiles, int nitems, unsigned *ctr, hipStream_t s)
+{
+    return launch_pass3d_nh(cfg, pl.NH, nblocks, items, tiles, nitems, ctr, nullptr, s);
 }
 
 static int *poll_buffer3()

@@ -118,6 +118,37 @@ int bgt_centerline3d(int dtype, const void *vol, int width, int height, int dept
 void bgt_free(void *p);
 
 /* ------------------------------------------------------------------------- */
+/* Batched thinning (no reference counterpart: the reference thins one image  */
+/* per call, 2D/LibThinning2D.cu:39-130).  Many device-resident images and    */
+/* volumes of uneven size are thinned together; each item converges on its    */
+/* own and its result equals the single-item result.                          */
+/* ------------------------------------------------------------------------- */
+typedef struct bgt_batch_s *bgt_batch;
+
+int bgt_batch_create(bgt_batch *out);
+/* Take the launch shapes current at this call (bgt_set_config2d/3d). */
+int bgt_batch_use_current_config(bgt_batch b);
+int bgt_batch_add2d(bgt_batch b, const uint8_t *d_in, size_t in_pitch, int width, int height,
+                    uint8_t *d_out, size_t out_pitch);
+int bgt_batch_add3d(bgt_batch b, const uint8_t *d_in, int width, int height, int depth,
+                    uint8_t *d_out);
+/* Thin every item (pack, passes to convergence, unpack); returns when done.
+ * use_graph != 0 replays the pass chunk as a captured hipGraph.  A batch can
+ * be run again (inputs are re-read). */
+int bgt_batch_run(bgt_batch b, void *stream, int use_graph);
+/* Iterations of every item (in the order added), after a run. */
+int bgt_batch_iterations(bgt_batch b, int *iterations);
+/* Device bytes the batch holds (0 before its first run). */
+int bgt_batch_device_bytes(bgt_batch b, size_t *bytes);
+/* Per-launch kernel timing with HIP events on the run's stream (direct
+ * launches only; a run with timing on does not use the graph).  After a run:
+ * total milliseconds and launch counts of the 2D and 3D pass kernels. */
+int bgt_batch_set_timing(bgt_batch b, int on);
+int bgt_batch_kernel_time(bgt_batch b, double *ms2d, int *launches2d, double *ms3d,
+                          int *launches3d);
+int bgt_batch_destroy(bgt_batch b);
+
+/* ------------------------------------------------------------------------- */
 /* Synthetic inputs (seeded, deterministic) for tests and benchmarks.         */
 /* ------------------------------------------------------------------------- */
 uint64_t bgt_synth_item_seed(int config, int item);
