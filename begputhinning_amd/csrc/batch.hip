@@ -1,12 +1,17 @@
 // batch.hip -- many images and volumes of uneven size thinned together.
 //
-// All items of a batch live in one device allocation.  A pass launch covers
-// the bands of every 2D item (one launch) and the bricks of every 3D item (one
-// launch per 3D line-width class, as the kernel is specialised on the register
-// words per x-line); items that have converged make their tiles exit at once,
-// and every item keeps its own pass counter, change flags and iteration count
-// on the device.  The host only polls one pair of live-item counters after each
-// chunk of passes, and the chunk can be replayed as a captured hipGraph.
+// All items of a batch live in one device allocation.  Each pass runs from a
+// work queue of active tiles (2D bands, 3D bricks): a tile is in the queue of
+// pass P+1 iff it changed in pass P, or it is non-empty and a neighbouring
+// tile changed in pass P; empty tiles are never queued.  Tiles never queued for
+// a pass hold identical data in both ping-pong buffers (packing writes both),
+// so skipping them is exact.  Queue kernels are launched with a fixed,
+// co-resident grid whose workgroups pull tiles from an atomic counter; a
+// one-workgroup finalize kernel then advances every item (converged when one
+// iteration of the pass deleted nothing), resets the counters and bumps the
+// pass number.  2D items and 3D items form two independent pipelines on two
+// streams; the host polls their live-item counters once per chunk of passes,
+// and a chunk can be replayed as a captured hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,6 +25,7 @@
 #include "thin2d_kernels.h"
 #include "thin3d.h"
 #include "thin3d_kernels.h"
+#include "sparse3d.h"
 
 namespace bgt {
 
@@ -33,6 +39,17 @@ struct IO2D {
 struct IO3D {
     const uint8_t *in;
     uint8_t *out;
+};
+
+// Work queue of one kernel class (all 2D items, or the 3D items of one line
+// width class).  Tile ids index `tiles`, `stamp` and `occ`.
+struct Queue {
+    int *list[2];   // active tile ids of pass parity 0 / 1
+    int *cnt;       // [2] entries in list[0] / list[1]
+    int *wctr;      // next list slot to hand out
+    int *stamp;     // per tile: last pass it was queued for
+    uint8_t *occ;   // per tile: interior non-empty
+    int ntiles;
 };
 
 namespace {
@@ -96,9 +113,10 @@ __device__ __forceinline__ void unpack_bytes(uint32_t v, uint8_t *row, int x0, i
     for (int k = 0; k < 32 && x0 + k < w; ++k) row[x0 + k] = (uint8_t)((v >> k) & 1u);
 }
 
-// one thread per 32-bit half of a 64-bit row word
+// One thread per 32-bit half of a 64-bit row word; writes both ping-pong
+// buffers and marks the band of every non-empty word occupied.
 __global__ void batch_pack2d(const Item2D *__restrict__ items, const IO2D *__restrict__ io,
-                             const long *__restrict__ off, int n, long total)
+                             const long *__restrict__ off, int n, long total, uint8_t *occ)
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total) return;
@@ -111,6 +129,8 @@ __global__ void batch_pack2d(const Item2D *__restrict__ items, const IO2D *__res
     const uint8_t *row = io[i].in + (size_t)y * io[i].in_pitch;
     const uint32_t v = pack_bytes(row, c * 64 + half * 32, it.w);
     reinterpret_cast<uint32_t *>(it.buf[0])[local] = v;
+    reinterpret_cast<uint32_t *>(it.buf[1])[local] = v;
+    if (v) occ[it.tile0 + y / it.band_rows] = 1;
 }
 
 __global__ void batch_unpack2d(const Item2D *__restrict__ items, const IO2D *__restrict__ io,
@@ -130,45 +150,122 @@ __global__ void batch_unpack2d(const Item2D *__restrict__ items, const IO2D *__r
     unpack_bytes(src[local], io[i].out + (size_t)y * io[i].out_pitch, x0, it.w);
 }
 
-__global__ void batch_pack3d(const Item3D *__restrict__ items, const IO3D *__restrict__ io,
+__global__ void batch_pack3d(const SItem3D *__restrict__ items, const IO3D *__restrict__ io,
                              const long *__restrict__ off, int n, long total)
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total) return;
     const int i = find_item(off, n, g);
-    const Item3D it = items[i];
+    const SItem3D it = items[i];
     const long local = g - off[i];
     const long line = local / it.nh;
     const int j = (int)(local - line * it.nh);
-    it.buf[0][local] = pack_bytes(io[i].in + (size_t)line * it.w, j * 32, it.w);
+    const uint32_t v = pack_bytes(io[i].in + (size_t)line * it.w, j * 32, it.w);
+    it.buf[0][local] = v;
+    it.buf[1][local] = v;
 }
 
-__global__ void batch_unpack3d(const Item3D *__restrict__ items, const IO3D *__restrict__ io,
+__global__ void batch_unpack3d(const SItem3D *__restrict__ items, const IO3D *__restrict__ io,
                                const long *__restrict__ off, int n, long total)
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total) return;
     const int i = find_item(off, n, g);
-    const Item3D it = items[i];
+    const SItem3D it = items[i];
     const long local = g - off[i];
     const long line = local / it.nh;
     const int j = (int)(local - line * it.nh);
     if (j * 32 >= it.w) return;
-    const uint32_t *src = it.st[ST_PARITY] ? it.buf[1] : it.buf[0];
-    unpack_bytes(src[local], io[i].out + (size_t)line * it.w, j * 32, it.w);
+    unpack_bytes(it.buf[0][local], io[i].out + (size_t)line * it.w, j * 32, it.w);
 }
 
-// zero every item state and the launch counters; live[0]/live[1] = item counts
-__global__ void batch_init(int *states, long nstate, unsigned *ctrs, int nctr, int *live, int n2,
-                           int n3)
+// zero states, tile flags and queue control words
+__global__ void batch_init(int *states, long nstate, uint8_t *occ, int *stamp, long ntiles,
+                           int *ctl, int nctl)
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < nstate) states[g] = 0;
-    if (g < nctr) ctrs[g] = 0;
-    if (g == 0) {
-        live[0] = n2;
-        live[1] = n3;
+    if (g < ntiles) {
+        occ[g] = 0;
+        stamp[g] = 0;
     }
+    if (g < nctl) ctl[g] = 0;
+}
+
+__global__ void batch_set_live(int *live2, int n2, int *live3, int n3)
+{
+    if (threadIdx.x == 0) {
+        *live2 = n2;
+        *live3 = n3;
+    }
+}
+
+// queue of pass 0 = every occupied tile
+__global__ void batch_build_list(Queue q)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= q.ntiles || !q.occ[t]) return;
+    const int k = atomicAdd(&q.cnt[0], 1);
+    q.list[0][k] = t;
+}
+
+__device__ __forceinline__ void enqueue(const Queue &q, int u, int next_pass, int *nlist,
+                                        int *ncnt, bool force)
+{
+    if (!force && !q.occ[u]) return;
+    if (atomicMax(&q.stamp[u], next_pass) < next_pass) nlist[atomicAdd(ncnt, 1)] = u;
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void thin2d_queue_pass(const Item2D *__restrict__ items,
+                                                          const Tile2D *__restrict__ tiles,
+                                                          Queue q, int K, const int *gpass)
+{
+    extern __shared__ uint64_t edges[];
+    __shared__ TileInfo info;
+    const int tid = threadIdx.x;
+    const int P = *gpass, cur = P & 1;
+    const int *list = q.list[cur];
+    int *nlist = q.list[cur ^ 1];
+    int *ncnt = &q.cnt[cur ^ 1];
+    const int n = q.cnt[cur];
+    for (;;) {
+        if (tid == 0) info.t = atomicAdd(q.wctr, 1);
+        __syncthreads();
+        const int t = info.t;
+        __syncthreads();
+        if (t >= n) break;
+        const int tile = list[t];
+        const Tile2D tl = tiles[tile];
+        const Item2D it = items[tl.item];
+        if (it.st[ST_DONE]) continue;
+        const int run = band_work<R>(it, tl.band, K, cur, edges, &info);
+        if (tid == 0) {
+            q.occ[tile] = info.occ != 0;
+            if (info.mask) atomicOr((unsigned *)&it.st[ST_MASK], info.mask);
+            if (it.nbands == 1) it.st[ST_RUN] = run + 1;
+        }
+        if (info.mask && it.nbands > 1 && tid < 3) {
+            const int b = tl.band - 1 + tid;
+            if (b >= 0 && b < it.nbands) enqueue(q, it.tile0 + b, P + 1, nlist, ncnt, tid == 1);
+        }
+        __syncthreads();
+    }
+}
+
+// advance the items of one pipeline and reset its queues for the next pass
+__global__ void batch_finalize(int *states, int nitems, int K, int *live, int *gpass,
+                               const Queue *queues, int nq)
+{
+    const int P = *gpass;
+    for (int i = threadIdx.x; i < nitems; i += blockDim.x)
+        finalize_state(states + (size_t)i * ST_INTS, K, P, live);
+    for (int k = threadIdx.x; k < nq; k += blockDim.x) {
+        queues[k].cnt[P & 1] = 0;
+        *queues[k].wctr = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *gpass = P + 1;
 }
 
 struct Carver {
@@ -182,6 +279,18 @@ struct Carver {
         return r;
     }
 };
+
+int occupancy_grid(const void *kernel, int block, size_t lds, int ntiles)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return std::max(1, std::min(ntiles, 1024));
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    if (cus < 1) cus = 256;
+    return std::max(1, std::min(ntiles, cus * per_cu));
+}
 
 }  // namespace
 
@@ -199,34 +308,50 @@ struct Batch {
     std::vector<In2> in2;
     std::vector<In3> in3;
 
-    // prepared state
     bool ready = false;
     void *mem = nullptr;
     size_t bytes = 0;
-    std::vector<int> order3;  // 3D items sorted by class: index into in3
-    struct Class3 {
-        int NH;
-        int first, count;     // range in the sorted 3D item array
-        Tile3D *tiles;
-        int ntiles;
-    };
-    std::vector<Class3> classes;
+
+    // 2D pipeline: band work queue
     Item2D *d_items2 = nullptr;
     IO2D *d_io2 = nullptr;
     long *d_off2 = nullptr;
-    Tile2D *d_tiles2 = nullptr;
-    int ntiles2 = 0;
     long words2 = 0;
-    Item3D *d_items3 = nullptr;
+    Tile2D *d_tiles2 = nullptr;
+    Queue q2{};
+    Queue *d_q2 = nullptr;
+    int grid2 = 0;
+    int *d_states2 = nullptr;
+    int *d_gpass2 = nullptr;
+    int *d_live2 = nullptr;
+    hipGraphExec_t graph2 = nullptr;
+    hipStream_t graph2_stream = nullptr;
+
+    // 3D pipeline: non-empty word lists
+    SItem3D *d_items3 = nullptr;
     IO3D *d_io3 = nullptr;
     long *d_off3 = nullptr;
     long words3 = 0;
-    int *d_states = nullptr;
-    long nstate = 0;
-    unsigned *d_ctr = nullptr;
-    int *d_live = nullptr;
-    int *h_live = nullptr;  // pinned
-    std::vector<int> iters;
+    SList list3[2]{};
+    uint32_t *d_prot3 = nullptr;
+    int *d_changed3 = nullptr;
+    int *d_iter3 = nullptr;
+    int *d_live3 = nullptr;
+    int *d_states3 = nullptr;
+    hipGraphExec_t graph3 = nullptr;
+    hipStream_t graph3_stream = nullptr;
+
+    // control words cleared by one init kernel
+    int *d_ctl = nullptr;
+    int nctl = 0;
+    uint8_t *d_occ = nullptr;
+    int *d_stamp = nullptr;
+    long ntiles2 = 0;
+
+    int *h_live = nullptr;  // pinned: [0] 2D live, [1] 3D live
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
     // optional per-launch kernel timing (direct launches only)
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -234,22 +359,24 @@ struct Batch {
     std::vector<std::pair<size_t, int>> ev_marks;  // (first event index, kind 0=2D 1=3D)
     double ms2d = 0, ms3d = 0;
     int n2d = 0, n3d = 0;
-    // captured chunk of passes
-    hipGraphExec_t graph = nullptr;
-    hipStream_t graph_stream = nullptr;
-    int graph_chunk = 0;
 
-    ~Batch() { release(); }
+    ~Batch()
+    {
+        release();
+        if (side) (void)hipStreamDestroy(side);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (h_live) (void)hipHostFree(h_live);
+    }
     void release()
     {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         events.clear();
-        if (graph) (void)hipGraphExecDestroy(graph);
-        graph = nullptr;
+        if (graph2) (void)hipGraphExecDestroy(graph2);
+        if (graph3) (void)hipGraphExecDestroy(graph3);
+        graph2 = graph3 = nullptr;
         if (mem) (void)hipFree(mem);
         mem = nullptr;
-        if (h_live) (void)hipHostFree(h_live);
-        h_live = nullptr;
         ready = false;
     }
 };
@@ -261,53 +388,66 @@ static int prepare(Batch &b)
     BGT_CHECK(check_config2d(b.c2));
     const int n2 = (int)b.in2.size(), n3 = (int)b.in3.size();
     std::vector<Plan2D> p2(n2);
-    std::vector<Plan3D> p3(n3);
     for (int i = 0; i < n2; ++i) BGT_CHECK(plan2d(b.in2[i].w, b.in2[i].h, b.c2, p2[i]));
-    for (int i = 0; i < n3; ++i) BGT_CHECK(plan3d(b.in3[i].w, b.in3[i].h, b.in3[i].d, b.c3, p3[i]));
-
-    // 3D items grouped by kernel class
-    b.order3.resize(n3);
-    std::iota(b.order3.begin(), b.order3.end(), 0);
-    std::stable_sort(b.order3.begin(), b.order3.end(),
-                     [&](int a, int c) { return p3[a].NH < p3[c].NH; });
-    b.classes.clear();
-    for (int k = 0; k < n3; ++k) {
-        const int NH = p3[b.order3[k]].NH;
-        if (b.classes.empty() || b.classes.back().NH != NH) b.classes.push_back({NH, k, 0, nullptr, 0});
-        b.classes.back().count++;
+    std::vector<int> nh3(n3);
+    std::vector<long> off3(n3 + 1, 0);
+    for (int i = 0; i < n3; ++i) {
+        nh3[i] = 2 * ((b.in3[i].w + 63) / 64);
+        off3[i + 1] = off3[i] + (long)nh3[i] * b.in3[i].h * b.in3[i].d;
     }
+    if (off3[n3] > 0x7fffffffL) return fail(BGT_ERR_UNSUPPORTED, "3D part of the batch too large");
+    const size_t lds2 = (size_t)2 * 2 * b.c2.waves * 64 * sizeof(uint64_t);
 
-    // 2D tiles: largest images first so that the launch tail holds small work
-    std::vector<int> ord2(n2);
-    std::iota(ord2.begin(), ord2.end(), 0);
-    std::stable_sort(ord2.begin(), ord2.end(),
-                     [&](int a, int c) { return p2[a].state_words > p2[c].state_words; });
     std::vector<Tile2D> t2;
-    for (int i : ord2)
+    std::vector<int> tile0(n2);
+    for (int i = 0; i < n2; ++i) {
+        tile0[i] = (int)t2.size();
         for (int k = 0; k < p2[i].nbands; ++k) t2.push_back({i, k});
-    std::vector<std::vector<Tile3D>> t3(b.classes.size());
-    for (size_t c = 0; c < b.classes.size(); ++c)
-        for (int k = 0; k < b.classes[c].count; ++k) {
-            const int it = b.order3[b.classes[c].first + k];
-            for (int br = 0; br < p3[it].nbricks; ++br) t3[c].push_back({k, br});
-        }
+    }
+    b.ntiles2 = (long)t2.size();
+    b.nctl = 16;
+    const long w3 = off3[n3];
 
-    // layout (two sweeps: size, then carve)
     for (int sweep = 0; sweep < 2; ++sweep) {
         Carver cv{sweep ? (char *)b.mem : nullptr};
+        b.d_ctl = cv.take<int>(sizeof(int) * b.nctl);
+        // 2D
         b.d_items2 = cv.take<Item2D>(sizeof(Item2D) * n2);
         b.d_io2 = cv.take<IO2D>(sizeof(IO2D) * n2);
         b.d_off2 = cv.take<long>(sizeof(long) * (n2 + 1));
         b.d_tiles2 = cv.take<Tile2D>(sizeof(Tile2D) * t2.size());
-        b.d_items3 = cv.take<Item3D>(sizeof(Item3D) * n3);
+        b.d_occ = cv.take<uint8_t>((size_t)b.ntiles2);
+        b.d_stamp = cv.take<int>(sizeof(int) * b.ntiles2);
+        int *lists2 = cv.take<int>(sizeof(int) * 2 * b.ntiles2);
+        b.d_q2 = cv.take<Queue>(sizeof(Queue));
+        b.d_states2 = cv.take<int>(sizeof(int) * (size_t)n2 * ST_INTS);
+        // 3D
+        b.d_items3 = cv.take<SItem3D>(sizeof(SItem3D) * n3);
         b.d_io3 = cv.take<IO3D>(sizeof(IO3D) * n3);
         b.d_off3 = cv.take<long>(sizeof(long) * (n3 + 1));
-        for (size_t c = 0; c < b.classes.size(); ++c)
-            b.classes[c].tiles = cv.take<Tile3D>(sizeof(Tile3D) * t3[c].size());
-        b.nstate = (long)(n2 + n3) * ST_INTS;
-        b.d_states = cv.take<int>(sizeof(int) * b.nstate);
-        b.d_ctr = cv.take<unsigned>(sizeof(unsigned) * (1 + b.classes.size()));
-        b.d_live = cv.take<int>(sizeof(int) * 2);
+        for (int k = 0; k < 2; ++k) {
+            b.list3[k].item = cv.take<int>(sizeof(int) * w3);
+            b.list3[k].word = cv.take<uint32_t>(sizeof(uint32_t) * w3);
+        }
+        b.d_prot3 = cv.take<uint32_t>(sizeof(uint32_t) * w3);
+        b.d_changed3 = cv.take<int>(sizeof(int) * n3);
+        b.d_states3 = cv.take<int>(sizeof(int) * (size_t)n3 * ST_INTS);
+        // control words: [0,1] 2D cnt, [2] 2D wctr, [3] 2D gpass, [4] 2D live,
+        // [5] 3D iteration, [6] 3D live, [7,8] 3D list counts
+        int *ctl = b.d_ctl;
+        b.q2.list[0] = lists2;
+        b.q2.list[1] = lists2 ? lists2 + b.ntiles2 : nullptr;
+        b.q2.cnt = ctl;
+        b.q2.wctr = ctl ? ctl + 2 : nullptr;
+        b.q2.stamp = b.d_stamp;
+        b.q2.occ = b.d_occ;
+        b.q2.ntiles = (int)b.ntiles2;
+        b.d_gpass2 = ctl ? ctl + 3 : nullptr;
+        b.d_live2 = ctl ? ctl + 4 : nullptr;
+        b.d_iter3 = ctl ? ctl + 5 : nullptr;
+        b.d_live3 = ctl ? ctl + 6 : nullptr;
+        b.list3[0].count = ctl ? ctl + 7 : nullptr;
+        b.list3[1].count = ctl ? ctl + 8 : nullptr;
 
         std::vector<Item2D> h2(n2);
         std::vector<long> off2(n2 + 1, 0);
@@ -315,28 +455,24 @@ static int prepare(Batch &b)
             Item2D &v = h2[i];
             v.buf[0] = cv.take<uint64_t>(p2[i].state_words * sizeof(uint64_t));
             v.buf[1] = cv.take<uint64_t>(p2[i].state_words * sizeof(uint64_t));
-            v.chg = cv.take<uint8_t>((size_t)2 * p2[i].nbands);
-            v.st = b.d_states ? b.d_states + (size_t)i * ST_INTS : nullptr;
+            v.chg = nullptr;
+            v.st = b.d_states2 ? b.d_states2 + (size_t)i * ST_INTS : nullptr;
             v.w = p2[i].w; v.h = p2[i].h; v.L = p2[i].L; v.lgL = p2[i].lgL;
             v.nbands = p2[i].nbands;
             v.band_rows = p2[i].band_rows;
+            v.tile0 = tile0[i];
             off2[i + 1] = off2[i] + (long)p2[i].state_words * 2;  // 32-bit halves
         }
-        std::vector<Item3D> h3(n3);
-        std::vector<long> off3(n3 + 1, 0);
+        std::vector<SItem3D> h3(n3);
         std::vector<IO3D> io3(n3);
-        for (int k = 0; k < n3; ++k) {
-            const int i = b.order3[k];
-            Item3D &v = h3[k];
-            v.buf[0] = cv.take<uint32_t>(p3[i].state_words * sizeof(uint32_t));
-            v.buf[1] = cv.take<uint32_t>(p3[i].state_words * sizeof(uint32_t));
-            v.chg = cv.take<uint8_t>((size_t)2 * p3[i].nbricks);
-            v.st = b.d_states ? b.d_states + (size_t)(n2 + k) * ST_INTS : nullptr;
-            v.w = p3[i].w; v.h = p3[i].h; v.d = p3[i].d; v.nh = p3[i].nh;
-            v.iy = p3[i].iy; v.iz = p3[i].iz;
-            v.nby = p3[i].nby; v.nbz = p3[i].nbz; v.nbricks = p3[i].nbricks;
-            off3[k + 1] = off3[k] + (long)p3[i].state_words;
-            io3[k] = b.in3[i].io;
+        for (int i = 0; i < n3; ++i) {
+            SItem3D &v = h3[i];
+            const long words = off3[i + 1] - off3[i];
+            v.buf[0] = cv.take<uint32_t>(words * sizeof(uint32_t));
+            v.buf[1] = cv.take<uint32_t>(words * sizeof(uint32_t));
+            v.st = b.d_states3 ? b.d_states3 + (size_t)i * ST_INTS : nullptr;
+            v.w = b.in3[i].w; v.h = b.in3[i].h; v.d = b.in3[i].d; v.nh = nh3[i];
+            io3[i] = b.in3[i].io;
         }
         if (sweep == 0) {
             b.bytes = cv.used;
@@ -351,25 +487,29 @@ static int prepare(Batch &b)
         std::vector<IO2D> io2(n2);
         for (int i = 0; i < n2; ++i) io2[i] = b.in2[i].io;
         b.words2 = off2[n2];
-        b.words3 = off3[n3];
-        b.ntiles2 = (int)t2.size();
-        BGT_HIP(hipMemcpy(b.d_items2, h2.data(), sizeof(Item2D) * n2, hipMemcpyHostToDevice));
-        BGT_HIP(hipMemcpy(b.d_io2, io2.data(), sizeof(IO2D) * n2, hipMemcpyHostToDevice));
-        BGT_HIP(hipMemcpy(b.d_off2, off2.data(), sizeof(long) * (n2 + 1), hipMemcpyHostToDevice));
-        if (!t2.empty())
+        b.words3 = w3;
+        if (n2) {
+            BGT_HIP(hipMemcpy(b.d_items2, h2.data(), sizeof(Item2D) * n2, hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(b.d_io2, io2.data(), sizeof(IO2D) * n2, hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(b.d_off2, off2.data(), sizeof(long) * (n2 + 1), hipMemcpyHostToDevice));
             BGT_HIP(hipMemcpy(b.d_tiles2, t2.data(), sizeof(Tile2D) * t2.size(), hipMemcpyHostToDevice));
-        BGT_HIP(hipMemcpy(b.d_items3, h3.data(), sizeof(Item3D) * n3, hipMemcpyHostToDevice));
-        BGT_HIP(hipMemcpy(b.d_io3, io3.data(), sizeof(IO3D) * n3, hipMemcpyHostToDevice));
-        BGT_HIP(hipMemcpy(b.d_off3, off3.data(), sizeof(long) * (n3 + 1), hipMemcpyHostToDevice));
-        for (size_t c = 0; c < b.classes.size(); ++c) {
-            b.classes[c].ntiles = (int)t3[c].size();
-            BGT_HIP(hipMemcpy(b.classes[c].tiles, t3[c].data(), sizeof(Tile3D) * t3[c].size(),
-                              hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(b.d_q2, &b.q2, sizeof(Queue), hipMemcpyHostToDevice));
+        }
+        if (n3) {
+            BGT_HIP(hipMemcpy(b.d_items3, h3.data(), sizeof(SItem3D) * n3, hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(b.d_io3, io3.data(), sizeof(IO3D) * n3, hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(b.d_off3, off3.data(), sizeof(long) * (n3 + 1), hipMemcpyHostToDevice));
         }
     }
+    if (n2)
+        b.grid2 = occupancy_grid(b.c2.rows == 8 ? (const void *)thin2d_queue_pass<8>
+                                                : (const void *)thin2d_queue_pass<16>,
+                                 b.c2.waves * 64, lds2, (int)b.ntiles2);
     if (!b.h_live && hipHostMalloc((void **)&b.h_live, 64, hipHostMallocDefault) != hipSuccess)
         return fail(BGT_ERR_NOMEM, "pinned allocation failed");
-    b.iters.assign(n2 + n3, 0);
+    if (!b.side) BGT_HIP(hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking));
+    if (!b.ev_fork) BGT_HIP(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
+    if (!b.ev_join) BGT_HIP(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
     b.ready = true;
     return BGT_OK;
 }
@@ -389,21 +529,68 @@ static int timed_mark(Batch &b, hipStream_t s, int kind, bool begin)
     return BGT_OK;
 }
 
-static int launch_one_pass(Batch &b, hipStream_t s)
+// one 2D pass: queue kernel + finalize
+static int launch_pass2d(Batch &b, hipStream_t s)
 {
-    const int n2 = (int)b.in2.size();
-    if (n2) {
-        BGT_CHECK(timed_mark(b, s, 0, true));
-        BGT_CHECK(launch_pass2d(b.c2, b.ntiles2, b.d_items2, b.d_tiles2, n2, b.d_ctr, b.d_live, s));
-        BGT_CHECK(timed_mark(b, s, 0, false));
+    const size_t lds = (size_t)2 * 2 * b.c2.waves * 64 * sizeof(uint64_t);
+    const dim3 g(b.grid2), blk(b.c2.waves * 64);
+    BGT_CHECK(timed_mark(b, s, 0, true));
+    if (b.c2.rows == 8)
+        hipLaunchKernelGGL(thin2d_queue_pass<8>, g, blk, lds, s, b.d_items2, b.d_tiles2, b.q2,
+                           b.c2.iters_per_pass, b.d_gpass2);
+    else
+        hipLaunchKernelGGL(thin2d_queue_pass<16>, g, blk, lds, s, b.d_items2, b.d_tiles2, b.q2,
+                           b.c2.iters_per_pass, b.d_gpass2);
+    BGT_HIP(hipGetLastError());
+    BGT_CHECK(timed_mark(b, s, 0, false));
+    hipLaunchKernelGGL(batch_finalize, dim3(1), dim3(256), 0, s, b.d_states2, (int)b.in2.size(),
+                       b.c2.iters_per_pass, b.d_live2, b.d_gpass2, b.d_q2, 1);
+    BGT_HIP(hipGetLastError());
+    return BGT_OK;
+}
+
+// one 3D iteration from list `cur` into list cur^1
+static int launch_iter3d(Batch &b, int cur, hipStream_t s)
+{
+    BGT_CHECK(timed_mark(b, s, 1, true));
+    BGT_CHECK(sparse3d_iteration(b.d_items3, (int)b.in3.size(), b.list3[cur], b.list3[cur ^ 1],
+                                 (int)b.words3, b.d_prot3, b.d_changed3, b.d_iter3, b.d_live3, s));
+    BGT_CHECK(timed_mark(b, s, 1, false));
+    return BGT_OK;
+}
+
+// A chunk is an even number of passes / iterations so that it starts and ends
+// on the same queue parity and can be captured once and replayed.
+static const int kChunk = 4;
+
+static int launch_chunk(Batch &b, bool three_d, bool use_graph, hipStream_t s)
+{
+    auto body = [&](hipStream_t st) -> int {
+        for (int i = 0; i < kChunk; ++i)
+            BGT_CHECK(three_d ? launch_iter3d(b, i & 1, st) : launch_pass2d(b, st));
+        return BGT_OK;
+    };
+    if (!use_graph) return body(s);
+    hipGraphExec_t &graph = three_d ? b.graph3 : b.graph2;
+    hipStream_t &gs = three_d ? b.graph3_stream : b.graph2_stream;
+    if (!graph || gs != s) {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        hipStream_t cap = nullptr;
+        BGT_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        BGT_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        const int rc = body(cap);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(cap, &g);
+        (void)hipStreamDestroy(cap);
+        if (rc != BGT_OK) return rc;
+        if (e != hipSuccess) return fail(BGT_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(BGT_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+        gs = s;
     }
-    for (size_t c = 0; c < b.classes.size(); ++c) {
-        const Batch::Class3 &k = b.classes[c];
-        BGT_CHECK(timed_mark(b, s, 1, true));
-        BGT_CHECK(launch_pass3d_nh(b.c3, k.NH, k.ntiles, b.d_items3 + k.first, k.tiles, k.count,
-                                   b.d_ctr + 1 + c, b.d_live + 1, s));
-        BGT_CHECK(timed_mark(b, s, 1, false));
-    }
+    BGT_HIP(hipGraphLaunch(graph, s));
     return BGT_OK;
 }
 
@@ -426,64 +613,63 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
 {
     BGT_CHECK(prepare(b));
     const int n2 = (int)b.in2.size(), n3 = (int)b.in3.size();
-    const long init_n = std::max<long>(b.nstate, 1 + (long)b.classes.size());
+    if (b.timing) use_graph = false;
+    b.ev_marks.clear();
+    b.ev_used = 0;
+
+    // set-up on the caller's stream
+    const long init_n = std::max<long>({(long)n2 * ST_INTS, (long)n3 * ST_INTS, b.ntiles2,
+                                        (long)b.nctl, (long)n3, 1L});
     hipLaunchKernelGGL(batch_init, dim3((unsigned)((init_n + 255) / 256)), dim3(256), 0, s,
-                       b.d_states, b.nstate, b.d_ctr, (int)(1 + b.classes.size()), b.d_live, n2, n3);
+                       b.d_states2, (long)n2 * ST_INTS, b.d_occ, b.d_stamp, b.ntiles2, b.d_ctl,
+                       b.nctl);
+    if (n3)
+        hipLaunchKernelGGL(batch_init, dim3((unsigned)((init_n + 255) / 256)), dim3(256), 0, s,
+                           b.d_states3, (long)n3 * ST_INTS, (uint8_t *)nullptr, (int *)nullptr,
+                           0L, b.d_changed3, n3);
+    hipLaunchKernelGGL(batch_set_live, dim3(1), dim3(64), 0, s, b.d_live2, n2, b.d_live3, n3);
     if (b.words2)
         hipLaunchKernelGGL(batch_pack2d, dim3((unsigned)((b.words2 + 255) / 256)), dim3(256), 0, s,
-                           b.d_items2, b.d_io2, b.d_off2, n2, b.words2);
+                           b.d_items2, b.d_io2, b.d_off2, n2, b.words2, b.d_occ);
+    if (n2)
+        hipLaunchKernelGGL(batch_build_list, dim3((unsigned)((b.ntiles2 + 255) / 256)), dim3(256),
+                           0, s, b.q2);
     if (b.words3)
         hipLaunchKernelGGL(batch_pack3d, dim3((unsigned)((b.words3 + 255) / 256)), dim3(256), 0, s,
                            b.d_items3, b.d_io3, b.d_off3, n3, b.words3);
     BGT_HIP(hipGetLastError());
+    if (n3) BGT_CHECK(sparse3d_build_list(b.d_items3, b.d_off3, n3, b.words3, b.list3[0], s));
 
-    const int chunk = 4;
-    if (b.timing) use_graph = false;
-    b.ev_marks.clear();
-    b.ev_used = 0;
-    if (use_graph && (!b.graph || b.graph_stream != s || b.graph_chunk != chunk)) {
-        if (b.graph) (void)hipGraphExecDestroy(b.graph);
-        b.graph = nullptr;
-        hipStream_t cap = nullptr;
-        BGT_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-        BGT_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-        int rc = BGT_OK;
-        for (int i = 0; i < chunk && rc == BGT_OK; ++i) rc = launch_one_pass(b, cap);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(cap, &g);
-        (void)hipStreamDestroy(cap);
-        if (rc != BGT_OK) return rc;
-        if (e != hipSuccess) return fail(BGT_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-        e = hipGraphInstantiate(&b.graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return fail(BGT_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
-        b.graph_stream = s;
-        b.graph_chunk = chunk;
-    }
+    // 3D pipeline on the side stream, forked from and joined into `s`
+    BGT_HIP(hipEventRecord(b.ev_fork, s));
+    BGT_HIP(hipStreamWaitEvent(b.side, b.ev_fork, 0));
+    bool live2 = n2 > 0, live3 = n3 > 0;
     long max_vox = 0;
     for (auto &v : b.in2) max_vox = std::max<long>(max_vox, (long)v.w * v.h);
     for (auto &v : b.in3) max_vox = std::max<long>(max_vox, (long)v.w * v.h * v.d);
-    const long max_pass = max_vox + 2;
-    long passes = 0;
-    for (;;) {
-        if (use_graph) {
-            BGT_HIP(hipGraphLaunch(b.graph, s));
-        } else {
-            for (int i = 0; i < chunk; ++i) BGT_CHECK(launch_one_pass(b, s));
-        }
-        passes += chunk;
-        BGT_HIP(hipMemcpyAsync(b.h_live, b.d_live, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-        BGT_HIP(hipStreamSynchronize(s));
-        if (b.h_live[0] <= 0 && b.h_live[1] <= 0) break;
-        if (passes > max_pass) return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
+    const long max_rounds = max_vox / kChunk + 2;
+    long rounds = 0;
+    while (live2 || live3) {
+        if (live2) BGT_CHECK(launch_chunk(b, false, use_graph, s));
+        if (live3) BGT_CHECK(launch_chunk(b, true, use_graph, b.side));
+        if (live2) BGT_HIP(hipMemcpyAsync(b.h_live, b.d_live2, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (live3)
+            BGT_HIP(hipMemcpyAsync(b.h_live + 1, b.d_live3, sizeof(int), hipMemcpyDeviceToHost, b.side));
+        if (live2) BGT_HIP(hipStreamSynchronize(s));
+        if (live3) BGT_HIP(hipStreamSynchronize(b.side));
+        if (live2 && b.h_live[0] <= 0) live2 = false;
+        if (live3 && b.h_live[1] <= 0) live3 = false;
+        if (++rounds > max_rounds) return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
     }
+    if (b.words3)
+        hipLaunchKernelGGL(batch_unpack3d, dim3((unsigned)((b.words3 + 255) / 256)), dim3(256), 0,
+                           b.side, b.d_items3, b.d_io3, b.d_off3, n3, b.words3);
+    BGT_HIP(hipEventRecord(b.ev_join, b.side));
     if (b.words2)
         hipLaunchKernelGGL(batch_unpack2d, dim3((unsigned)((b.words2 + 255) / 256)), dim3(256), 0, s,
                            b.d_items2, b.d_io2, b.d_off2, n2, b.words2);
-    if (b.words3)
-        hipLaunchKernelGGL(batch_unpack3d, dim3((unsigned)((b.words3 + 255) / 256)), dim3(256), 0, s,
-                           b.d_items3, b.d_io3, b.d_off3, n3, b.words3);
     BGT_HIP(hipGetLastError());
+    BGT_HIP(hipStreamWaitEvent(s, b.ev_join, 0));
     BGT_HIP(hipStreamSynchronize(s));
     if (b.timing) BGT_CHECK(collect_timing(b));
     return BGT_OK;
@@ -500,10 +686,15 @@ int batch_iterations(Batch &b, int *out)
 {
     if (!b.ready) return fail(BGT_ERR_INVALID, "batch has not run");
     const int n2 = (int)b.in2.size(), n3 = (int)b.in3.size();
-    std::vector<int> st((size_t)(n2 + n3) * ST_INTS);
-    BGT_HIP(hipMemcpy(st.data(), b.d_states, st.size() * sizeof(int), hipMemcpyDeviceToHost));
-    for (int i = 0; i < n2; ++i) out[i] = st[(size_t)i * ST_INTS + ST_ITERS];
-    for (int k = 0; k < n3; ++k) out[n2 + b.order3[k]] = st[(size_t)(n2 + k) * ST_INTS + ST_ITERS];
+    std::vector<int> st((size_t)std::max(n2, n3) * ST_INTS);
+    if (n2) {
+        BGT_HIP(hipMemcpy(st.data(), b.d_states2, (size_t)n2 * ST_INTS * sizeof(int), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n2; ++i) out[i] = st[(size_t)i * ST_INTS + ST_ITERS];
+    }
+    if (n3) {
+        BGT_HIP(hipMemcpy(st.data(), b.d_states3, (size_t)n3 * ST_INTS * sizeof(int), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n3; ++i) out[n2 + i] = st[(size_t)i * ST_INTS + ST_ITERS];
+    }
     return BGT_OK;
 }
 
@@ -540,10 +731,7 @@ int batch_set_config(Batch &b, const Config2D &c2, const Config3D &c3)
     return BGT_OK;
 }
 
-size_t batch_device_bytes(Batch &b)
-{
-    return b.ready ? b.bytes : 0;
-}
+size_t batch_device_bytes(Batch &b) { return b.ready ? b.bytes : 0; }
 
 int batch_size(const Batch &b) { return (int)(b.in2.size() + b.in3.size()); }
 

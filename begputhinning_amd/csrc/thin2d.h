@@ -30,8 +30,6 @@ int band_rows_for(int L, const Config2D &cfg);
 size_t workspace_bytes2d(const Plan2D &pl);
 int setup_item2d(const Plan2D &pl, void *ws, hipStream_t s, Item2D **d_item, unsigned **d_ctr,
                  Item2D *host_item);
-int launch_pass2d(const Config2D &cfg, int nblocks, const Item2D *items, const Tile2D *tiles,
-                  int nitems, unsigned *ctr, int *live, hipStream_t s);
 int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned *d_ctr,
                  const int *d_state, hipStream_t s, int *iterations);
 

@@ -164,42 +164,19 @@ __global__ void init_item2d(Item2D *item, Item2D val, unsigned *ctr)
 // the fused pass kernel
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void finalize_item2d(const Item2D &it, int K, int *live)
-{
-    int *st = it.st;
-    if (st[ST_DONE]) return;
-    const int p = st[ST_PASS];
-    const unsigned mask = __hip_atomic_load((unsigned *)&st[ST_MASK], __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-    const int m = __popc(mask);  // iterations of this pass that deleted something
-    if (m < K) {
-        st[ST_ITERS] = p * K + m + 1;
-        st[ST_PARITY] = (p + 1) & 1;
-        st[ST_DONE] = 1;
-        if (live) atomicSub(live, 1);
-    } else {
-        st[ST_PASS] = p + 1;
-    }
-    st[ST_MASK] = 0;
-}
-
+// Per-image pass kernel: one workgroup per band, the last workgroup of the
+// launch advances the item.
 template <int R>
-__global__ __launch_bounds__(256) void thin2d_pass(const Item2D *__restrict__ items,
-                                                    const Tile2D *__restrict__ tiles, int nitems,
-                                                    int K, unsigned *__restrict__ ctr,
-                                                    int *__restrict__ live)
+__global__ __launch_bounds__(256) void thin2d_pass(const Item2D *__restrict__ items, int K,
+                                                    unsigned *__restrict__ ctr)
 {
     extern __shared__ uint64_t edges[];
-    __shared__ unsigned s_mask;
+    __shared__ TileInfo info;
     __shared__ int s_last;
 
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int NW = blockDim.x >> 6;
-    const Tile2D tl = tiles ? tiles[blockIdx.x] : Tile2D{0, (int)blockIdx.x};
-    const Item2D it = items[tl.item];
-    const int band = tl.band;
-
-    if (tid == 0) s_mask = 0;
+    const int tid = threadIdx.x;
+    const Item2D it = items[0];
+    const int band = blockIdx.x;
     bool work = !it.st[ST_DONE];
     int p = 0, par = 0;
     if (work) {
@@ -215,89 +192,24 @@ __global__ __launch_bounds__(256) void thin2d_pass(const Item2D *__restrict__ it
             }
         }
     }
-
     if (work) {
-        const int L = it.L, lgL = it.lgL, G = 64 >> lgL, S = NW * G, SL = NW * 64;
-        const int g = lane >> lgL, c = lane & (L - 1), q = wv * G + g;
-        const int H = 4 * K;
-        const int y_reg0 = band * it.band_rows - H + q * R;  // image row of register row 0
-        const uint32_t mw = c == 0 ? 0u : 0xffffffffu;
-        const uint32_t me = c == L - 1 ? 0u : 0xffffffffu;
-        // rows of this segment inside the band interior, as a bit mask
-        const int ra = max(0, H - q * R), rb = min(R, H + it.band_rows - q * R);
-        const uint32_t interior = rb > ra ? (uint32_t)(((1ull << rb) - 1ull) & ~((1ull << ra) - 1ull)) : 0u;
-        const uint64_t *src = par ? it.buf[1] : it.buf[0];
-        uint64_t *dst = par ? it.buf[0] : it.buf[1];
-
-        uint32_t lo[R], hi[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int y = y_reg0 + r;
-            uint64_t v = 0;
-            if (y >= 0 && y < it.h) v = src[(size_t)y * L + c];
-            lo[r] = (uint32_t)v;
-            hi[r] = (uint32_t)(v >> 32);
-        }
-        // edges written after subiteration s live in buffer (s + 1) & 1;
-        // buffer 0 holds the initial rows.  Layout [buf][top/bot][S*L]; the
-        // entry of lane (g, c) of wave v is v*64 + lane.
-        const int me_idx = tid;
-        const int up_idx = q > 0 ? me_idx - L : me_idx;
-        const int dn_idx = q + 1 < S ? me_idx + L : me_idx;
-        const uint32_t up_ok = q > 0 ? 0xffffffffu : 0u;
-        const uint32_t dn_ok = q + 1 < S ? 0xffffffffu : 0u;
-        edges[(0 * 2 + 0) * SL + me_idx] = ((uint64_t)hi[0] << 32) | lo[0];
-        edges[(0 * 2 + 1) * SL + me_idx] = ((uint64_t)hi[R - 1] << 32) | lo[R - 1];
-        __syncthreads();
-
-        unsigned bits = 0;
-        int s = 0;
-#pragma unroll 1
-        for (int itn = 0; itn < K; ++itn) {
-            uint32_t rows_changed = 0;
-#define BGT_SUB(D)                                                                           \
-    {                                                                                        \
-        const int rb_ = s & 1, wb_ = (s + 1) & 1;                                            \
-        const uint64_t ue = edges[(rb_ * 2 + 1) * SL + up_idx];                              \
-        const uint64_t de = edges[(rb_ * 2 + 0) * SL + dn_idx];                              \
-        subiteration<D, R>(lo, hi, W64{(uint32_t)ue & up_ok, (uint32_t)(ue >> 32) & up_ok},  \
-                           W64{(uint32_t)de & dn_ok, (uint32_t)(de >> 32) & dn_ok}, mw, me,  \
-                           rows_changed);                                                    \
-        edges[(wb_ * 2 + 0) * SL + me_idx] = ((uint64_t)hi[0] << 32) | lo[0];                \
-        edges[(wb_ * 2 + 1) * SL + me_idx] = ((uint64_t)hi[R - 1] << 32) | lo[R - 1];        \
-        __syncthreads();                                                                     \
-        ++s;                                                                                 \
-    }
-            BGT_SUB(0) BGT_SUB(1) BGT_SUB(2) BGT_SUB(3)
-#undef BGT_SUB
-            if (rows_changed & interior) bits |= 1u << itn;
-        }
-
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int y = y_reg0 + r;
-            if (((interior >> r) & 1u) && y >= 0 && y < it.h)
-                dst[(size_t)y * L + c] = ((uint64_t)hi[r] << 32) | lo[r];
-        }
-        if (bits) atomicOr(&s_mask, bits);
-        __syncthreads();
+        const int run = band_work<R>(it, band, K, par, edges, &info);
         if (tid == 0) {
-            it.chg[par * it.nbands + band] = s_mask != 0;
-            if (s_mask) atomicOr((unsigned *)&it.st[ST_MASK], s_mask);
+            it.chg[par * it.nbands + band] = info.mask != 0;
+            if (info.mask) atomicOr((unsigned *)&it.st[ST_MASK], info.mask);
+            if (it.nbands == 1) it.st[ST_RUN] = run + 1;
         }
     }
-
-    // the last workgroup of the launch advances every item's pass state
     if (tid == 0) {
         __threadfence();
         s_last = atomicAdd(ctr, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (s_last) {
+    if (s_last && tid == 0) {
         __threadfence();
-        for (int i = tid; i < nitems; i += blockDim.x) finalize_item2d(items[i], K, live);
+        finalize_state(it.st, K, it.st[ST_PASS], nullptr);
         __threadfence();
-        if (tid == 0) atomicExch(ctr, 0u);
+        atomicExch(ctr, 0u);
     }
 }
 
@@ -313,30 +225,24 @@ int band_rows_for(int L, const Config2D &cfg)
 
 int check_config2d(const Config2D &cfg)
 {
-    if (cfg.waves < 1 || cfg.waves > 4 || (cfg.rows != 8 && cfg.rows != 16 && cfg.rows != 32) ||
+    if (cfg.waves < 1 || cfg.waves > 4 || (cfg.rows != 8 && cfg.rows != 16) ||
         cfg.iters_per_pass < 1 || cfg.iters_per_pass > 16)
         return fail(BGT_ERR_INVALID, "bad 2D config (waves=%d rows=%d k=%d)", cfg.waves, cfg.rows,
                     cfg.iters_per_pass);
     return BGT_OK;
 }
 
-int launch_pass2d(const Config2D &cfg, int nblocks, const Item2D *items, const Tile2D *tiles,
-                  int nitems, unsigned *ctr, int *live, hipStream_t s)
+static int launch_image_pass2d(const Config2D &cfg, int nblocks, const Item2D *item,
+                               unsigned *ctr, hipStream_t s)
 {
     const size_t lds = (size_t)2 * 2 * cfg.waves * 64 * sizeof(uint64_t);
     dim3 blk(cfg.waves * 64);
     switch (cfg.rows) {
     case 8:
-        hipLaunchKernelGGL(thin2d_pass<8>, dim3(nblocks), blk, lds, s, items, tiles, nitems,
-                           cfg.iters_per_pass, ctr, live);
-        break;
-    case 16:
-        hipLaunchKernelGGL(thin2d_pass<16>, dim3(nblocks), blk, lds, s, items, tiles, nitems,
-                           cfg.iters_per_pass, ctr, live);
+        hipLaunchKernelGGL(thin2d_pass<8>, dim3(nblocks), blk, lds, s, item, cfg.iters_per_pass, ctr);
         break;
     default:
-        hipLaunchKernelGGL(thin2d_pass<32>, dim3(nblocks), blk, lds, s, items, tiles, nitems,
-                           cfg.iters_per_pass, ctr, live);
+        hipLaunchKernelGGL(thin2d_pass<16>, dim3(nblocks), blk, lds, s, item, cfg.iters_per_pass, ctr);
         break;
     }
     BGT_HIP(hipGetLastError());
@@ -400,6 +306,7 @@ int setup_item2d(const Plan2D &pl, void *ws, hipStream_t s, Item2D **d_item, uns
     v.w = pl.w; v.h = pl.h; v.L = pl.L; v.lgL = pl.lgL;
     v.nbands = pl.nbands;
     v.band_rows = pl.band_rows;
+    v.tile0 = 0;
     hipLaunchKernelGGL(init_item2d, dim3(1), dim3(64), 0, s, item, v, ctr);
     BGT_HIP(hipGetLastError());
     *d_item = item;
@@ -428,7 +335,7 @@ int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned
     int chunk = 2;
     for (;;) {
         for (int i = 0; i < chunk; ++i)
-            BGT_CHECK(launch_pass2d(cfg, pl.nbands, d_item, nullptr, 1, d_ctr, nullptr, s));
+            BGT_CHECK(launch_image_pass2d(cfg, pl.nbands, d_item, d_ctr, s));
         launched += chunk;
         BGT_HIP(hipMemcpyAsync(hp, d_state, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
         BGT_HIP(hipStreamSynchronize(s));

@@ -24,7 +24,11 @@
 
 namespace bgt {
 
-enum { ST_PASS = 0, ST_DONE = 1, ST_ITERS = 2, ST_MASK = 3, ST_PARITY = 4, ST_INTS = 8 };
+// Per-item state words.  ST_MASK collects, over the tiles of a pass, bit j =
+// "iteration j of this pass deleted something"; ST_RUN is set by a tile that
+// iterated its item to convergence on its own (single-tile items) and then
+// holds the number of iterations that deleted something.
+enum { ST_PASS = 0, ST_DONE = 1, ST_ITERS = 2, ST_MASK = 3, ST_PARITY = 4, ST_RUN = 5, ST_INTS = 8 };
 
 struct Item2D {
     uint64_t *buf[2];   // ping-pong packed state (L words per row)
@@ -33,6 +37,7 @@ struct Item2D {
     int w, h, L, lgL;   // L lanes per row, lgL = log2(L)
     int nbands;         // bands covering the image
     int band_rows;      // interior rows per band
+    int tile0;          // id of band 0 in the batch's tile arrays
 };
 
 struct Tile2D {
@@ -126,6 +131,136 @@ __device__ __forceinline__ void subiteration(uint32_t (&lo)[R], uint32_t (&hi)[R
         // them (that multiplies live registers without adding useful ILP)
         __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+// Advance one item after a pass: converged when some iteration of the pass
+// deleted nothing (or a tile iterated the item to convergence on its own).
+__device__ __forceinline__ void finalize_state(int *st, int K, int pass, int *live)
+{
+    if (st[ST_DONE]) return;
+    const unsigned mask =
+        __hip_atomic_load((unsigned *)&st[ST_MASK], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int run = st[ST_RUN];
+    const int m = run ? run - 1 : __popc(mask);  // iterations of this pass that deleted something
+    if (run || m < K) {
+        st[ST_ITERS] = pass * K + m + 1;
+        st[ST_PARITY] = (pass + 1) & 1;
+        st[ST_DONE] = 1;
+        if (live) atomicSub(live, 1);
+    }
+    st[ST_PASS] = pass + 1;
+    st[ST_MASK] = 0;
+}
+
+// LDS words shared by a workgroup processing one band / brick
+struct TileInfo {
+    unsigned mask;   // iterations (bit j) in which an interior pixel was deleted
+    unsigned occ;    // interior non-empty after the pass
+    unsigned flag;   // per-iteration change flag (tile-local convergence)
+    int t;           // work-queue slot
+};
+
+// Run one pass over band `band` of item `it`: K iterations (4K subiterations)
+// from buffer `par` into buffer par^1, or -- when the band is the item's only
+// band, so that its halo rows are all outside the image -- iterations until one
+// deletes nothing.  Results land in `info`; returns the number of iterations
+// that deleted something in the unbounded case (else 0).
+template <int R>
+__device__ int band_work(const Item2D &it, int band, int K, int par, uint64_t *edges,
+                         TileInfo *info)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int NW = blockDim.x >> 6;
+    const int L = it.L, lgL = it.lgL, G = 64 >> lgL, S = NW * G, SL = NW * 64;
+    const int g = lane >> lgL, c = lane & (L - 1), q = wv * G + g;
+    const int H = 4 * K;
+    const bool whole = it.nbands == 1;  // halo rows all outside the image
+    const int y_reg0 = band * it.band_rows - H + q * R;  // image row of register row 0
+    const uint32_t mw = c == 0 ? 0u : 0xffffffffu;
+    const uint32_t me = c == L - 1 ? 0u : 0xffffffffu;
+    // rows of this segment inside the band interior, as a bit mask
+    const int ra = max(0, H - q * R), rb = min(R, H + it.band_rows - q * R);
+    const uint32_t interior = rb > ra ? (uint32_t)(((1ull << rb) - 1ull) & ~((1ull << ra) - 1ull)) : 0u;
+    const uint64_t *src = par ? it.buf[1] : it.buf[0];
+    uint64_t *dst = par ? it.buf[0] : it.buf[1];
+
+    uint32_t lo[R], hi[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int y = y_reg0 + r;
+        uint64_t v = 0;
+        if (y >= 0 && y < it.h) v = src[(size_t)y * L + c];
+        lo[r] = (uint32_t)v;
+        hi[r] = (uint32_t)(v >> 32);
+    }
+    // edges written after subiteration s live in buffer (s + 1) & 1; buffer 0
+    // holds the initial rows.  Layout [buf][top/bot][NW*64]; the entry of lane
+    // (g, c) of wave v is v*64 + lane, segment q-1 is L entries below.
+    const int me_idx = tid;
+    const int up_idx = q > 0 ? me_idx - L : me_idx;
+    const int dn_idx = q + 1 < S ? me_idx + L : me_idx;
+    const uint32_t up_ok = q > 0 ? 0xffffffffu : 0u;
+    const uint32_t dn_ok = q + 1 < S ? 0xffffffffu : 0u;
+    edges[(0 * 2 + 0) * SL + me_idx] = ((uint64_t)hi[0] << 32) | lo[0];
+    edges[(0 * 2 + 1) * SL + me_idx] = ((uint64_t)hi[R - 1] << 32) | lo[R - 1];
+    if (tid == 0) {
+        info->mask = 0;
+        info->occ = 0;
+        info->flag = 0;
+    }
+    __syncthreads();
+
+    unsigned bits = 0;
+    int s = 0, changed_iters = 0;
+    const int max_iters = whole ? 0x7fffffff : K;
+#pragma unroll 1
+    for (int itn = 0; itn < max_iters; ++itn) {
+        uint32_t rows_changed = 0;
+#define BGT_SUB(D)                                                                           \
+    {                                                                                        \
+        const int rb_ = s & 1, wb_ = (s + 1) & 1;                                            \
+        const uint64_t ue = edges[(rb_ * 2 + 1) * SL + up_idx];                              \
+        const uint64_t de = edges[(rb_ * 2 + 0) * SL + dn_idx];                              \
+        subiteration<D, R>(lo, hi, W64{(uint32_t)ue & up_ok, (uint32_t)(ue >> 32) & up_ok},  \
+                           W64{(uint32_t)de & dn_ok, (uint32_t)(de >> 32) & dn_ok}, mw, me,  \
+                           rows_changed);                                                    \
+        edges[(wb_ * 2 + 0) * SL + me_idx] = ((uint64_t)hi[0] << 32) | lo[0];                \
+        edges[(wb_ * 2 + 1) * SL + me_idx] = ((uint64_t)hi[R - 1] << 32) | lo[R - 1];        \
+        __syncthreads();                                                                     \
+        ++s;                                                                                 \
+    }
+        BGT_SUB(0) BGT_SUB(1) BGT_SUB(2) BGT_SUB(3)
+#undef BGT_SUB
+        const bool ch = (rows_changed & interior) != 0u;
+        if (!whole) {
+            if (ch) bits |= 1u << itn;
+            continue;
+        }
+        // tile-local convergence: stop after the first iteration that deletes nothing
+        if (ch) atomicOr(&info->flag, 1u);
+        __syncthreads();
+        const unsigned any = info->flag;
+        __syncthreads();
+        if (!any) break;
+        ++changed_iters;
+        if (tid == 0) info->flag = 0;
+        // the reset above is ordered before the next iteration's first write by
+        // the barrier that ends its first subiteration
+    }
+
+    uint32_t occ = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int y = y_reg0 + r;
+        if (((interior >> r) & 1u) && y >= 0 && y < it.h) {
+            dst[(size_t)y * L + c] = ((uint64_t)hi[r] << 32) | lo[r];
+            occ |= lo[r] | hi[r];
+        }
+    }
+    if (bits) atomicOr(&info->mask, bits);
+    if (occ) atomicOr(&info->occ, 1u);
+    __syncthreads();
+    return changed_iters;
 }
 
 }  // namespace bgt

@@ -29,10 +29,7 @@ int plan3d(int w, int h, int d, const Config3D &cfg, Plan3D &pl);
 size_t workspace_bytes3d(const Plan3D &pl);
 int setup_item3d(const Plan3D &pl, void *ws, hipStream_t s, Item3D **d_item, unsigned **d_ctr,
                  Item3D *host_item);
-int launch_pass3d(const Config3D &cfg, const Plan3D &pl, int nblocks, const Item3D *items,
-                  const Tile3D *tiles, int nitems, unsigned *ctr, hipStream_t s);
-int launch_pass3d_nh(const Config3D &cfg, int NH, int nblocks, const Item3D *items,
-                     const Tile3D *tiles, int nitems, unsigned *ctr, int *live, hipStream_t s);
+int lds_limit();
 size_t lds_bytes3d(const Config3D &cfg, int NH);
 int rows_for_nh(int nh);
 int run_passes3d(const Config3D &cfg, const Plan3D &pl, Item3D *d_item, unsigned *d_ctr,

@@ -15,6 +15,7 @@
 #include "common.h"
 #include "thin3d.h"
 #include "thin3d_kernels.h"
+#include "sparse3d.h"
 
 namespace bgt {
 
@@ -117,99 +118,24 @@ __global__ void init_item3d(Item3D *item, Item3D val, unsigned *ctr)
 }
 
 // ---------------------------------------------------------------------------
-// pass kernel
+// per-volume pass kernel
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void finalize_state(int *st, int K, int *live)
-{
-    if (st[ST_DONE]) return;
-    const int p = st[ST_PASS];
-    const unsigned mask =
-        __hip_atomic_load((unsigned *)&st[ST_MASK], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int m = __popc(mask);
-    if (m < K) {
-        st[ST_ITERS] = p * K + m + 1;
-        st[ST_PARITY] = (p + 1) & 1;
-        st[ST_DONE] = 1;
-        if (live) atomicSub(live, 1);
-    } else {
-        st[ST_PASS] = p + 1;
-    }
-    st[ST_MASK] = 0;
-}
-
-// One subiteration of direction D (0..5 = U, N, E, D, S, W) over the R x NH
-// register words of a lane.  The Up step also evaluates the protection mask P
-// = voxels deletable in no direction (3D/CUDA_ThinningKernel3D_Linear.cuh:
-// 690-702); later steps delete only voxels outside P.  Bit r of rows_changed is
-// set when z-row r lost a voxel in one of the interior words [xa, xa + nx).
-template <int D, int NH, int R>
-__device__ __forceinline__ void step3d(uint32_t (&s)[R][NH], uint32_t *P,
-                                       const uint32_t *edges_up, const uint32_t *edges_dn,
-                                       uint32_t up_ok, uint32_t dn_ok, int lane, int xa, int nx,
-                                       uint32_t &rows_changed)
-{
-    uint32_t pv[NH];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        uint32_t zm[NH], zp[NH];
-#pragma unroll
-        for (int j = 0; j < NH; ++j) {
-            if (r == 0) zm[j] = edges_up[j * 64 + lane] & up_ok;
-            else zm[j] = pv[j];
-            if (r + 1 < R) zp[j] = s[r + 1 < R ? r + 1 : 0][j];
-            else zp[j] = edges_dn[j * 64 + lane] & dn_ok;
-        }
-        uint32_t nw[NH];
-        uint32_t racc = 0;
-#pragma unroll
-        for (int j = 0; j < NH; ++j) {
-            uint32_t nb[27];
-            gather27<NH>(zm, s[r], zp, j, nb);
-            const uint32_t v = s[r][j];
-            uint32_t del;
-            if (D == 0) {
-                const uint32_t cu = rule3<0>(nb);
-                const uint32_t any = cu | rule3<1>(nb) | rule3<2>(nb) | rule3<3>(nb) |
-                                     rule3<4>(nb) | rule3<5>(nb);
-                P[r * NH + j] = v & ~any;
-                del = cu & v;
-            } else {
-                del = rule3<D>(nb) & v & ~P[r * NH + j];
-            }
-            nw[j] = v ^ del;
-            if ((unsigned)(j - xa) < (unsigned)nx) racc |= del;
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        rows_changed |= (racc != 0u ? 1u : 0u) << r;
-#pragma unroll
-        for (int j = 0; j < NH; ++j) {
-            pv[j] = s[r][j];
-            s[r][j] = nw[j];
-        }
-    }
-}
-
+// Per-volume pass kernel: one workgroup per brick, the last workgroup of the
+// launch advances the item.
 template <int NH, int R>
-__global__ __launch_bounds__(512) void thin3d_pass(const Item3D *__restrict__ items,
-                                                    const Tile3D *__restrict__ tiles, int nitems,
-                                                    int K, unsigned *__restrict__ ctr,
-                                                    int *__restrict__ live)
+__global__ __launch_bounds__(512) void thin3d_pass(const Item3D *__restrict__ items, int K,
+                                                    unsigned *__restrict__ ctr)
 {
-    extern __shared__ uint32_t edges3[];  // [buf 2][top/bot 2][NW][NH][64]
-    __shared__ unsigned s_mask;
+    extern __shared__ uint32_t edges3[];
+    __shared__ TileInfo info;
     __shared__ int s_last;
 
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int NW = blockDim.x >> 6;
-    const Tile3D tl = tiles ? tiles[blockIdx.x] : Tile3D{0, (int)blockIdx.x};
-    const Item3D it = items[tl.item];
-    const int brick = tl.brick;
-    // brick index = (bz * nby + by) * nbx + bx
+    const int tid = threadIdx.x;
+    const Item3D it = items[0];
+    const int brick = blockIdx.x;
     const int nbx = it.nbricks / (it.nby * it.nbz);
     const int bx = brick % nbx, by = (brick / nbx) % it.nby, bz = brick / (nbx * it.nby);
-
-    if (tid == 0) s_mask = 0;
     bool work = !it.st[ST_DONE];
     int p = 0, par = 0;
     if (work) {
@@ -231,103 +157,23 @@ __global__ __launch_bounds__(512) void thin3d_pass(const Item3D *__restrict__ it
             }
         }
     }
-
     if (work) {
-        const int H = 6 * K;
-        const int xhalo = nbx > 1 ? 1 : 0;
-        const int nhc = NH - 2 * xhalo;            // interior words per brick
-        const int j_mem0 = bx * nhc - xhalo;       // memory word of register word 0
-        const int y = by * it.iy - H + lane;
-        const int z_reg0 = bz * it.iz - H + wv * R;
-        const bool y_ok = y >= 0 && y < it.h;
-        const uint32_t ymask = (lane >= H && lane < H + it.iy) ? 0xffffffffu : 0u;
-        const int ra = max(0, H - wv * R), rb = min(R, H + it.iz - wv * R);
-        const uint32_t zint = rb > ra ? (uint32_t)(((1ull << rb) - 1ull) & ~((1ull << ra) - 1ull)) : 0u;
-        const int xa = xhalo, nx = min(nhc, it.nh - bx * nhc);
-        const uint32_t *src = par ? it.buf[1] : it.buf[0];
-        uint32_t *dst = par ? it.buf[0] : it.buf[1];
-
-        uint32_t s[R][NH];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int z = z_reg0 + r;
-            const bool ok = y_ok && z >= 0 && z < it.d;
-            const size_t base = ((size_t)z * it.h + y) * it.nh;
-#pragma unroll
-            for (int j = 0; j < NH; ++j) {
-                const int jm = j_mem0 + j;
-                s[r][j] = (ok && jm >= 0 && jm < it.nh) ? src[base + jm] : 0u;
-            }
-        }
-        // protection masks of this lane after the edge buffers: R*NH words per
-        // lane at a stride of R*NH+1 words (odd, so the lanes hit distinct banks)
-        uint32_t *P = edges3 + 2 * 2 * NW * NH * 64 + tid * (R * NH + 1);
-        const int plane = NH * 64;  // words per (buf, tb, wave) block
-        auto eidx = [&](int b, int tb, int wave) { return ((b * 2 + tb) * NW + wave) * plane; };
-        const int wup = wv > 0 ? wv - 1 : wv, wdn = wv + 1 < NW ? wv + 1 : wv;
-        const uint32_t up_ok = wv > 0 ? 0xffffffffu : 0u, dn_ok = wv + 1 < NW ? 0xffffffffu : 0u;
-#pragma unroll
-        for (int j = 0; j < NH; ++j) {
-            edges3[eidx(0, 0, wv) + j * 64 + lane] = s[0][j];
-            edges3[eidx(0, 1, wv) + j * 64 + lane] = s[R - 1][j];
-        }
-        __syncthreads();
-
-        unsigned bits = 0;
-        int sidx = 0;
-#pragma unroll 1
-        for (int itn = 0; itn < K; ++itn) {
-            uint32_t rows_changed = 0;
-#define BGT_STEP3(D)                                                                          \
-    {                                                                                         \
-        const int rb_ = sidx & 1, wb_ = (sidx + 1) & 1;                                       \
-        step3d<D, NH, R>(s, P, edges3 + eidx(rb_, 1, wup), edges3 + eidx(rb_, 0, wdn), up_ok, \
-                         dn_ok, lane, xa, nx, rows_changed);                                  \
-        _Pragma("unroll") for (int j = 0; j < NH; ++j)                                        \
-        {                                                                                     \
-            edges3[eidx(wb_, 0, wv) + j * 64 + lane] = s[0][j];                               \
-            edges3[eidx(wb_, 1, wv) + j * 64 + lane] = s[R - 1][j];                           \
-        }                                                                                     \
-        __syncthreads();                                                                      \
-        ++sidx;                                                                               \
-    }
-            BGT_STEP3(0) BGT_STEP3(1) BGT_STEP3(2) BGT_STEP3(3) BGT_STEP3(4) BGT_STEP3(5)
-#undef BGT_STEP3
-            if (ymask && (rows_changed & zint)) bits |= 1u << itn;
-        }
-
-        if (ymask && y_ok) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int z = z_reg0 + r;
-                if (((zint >> r) & 1u) && z >= 0 && z < it.d) {
-                    const size_t base = ((size_t)z * it.h + y) * it.nh;
-#pragma unroll
-                    for (int j = 0; j < NH; ++j) {
-                        const int jm = j_mem0 + j;
-                        if ((unsigned)(j - xa) < (unsigned)nx) dst[base + jm] = s[r][j];
-                    }
-                }
-            }
-        }
-        if (bits) atomicOr(&s_mask, bits);
-        __syncthreads();
+        brick_work<NH, R>(it, brick, K, par, edges3, &info);
         if (tid == 0) {
-            it.chg[par * it.nbricks + brick] = s_mask != 0;
-            if (s_mask) atomicOr((unsigned *)&it.st[ST_MASK], s_mask);
+            it.chg[par * it.nbricks + brick] = info.mask != 0;
+            if (info.mask) atomicOr((unsigned *)&it.st[ST_MASK], info.mask);
         }
     }
-
     if (tid == 0) {
         __threadfence();
         s_last = atomicAdd(ctr, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (s_last) {
+    if (s_last && tid == 0) {
         __threadfence();
-        for (int i = tid; i < nitems; i += blockDim.x) finalize_state(items[i].st, K, live);
+        finalize_state(it.st, K, it.st[ST_PASS], nullptr);
         __threadfence();
-        if (tid == 0) atomicExch(ctr, 0u);
+        atomicExch(ctr, 0u);
     }
 }
 
@@ -413,6 +259,9 @@ int setup_item3d(const Plan3D &pl, void *ws, hipStream_t s, Item3D **d_item, uns
     v.w = pl.w; v.h = pl.h; v.d = pl.d; v.nh = pl.nh;
     v.iy = pl.iy; v.iz = pl.iz;
     v.nby = pl.nby; v.nbz = pl.nbz; v.nbricks = pl.nbricks;
+    v.nhc = pl.nbx > 1 ? pl.NH - 2 : pl.nh;
+    v.tile0 = 0;
+    v.occ0 = 0;
     hipLaunchKernelGGL(init_item3d, dim3(1), dim3(64), 0, s, item, v, ctr);
     BGT_HIP(hipGetLastError());
     *d_item = item;
@@ -429,35 +278,36 @@ size_t lds_bytes3d(const Config3D &cfg, int NH)
            sizeof(uint32_t);
 }
 
-int launch_pass3d_nh(const Config3D &cfg, int NH, int nblocks, const Item3D *items,
-                     const Tile3D *tiles, int nitems, unsigned *ctr, int *live, hipStream_t s)
+int lds_limit()
 {
-    const size_t lds = lds_bytes3d(cfg, NH);
-    const dim3 g(nblocks), b(cfg.waves * 64);
-    const int K = cfg.iters_per_pass;
     static thread_local int max_lds = -1;
     if (max_lds < 0) {
         int dev = 0;
-        BGT_HIP(hipGetDevice(&dev));
-        BGT_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess)
+            max_lds = 65536;
     }
-    if (lds + 64 > (size_t)max_lds)
-        return fail(BGT_ERR_UNSUPPORTED, "3D pass needs %zu B of LDS, device allows %d", lds, max_lds);
-    switch (NH) {
-    case 2: hipLaunchKernelGGL((thin3d_pass<2, 8>), g, b, lds, s, items, tiles, nitems, K, ctr, live); break;
-    case 4: hipLaunchKernelGGL((thin3d_pass<4, 6>), g, b, lds, s, items, tiles, nitems, K, ctr, live); break;
-    case 6: hipLaunchKernelGGL((thin3d_pass<6, 4>), g, b, lds, s, items, tiles, nitems, K, ctr, live); break;
-    case 8: hipLaunchKernelGGL((thin3d_pass<8, 3>), g, b, lds, s, items, tiles, nitems, K, ctr, live); break;
-    default: return fail(BGT_ERR_UNSUPPORTED, "no 3D kernel for %d words per line", NH);
+    return max_lds;
+}
+
+static int launch_volume_pass3d(const Config3D &cfg, const Plan3D &pl, const Item3D *item,
+                                unsigned *ctr, hipStream_t s)
+{
+    const size_t lds = lds_bytes3d(cfg, pl.NH);
+    if (lds + 64 > (size_t)lds_limit())
+        return fail(BGT_ERR_UNSUPPORTED, "3D pass needs %zu B of LDS, device allows %d", lds, lds_limit());
+    const dim3 g(pl.nbricks), b(cfg.waves * 64);
+    const int K = cfg.iters_per_pass;
+    switch (pl.NH) {
+    case 2: hipLaunchKernelGGL((thin3d_pass<2, 8>), g, b, lds, s, item, K, ctr); break;
+    case 4: hipLaunchKernelGGL((thin3d_pass<4, 6>), g, b, lds, s, item, K, ctr); break;
+    case 6: hipLaunchKernelGGL((thin3d_pass<6, 4>), g, b, lds, s, item, K, ctr); break;
+    case 8: hipLaunchKernelGGL((thin3d_pass<8, 3>), g, b, lds, s, item, K, ctr); break;
+    default: return fail(BGT_ERR_UNSUPPORTED, "no 3D kernel for %d words per line", pl.NH);
     }
     BGT_HIP(hipGetLastError());
     return BGT_OK;
-}
-
-int launch_pass3d(const Config3D &cfg, const Plan3D &pl, int nblocks, const Item3D *items,
-                  const Tile3D *tiles, int nitems, unsigned *ctr, hipStream_t s)
-{
-    return launch_pass3d_nh(cfg, pl.NH, nblocks, items, tiles, nitems, ctr, nullptr, s);
 }
 
 static int *poll_buffer3()
@@ -479,7 +329,7 @@ int run_passes3d(const Config3D &cfg, const Plan3D &pl, Item3D *d_item, unsigned
     int chunk = 2;
     for (;;) {
         for (int i = 0; i < chunk; ++i)
-            BGT_CHECK(launch_pass3d(cfg, pl, pl.nbricks, d_item, nullptr, 1, d_ctr, s));
+            BGT_CHECK(launch_volume_pass3d(cfg, pl, d_item, d_ctr, s));
         launched += chunk;
         BGT_HIP(hipMemcpyAsync(hp, d_state, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
         BGT_HIP(hipStreamSynchronize(s));
@@ -491,24 +341,171 @@ int run_passes3d(const Config3D &cfg, const Plan3D &pl, Item3D *d_item, unsigned
     return BGT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// per-volume path on the non-empty-word list engine
+// ---------------------------------------------------------------------------
+
+__global__ void pack3d_both(const uint8_t *__restrict__ in, int w, long lines, int nh,
+                            uint32_t *__restrict__ b0, uint32_t *__restrict__ b1)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= lines * nh) return;
+    const long line = idx / nh;
+    const int j = (int)(idx % nh);
+    const uint8_t *row = in + (size_t)line * w;
+    const int x0 = j * 32;
+    uint32_t v = 0;
+    for (int k = 0; k < 32 && x0 + k < w; ++k)
+        if (row[x0 + k]) v |= 1u << k;
+    b0[idx] = v;
+    b1[idx] = v;
+}
+
+template <typename T>
+__global__ void pack3d_both_t(const T *__restrict__ in, int w, long lines, int nh,
+                              uint32_t *__restrict__ b0, uint32_t *__restrict__ b1)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= lines * nh) return;
+    const long line = idx / nh;
+    const int j = (int)(idx % nh);
+    const T *row = in + (size_t)line * w;
+    uint32_t v = 0;
+    for (int k = 0; k < 32 && j * 32 + k < w; ++k)
+        if (fg3d<T>(row[j * 32 + k])) v |= 1u << k;
+    b0[idx] = v;
+    b1[idx] = v;
+}
+
+template <typename T>
+__global__ void unpack3d_sparse(const uint32_t *__restrict__ b0, int w, long lines, int nh,
+                                T *__restrict__ out)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= lines * nh) return;
+    const long line = idx / nh;
+    const int j = (int)(idx % nh);
+    const uint32_t v = b0[idx];
+    T *row = out + (size_t)line * w;
+    for (int k = 0; k < 32 && j * 32 + k < w; ++k) row[j * 32 + k] = ((v >> k) & 1) ? (T)1 : (T)0;
+}
+
+__global__ void init_sparse3d(SItem3D *item, SItem3D val, int *ctl)
+{
+    // ctl: [0] changed, [1] iteration, [2] live, [3] count A, [4] count B
+    const int t = threadIdx.x;
+    if (t == 0) *item = val;
+    if (t < ST_INTS) val.st[t] = 0;
+    if (t < 5) ctl[t] = t == 2 ? 1 : 0;
+}
+
+struct SparseVol {
+    SItem3D *d_item;
+    SItem3D host;
+    SList list[2];
+    uint32_t *prot;
+    int *ctl;
+    long words;
+    int capacity;
+};
+
+static size_t sparse_bytes(long words)
+{
+    return round_up(sizeof(SItem3D), 256) + round_up(ST_INTS * sizeof(int), 256) +
+           round_up(8 * sizeof(int), 256) + 2 * round_up(words * sizeof(uint32_t), 256) +
+           2 * (round_up(words * sizeof(int), 256) + round_up(words * sizeof(uint32_t), 256)) +
+           round_up(words * sizeof(uint32_t), 256) + round_up(sizeof(long) * 2, 256);
+}
+
+static int sparse_setup(int w, int h, int d, void *ws, hipStream_t s, SparseVol &sv)
+{
+    const int nh = 2 * ((w + 63) / 64);
+    const long words = (long)nh * h * d;
+    if (words > 0x7fffffffL) return fail(BGT_ERR_UNSUPPORTED, "3D volume too large");
+    char *p = (char *)ws;
+    auto take = [&](size_t bytes) {
+        char *r = p;
+        p += round_up(bytes, 256);
+        return r;
+    };
+    sv.d_item = (SItem3D *)take(sizeof(SItem3D));
+    int *st = (int *)take(ST_INTS * sizeof(int));
+    sv.ctl = (int *)take(8 * sizeof(int));
+    SItem3D v{};
+    v.buf[0] = (uint32_t *)take(words * sizeof(uint32_t));
+    v.buf[1] = (uint32_t *)take(words * sizeof(uint32_t));
+    v.st = st;
+    v.w = w; v.h = h; v.d = d; v.nh = nh;
+    for (int k = 0; k < 2; ++k) {
+        sv.list[k].item = (int *)take(words * sizeof(int));
+        sv.list[k].word = (uint32_t *)take(words * sizeof(uint32_t));
+        sv.list[k].count = sv.ctl + 3 + k;
+    }
+    sv.prot = (uint32_t *)take(words * sizeof(uint32_t));
+    long *off = (long *)take(sizeof(long) * 2);
+    (void)off;
+    sv.host = v;
+    sv.words = words;
+    sv.capacity = (int)words;
+    hipLaunchKernelGGL(init_sparse3d, dim3(1), dim3(64), 0, s, sv.d_item, v, sv.ctl);
+    BGT_HIP(hipGetLastError());
+    return BGT_OK;
+}
+
+__global__ void set_offsets(long *off, long total)
+{
+    off[0] = 0;
+    off[1] = total;
+}
+
+static int sparse_run(SparseVol &sv, hipStream_t s, int *iterations)
+{
+    // offsets for the list builder (one item)
+    long *off = (long *)((char *)sv.prot + round_up(sv.words * sizeof(uint32_t), 256));
+    hipLaunchKernelGGL(set_offsets, dim3(1), dim3(1), 0, s, off, sv.words);
+    BGT_CHECK(sparse3d_build_list(sv.d_item, off, 1, sv.words, sv.list[0], s));
+    int *hp = poll_buffer3();
+    if (!hp) return fail(BGT_ERR_NOMEM, "pinned poll buffer allocation failed");
+    const long max_iter = (long)sv.host.w * sv.host.h * sv.host.d + 2;
+    long done_iters = 0;
+    int cur = 0, chunk = 2;
+    for (;;) {
+        for (int i = 0; i < chunk; ++i) {
+            BGT_CHECK(sparse3d_iteration(sv.d_item, 1, sv.list[cur], sv.list[cur ^ 1], sv.capacity,
+                                         sv.prot, sv.ctl + 0, sv.ctl + 1, sv.ctl + 2, s));
+            cur ^= 1;
+        }
+        done_iters += chunk;
+        BGT_HIP(hipMemcpyAsync(hp, sv.host.st, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
+        BGT_HIP(hipStreamSynchronize(s));
+        if (hp[ST_DONE]) break;
+        if (done_iters > max_iter) return fail(BGT_ERR_NOT_CONVERGED, "3D thinning did not converge");
+        chunk = std::min(chunk * 2, 16);
+    }
+    if (iterations) *iterations = hp[ST_ITERS];
+    return BGT_OK;
+}
+
 int thin3d_device_u8(const uint8_t *d_in, int w, int h, int d, uint8_t *d_out, hipStream_t s,
                      int *iterations, const Config3D &cfg)
 {
+    (void)cfg;
     if (!d_in || !d_out) return fail(BGT_ERR_INVALID, "null device pointer");
-    Plan3D pl;
-    BGT_CHECK(plan3d(w, h, d, cfg, pl));
+    if (w <= 0 || h <= 0 || d <= 0)
+        return fail(BGT_ERR_INVALID, "3D size must be >= 1 (got %d x %d x %d)", w, h, d);
+    const long words = (long)(2 * ((w + 63) / 64)) * h * d;
     Workspace &ws = thread_workspace(2);
-    BGT_CHECK(workspace_reserve(ws, workspace_bytes3d(pl)));
-    Item3D *d_item;
-    unsigned *d_ctr;
-    Item3D hv;
-    BGT_CHECK(setup_item3d(pl, ws.ptr, s, &d_item, &d_ctr, &hv));
-    const long nw = (long)pl.state_words;
-    const dim3 g((unsigned)((nw + 255) / 256)), b(256);
-    hipLaunchKernelGGL(pack3d_u8, g, b, 0, s, d_in, w, h, d, pl.nh, hv.buf[0]);
+    BGT_CHECK(workspace_reserve(ws, sparse_bytes(words)));
+    SparseVol sv;
+    BGT_CHECK(sparse_setup(w, h, d, ws.ptr, s, sv));
+    const long lines = (long)h * d;
+    const dim3 g((unsigned)((words + 255) / 256)), b(256);
+    hipLaunchKernelGGL(pack3d_both, g, b, 0, s, d_in, w, lines, sv.host.nh, sv.host.buf[0],
+                       sv.host.buf[1]);
     BGT_HIP(hipGetLastError());
-    BGT_CHECK(run_passes3d(cfg, pl, d_item, d_ctr, hv.st, s, iterations));
-    hipLaunchKernelGGL(unpack3d<uint8_t>, g, b, 0, s, d_item, d_out);
+    BGT_CHECK(sparse_run(sv, s, iterations));
+    hipLaunchKernelGGL(unpack3d_sparse<uint8_t>, g, b, 0, s, sv.host.buf[0], w, lines, sv.host.nh,
+                       d_out);
     BGT_HIP(hipGetLastError());
     BGT_HIP(hipStreamSynchronize(s));
     return BGT_OK;
@@ -517,28 +514,27 @@ int thin3d_device_u8(const uint8_t *d_in, int w, int h, int d, uint8_t *d_out, h
 template <typename T>
 static int thin3d_host_t(int w, int h, int d, const T *in, T *out, const Config3D &cfg)
 {
-    Plan3D pl;
-    BGT_CHECK(plan3d(w, h, d, cfg, pl));
+    (void)cfg;
+    if (w <= 0 || h <= 0 || d <= 0)
+        return fail(BGT_ERR_INVALID, "3D size must be >= 1 (got %d x %d x %d)", w, h, d);
     const size_t bytes = (size_t)w * h * d * sizeof(T);
+    const long words = (long)(2 * ((w + 63) / 64)) * h * d;
     Workspace &ws = thread_workspace(2);
     Workspace &io = thread_workspace(3);
-    BGT_CHECK(workspace_reserve(ws, workspace_bytes3d(pl)));
+    BGT_CHECK(workspace_reserve(ws, sparse_bytes(words)));
     BGT_CHECK(workspace_reserve(io, bytes));
     hipStream_t s = nullptr;
     BGT_HIP(hipMemcpyAsync(io.ptr, in, bytes, hipMemcpyHostToDevice, s));
-    Item3D *d_item;
-    unsigned *d_ctr;
-    Item3D hv;
-    BGT_CHECK(setup_item3d(pl, ws.ptr, s, &d_item, &d_ctr, &hv));
-    const long nw = (long)pl.state_words;
-    const dim3 g((unsigned)((nw + 255) / 256)), b(256);
-    if (std::is_same<T, uint8_t>::value)
-        hipLaunchKernelGGL(pack3d_u8, g, b, 0, s, (const uint8_t *)io.ptr, w, h, d, pl.nh, hv.buf[0]);
-    else
-        hipLaunchKernelGGL(pack3d_t<T>, g, b, 0, s, (const T *)io.ptr, w, h, d, pl.nh, hv.buf[0]);
+    SparseVol sv;
+    BGT_CHECK(sparse_setup(w, h, d, ws.ptr, s, sv));
+    const long lines = (long)h * d;
+    const dim3 g((unsigned)((words + 255) / 256)), b(256);
+    hipLaunchKernelGGL(pack3d_both_t<T>, g, b, 0, s, (const T *)io.ptr, w, lines, sv.host.nh,
+                       sv.host.buf[0], sv.host.buf[1]);
     BGT_HIP(hipGetLastError());
-    BGT_CHECK(run_passes3d(cfg, pl, d_item, d_ctr, hv.st, s, nullptr));
-    hipLaunchKernelGGL(unpack3d<T>, g, b, 0, s, d_item, (T *)io.ptr);
+    BGT_CHECK(sparse_run(sv, s, nullptr));
+    hipLaunchKernelGGL(unpack3d_sparse<T>, g, b, 0, s, sv.host.buf[0], w, lines, sv.host.nh,
+                       (T *)io.ptr);
     BGT_HIP(hipGetLastError());
     BGT_HIP(hipMemcpyAsync(out, io.ptr, bytes, hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
