@@ -25,20 +25,26 @@
 #include "thin2d_kernels.h"
 #include "thin3d.h"
 #include "thin3d_kernels.h"
+#include "pack.h"
 #include "sparse3d.h"
 
 namespace bgt {
 
+// Caller buffers of one item and the first block of its pack/unpack grid.
+// The pack/unpack kernels give every item whole 256-thread blocks; a table
+// maps each block to its item.
 struct IO2D {
     const uint8_t *in;
     size_t in_pitch;
     uint8_t *out;
     size_t out_pitch;
+    int blk0;
 };
 
 struct IO3D {
     const uint8_t *in;
     uint8_t *out;
+    int blk0;
 };
 
 // Work queue of one kernel class (all 2D items, or the 3D items of one line
@@ -54,129 +60,74 @@ struct Queue {
 
 namespace {
 
-__device__ __forceinline__ int find_item(const long *off, int n, long g)
+// One thread per 64-bit row word; writes both ping-pong buffers and marks the
+// band of every non-empty word occupied.
+__global__ __launch_bounds__(256) void batch_pack2d(const Item2D *__restrict__ items,
+                                                    const IO2D *__restrict__ io,
+                                                    const int *__restrict__ blk_item, uint8_t *occ)
 {
-    int lo = 0, hi = n - 1;  // largest i with off[i] <= g
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t nz4b(uint32_t x)
-{
-    uint32_t t = x | (x >> 4);
-    t |= t >> 2;
-    t |= t >> 1;
-    t &= 0x01010101u;
-    return (t | (t >> 7) | (t >> 14) | (t >> 21)) & 0xfu;
-}
-
-__device__ __forceinline__ uint32_t spread4b(uint32_t b)
-{
-    return ((b & 0xfu) * 0x00204081u) & 0x01010101u;
-}
-
-// 32 bytes -> 32 bits (p 16-byte aligned)
-__device__ __forceinline__ uint32_t pack32(const uint8_t *p)
-{
-    const uint4 a = reinterpret_cast<const uint4 *>(p)[0];
-    const uint4 b = reinterpret_cast<const uint4 *>(p)[1];
-    return nz4b(a.x) | nz4b(a.y) << 4 | nz4b(a.z) << 8 | nz4b(a.w) << 12 | nz4b(b.x) << 16 |
-           nz4b(b.y) << 20 | nz4b(b.z) << 24 | nz4b(b.w) << 28;
-}
-
-__device__ __forceinline__ void unpack32(uint32_t v, uint8_t *p)
-{
-    uint4 *q = reinterpret_cast<uint4 *>(p);
-    q[0] = make_uint4(spread4b(v), spread4b(v >> 4), spread4b(v >> 8), spread4b(v >> 12));
-    q[1] = make_uint4(spread4b(v >> 16), spread4b(v >> 20), spread4b(v >> 24), spread4b(v >> 28));
-}
-
-__device__ __forceinline__ uint32_t pack_bytes(const uint8_t *row, int x0, int w)
-{
-    if (x0 + 32 <= w && ((((uintptr_t)(row + x0)) & 15) == 0)) return pack32(row + x0);
-    uint32_t v = 0;
-    for (int k = 0; k < 32 && x0 + k < w; ++k)
-        if (row[x0 + k]) v |= 1u << k;
-    return v;
-}
-
-__device__ __forceinline__ void unpack_bytes(uint32_t v, uint8_t *row, int x0, int w)
-{
-    if (x0 + 32 <= w && ((((uintptr_t)(row + x0)) & 15) == 0)) {
-        unpack32(v, row + x0);
-        return;
-    }
-    for (int k = 0; k < 32 && x0 + k < w; ++k) row[x0 + k] = (uint8_t)((v >> k) & 1u);
-}
-
-// One thread per 32-bit half of a 64-bit row word; writes both ping-pong
-// buffers and marks the band of every non-empty word occupied.
-__global__ void batch_pack2d(const Item2D *__restrict__ items, const IO2D *__restrict__ io,
-                             const long *__restrict__ off, int n, long total, uint8_t *occ)
-{
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    const int i = find_item(off, n, g);
+    const int i = blk_item[blockIdx.x];
     const Item2D it = items[i];
-    const long local = g - off[i];  // half-word index
-    const long wi = local >> 1;
-    const int half = (int)(local & 1);
-    const int y = (int)(wi >> it.lgL), c = (int)(wi & (it.L - 1));
-    const uint8_t *row = io[i].in + (size_t)y * io[i].in_pitch;
-    const uint32_t v = pack_bytes(row, c * 64 + half * 32, it.w);
-    reinterpret_cast<uint32_t *>(it.buf[0])[local] = v;
-    reinterpret_cast<uint32_t *>(it.buf[1])[local] = v;
+    const IO2D f = io[i];
+    const int t = (blockIdx.x - f.blk0) * 256 + threadIdx.x;
+    if (t >= it.h * it.L) return;
+    const int y = t >> it.lgL, x0 = (t & (it.L - 1)) * 64;
+    const uint64_t v =
+        x0 < it.w ? pack_run_u8(f.in + (size_t)y * f.in_pitch + x0, min(64, it.w - x0)) : 0ull;
+    it.buf[0][t] = v;
+    it.buf[1][t] = v;
     if (v) occ[it.tile0 + y / it.band_rows] = 1;
 }
 
-__global__ void batch_unpack2d(const Item2D *__restrict__ items, const IO2D *__restrict__ io,
-                               const long *__restrict__ off, int n, long total)
+__global__ __launch_bounds__(256) void batch_unpack2d(const Item2D *__restrict__ items,
+                                                      const IO2D *__restrict__ io,
+                                                      const int *__restrict__ blk_item)
 {
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    const int i = find_item(off, n, g);
+    const int i = blk_item[blockIdx.x];
     const Item2D it = items[i];
-    const long local = g - off[i];
-    const long wi = local >> 1;
-    const int half = (int)(local & 1);
-    const int y = (int)(wi >> it.lgL), c = (int)(wi & (it.L - 1));
-    const int x0 = c * 64 + half * 32;
+    const IO2D f = io[i];
+    const int t = (blockIdx.x - f.blk0) * 256 + threadIdx.x;
+    if (t >= it.h * it.L) return;
+    const int y = t >> it.lgL, x0 = (t & (it.L - 1)) * 64;
     if (x0 >= it.w) return;
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(it.st[ST_PARITY] ? it.buf[1] : it.buf[0]);
-    unpack_bytes(src[local], io[i].out + (size_t)y * io[i].out_pitch, x0, it.w);
+    const uint64_t v = (it.st[ST_PARITY] ? it.buf[1] : it.buf[0])[t];
+    unpack_run_u8(v, f.out + (size_t)y * f.out_pitch + x0, min(64, it.w - x0));
 }
 
-__global__ void batch_pack3d(const SItem3D *__restrict__ items, const IO3D *__restrict__ io,
-                             const long *__restrict__ off, int n, long total)
+// One thread per 64 voxels (two 32-bit state words) of an x-line.
+__global__ __launch_bounds__(256) void batch_pack3d(const SItem3D *__restrict__ items,
+                                                    const IO3D *__restrict__ io,
+                                                    const int *__restrict__ blk_item)
 {
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    const int i = find_item(off, n, g);
+    const int i = blk_item[blockIdx.x];
     const SItem3D it = items[i];
-    const long local = g - off[i];
-    const long line = local / it.nh;
-    const int j = (int)(local - line * it.nh);
-    const uint32_t v = pack_bytes(io[i].in + (size_t)line * it.w, j * 32, it.w);
-    it.buf[0][local] = v;
-    it.buf[1][local] = v;
+    const IO3D f = io[i];
+    const int nq = it.nh >> 1;
+    const long t = (long)(blockIdx.x - f.blk0) * 256 + threadIdx.x;
+    if (t >= (long)it.h * it.d * nq) return;
+    const long line = t / nq;
+    const int x0 = (int)(t - line * nq) * 64;
+    const uint64_t v =
+        x0 < it.w ? pack_run_u8(f.in + (size_t)line * it.w + x0, min(64, it.w - x0)) : 0ull;
+    reinterpret_cast<uint64_t *>(it.buf[0])[t] = v;
+    reinterpret_cast<uint64_t *>(it.buf[1])[t] = v;
 }
 
-__global__ void batch_unpack3d(const SItem3D *__restrict__ items, const IO3D *__restrict__ io,
-                               const long *__restrict__ off, int n, long total)
+__global__ __launch_bounds__(256) void batch_unpack3d(const SItem3D *__restrict__ items,
+                                                      const IO3D *__restrict__ io,
+                                                      const int *__restrict__ blk_item)
 {
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    const int i = find_item(off, n, g);
+    const int i = blk_item[blockIdx.x];
     const SItem3D it = items[i];
-    const long local = g - off[i];
-    const long line = local / it.nh;
-    const int j = (int)(local - line * it.nh);
-    if (j * 32 >= it.w) return;
-    unpack_bytes(it.buf[0][local], io[i].out + (size_t)line * it.w, j * 32, it.w);
+    const IO3D f = io[i];
+    const int nq = it.nh >> 1;
+    const long t = (long)(blockIdx.x - f.blk0) * 256 + threadIdx.x;
+    if (t >= (long)it.h * it.d * nq) return;
+    const long line = t / nq;
+    const int x0 = (int)(t - line * nq) * 64;
+    if (x0 >= it.w) return;
+    unpack_run_u8(reinterpret_cast<const uint64_t *>(it.buf[0])[t], f.out + (size_t)line * it.w + x0,
+                  min(64, it.w - x0));
 }
 
 // zero states, tile flags and queue control words
@@ -315,8 +266,8 @@ struct Batch {
     // 2D pipeline: band work queue
     Item2D *d_items2 = nullptr;
     IO2D *d_io2 = nullptr;
-    long *d_off2 = nullptr;
-    long words2 = 0;
+    int *d_blk2 = nullptr;  // pack/unpack block -> item
+    int nblk2 = 0;
     Tile2D *d_tiles2 = nullptr;
     Queue q2{};
     Queue *d_q2 = nullptr;
@@ -330,7 +281,9 @@ struct Batch {
     // 3D pipeline: non-empty word lists
     SItem3D *d_items3 = nullptr;
     IO3D *d_io3 = nullptr;
-    long *d_off3 = nullptr;
+    int *d_blk3 = nullptr;
+    int nblk3 = 0;
+    long *d_off3 = nullptr;  // first state word of each item
     long words3 = 0;
     SList list3[2]{};
     uint32_t *d_prot3 = nullptr;
@@ -407,6 +360,20 @@ static int prepare(Batch &b)
     b.ntiles2 = (long)t2.size();
     b.nctl = 16;
     const long w3 = off3[n3];
+    // pack/unpack grids: whole blocks per item, one thread per 64 pixels
+    std::vector<int> blk2, blk3, blk0_2(n2), blk0_3(n3);
+    for (int i = 0; i < n2; ++i) {
+        blk0_2[i] = (int)blk2.size();
+        const long nt = (long)p2[i].state_words;
+        for (long k = 0; k < (nt + 255) / 256; ++k) blk2.push_back(i);
+    }
+    for (int i = 0; i < n3; ++i) {
+        blk0_3[i] = (int)blk3.size();
+        const long nt = (off3[i + 1] - off3[i]) / 2;
+        for (long k = 0; k < (nt + 255) / 256; ++k) blk3.push_back(i);
+    }
+    b.nblk2 = (int)blk2.size();
+    b.nblk3 = (int)blk3.size();
 
     for (int sweep = 0; sweep < 2; ++sweep) {
         Carver cv{sweep ? (char *)b.mem : nullptr};
@@ -414,7 +381,7 @@ static int prepare(Batch &b)
         // 2D
         b.d_items2 = cv.take<Item2D>(sizeof(Item2D) * n2);
         b.d_io2 = cv.take<IO2D>(sizeof(IO2D) * n2);
-        b.d_off2 = cv.take<long>(sizeof(long) * (n2 + 1));
+        b.d_blk2 = cv.take<int>(sizeof(int) * blk2.size());
         b.d_tiles2 = cv.take<Tile2D>(sizeof(Tile2D) * t2.size());
         b.d_occ = cv.take<uint8_t>((size_t)b.ntiles2);
         b.d_stamp = cv.take<int>(sizeof(int) * b.ntiles2);
@@ -425,6 +392,7 @@ static int prepare(Batch &b)
         b.d_items3 = cv.take<SItem3D>(sizeof(SItem3D) * n3);
         b.d_io3 = cv.take<IO3D>(sizeof(IO3D) * n3);
         b.d_off3 = cv.take<long>(sizeof(long) * (n3 + 1));
+        b.d_blk3 = cv.take<int>(sizeof(int) * blk3.size());
         for (int k = 0; k < 2; ++k) {
             b.list3[k].item = cv.take<int>(sizeof(int) * w3);
             b.list3[k].word = cv.take<uint32_t>(sizeof(uint32_t) * w3);
@@ -450,7 +418,6 @@ static int prepare(Batch &b)
         b.list3[1].count = ctl ? ctl + 8 : nullptr;
 
         std::vector<Item2D> h2(n2);
-        std::vector<long> off2(n2 + 1, 0);
         for (int i = 0; i < n2; ++i) {
             Item2D &v = h2[i];
             v.buf[0] = cv.take<uint64_t>(p2[i].state_words * sizeof(uint64_t));
@@ -461,7 +428,6 @@ static int prepare(Batch &b)
             v.nbands = p2[i].nbands;
             v.band_rows = p2[i].band_rows;
             v.tile0 = tile0[i];
-            off2[i + 1] = off2[i] + (long)p2[i].state_words * 2;  // 32-bit halves
         }
         std::vector<SItem3D> h3(n3);
         std::vector<IO3D> io3(n3);
@@ -473,6 +439,7 @@ static int prepare(Batch &b)
             v.st = b.d_states3 ? b.d_states3 + (size_t)i * ST_INTS : nullptr;
             v.w = b.in3[i].w; v.h = b.in3[i].h; v.d = b.in3[i].d; v.nh = nh3[i];
             io3[i] = b.in3[i].io;
+            io3[i].blk0 = blk0_3[i];
         }
         if (sweep == 0) {
             b.bytes = cv.used;
@@ -485,13 +452,15 @@ static int prepare(Batch &b)
             continue;
         }
         std::vector<IO2D> io2(n2);
-        for (int i = 0; i < n2; ++i) io2[i] = b.in2[i].io;
-        b.words2 = off2[n2];
+        for (int i = 0; i < n2; ++i) {
+            io2[i] = b.in2[i].io;
+            io2[i].blk0 = blk0_2[i];
+        }
         b.words3 = w3;
         if (n2) {
             BGT_HIP(hipMemcpy(b.d_items2, h2.data(), sizeof(Item2D) * n2, hipMemcpyHostToDevice));
             BGT_HIP(hipMemcpy(b.d_io2, io2.data(), sizeof(IO2D) * n2, hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(b.d_off2, off2.data(), sizeof(long) * (n2 + 1), hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(b.d_blk2, blk2.data(), sizeof(int) * blk2.size(), hipMemcpyHostToDevice));
             BGT_HIP(hipMemcpy(b.d_tiles2, t2.data(), sizeof(Tile2D) * t2.size(), hipMemcpyHostToDevice));
             BGT_HIP(hipMemcpy(b.d_q2, &b.q2, sizeof(Queue), hipMemcpyHostToDevice));
         }
@@ -499,6 +468,7 @@ static int prepare(Batch &b)
             BGT_HIP(hipMemcpy(b.d_items3, h3.data(), sizeof(SItem3D) * n3, hipMemcpyHostToDevice));
             BGT_HIP(hipMemcpy(b.d_io3, io3.data(), sizeof(IO3D) * n3, hipMemcpyHostToDevice));
             BGT_HIP(hipMemcpy(b.d_off3, off3.data(), sizeof(long) * (n3 + 1), hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(b.d_blk3, blk3.data(), sizeof(int) * blk3.size(), hipMemcpyHostToDevice));
         }
     }
     if (n2)
@@ -554,7 +524,8 @@ static int launch_iter3d(Batch &b, int cur, hipStream_t s)
 {
     BGT_CHECK(timed_mark(b, s, 1, true));
     BGT_CHECK(sparse3d_iteration(b.d_items3, (int)b.in3.size(), b.list3[cur], b.list3[cur ^ 1],
-                                 (int)b.words3, b.d_prot3, b.d_changed3, b.d_iter3, b.d_live3, s));
+                                 (int)b.words3, b.d_prot3, b.d_changed3, b.d_iter3, b.d_live3,
+                                 sparse3d_grid(b.c3.blocks_per_cu), s));
     BGT_CHECK(timed_mark(b, s, 1, false));
     return BGT_OK;
 }
@@ -628,15 +599,15 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
                            b.d_states3, (long)n3 * ST_INTS, (uint8_t *)nullptr, (int *)nullptr,
                            0L, b.d_changed3, n3);
     hipLaunchKernelGGL(batch_set_live, dim3(1), dim3(64), 0, s, b.d_live2, n2, b.d_live3, n3);
-    if (b.words2)
-        hipLaunchKernelGGL(batch_pack2d, dim3((unsigned)((b.words2 + 255) / 256)), dim3(256), 0, s,
-                           b.d_items2, b.d_io2, b.d_off2, n2, b.words2, b.d_occ);
+    if (b.nblk2)
+        hipLaunchKernelGGL(batch_pack2d, dim3(b.nblk2), dim3(256), 0, s, b.d_items2, b.d_io2,
+                           b.d_blk2, b.d_occ);
     if (n2)
         hipLaunchKernelGGL(batch_build_list, dim3((unsigned)((b.ntiles2 + 255) / 256)), dim3(256),
                            0, s, b.q2);
-    if (b.words3)
-        hipLaunchKernelGGL(batch_pack3d, dim3((unsigned)((b.words3 + 255) / 256)), dim3(256), 0, s,
-                           b.d_items3, b.d_io3, b.d_off3, n3, b.words3);
+    if (b.nblk3)
+        hipLaunchKernelGGL(batch_pack3d, dim3(b.nblk3), dim3(256), 0, s, b.d_items3, b.d_io3,
+                           b.d_blk3);
     BGT_HIP(hipGetLastError());
     if (n3) BGT_CHECK(sparse3d_build_list(b.d_items3, b.d_off3, n3, b.words3, b.list3[0], s));
 
@@ -661,13 +632,13 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         if (live3 && b.h_live[1] <= 0) live3 = false;
         if (++rounds > max_rounds) return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
     }
-    if (b.words3)
-        hipLaunchKernelGGL(batch_unpack3d, dim3((unsigned)((b.words3 + 255) / 256)), dim3(256), 0,
-                           b.side, b.d_items3, b.d_io3, b.d_off3, n3, b.words3);
+    if (b.nblk3)
+        hipLaunchKernelGGL(batch_unpack3d, dim3(b.nblk3), dim3(256), 0, b.side, b.d_items3, b.d_io3,
+                           b.d_blk3);
     BGT_HIP(hipEventRecord(b.ev_join, b.side));
-    if (b.words2)
-        hipLaunchKernelGGL(batch_unpack2d, dim3((unsigned)((b.words2 + 255) / 256)), dim3(256), 0, s,
-                           b.d_items2, b.d_io2, b.d_off2, n2, b.words2);
+    if (b.nblk2)
+        hipLaunchKernelGGL(batch_unpack2d, dim3(b.nblk2), dim3(256), 0, s, b.d_items2, b.d_io2,
+                           b.d_blk2);
     BGT_HIP(hipGetLastError());
     BGT_HIP(hipStreamWaitEvent(s, b.ev_join, 0));
     BGT_HIP(hipStreamSynchronize(s));
@@ -725,6 +696,7 @@ int batch_add3d(Batch &b, const uint8_t *in, int w, int h, int d, uint8_t *out)
 int batch_set_config(Batch &b, const Config2D &c2, const Config3D &c3)
 {
     BGT_CHECK(check_config2d(c2));
+    BGT_CHECK(check_config3d(c3));
     b.release();
     b.c2 = c2;
     b.c3 = c3;

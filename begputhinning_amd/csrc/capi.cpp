@@ -87,8 +87,8 @@ static Config3D &config3d_storage()
 {
     static Config3D c = [] {
         Config3D v;
-        if (const char *e = std::getenv("BGT_3D_WAVES")) v.waves = std::atoi(e);
-        if (const char *e = std::getenv("BGT_3D_K")) v.iters_per_pass = std::atoi(e);
+        if (const char *e = std::getenv("BGT_3D_BLOCKS")) v.blocks_per_cu = std::atoi(e);
+        if (const char *e = std::getenv("BGT_3D_CHUNK")) v.max_chunk = std::atoi(e);
         return v;
     }();
     return c;
@@ -117,13 +117,12 @@ int bgt_set_config2d(int waves, int rows, int iters_per_pass)
     return BGT_OK;
 }
 
-int bgt_set_config3d(int waves, int iters_per_pass)
+int bgt_set_config3d(int blocks_per_cu, int max_chunk)
 {
-    if (waves < 1 || waves > 8 || iters_per_pass < 1 || iters_per_pass > 5)
-        return fail(BGT_ERR_INVALID, "bad 3D config (waves=%d k=%d)", waves, iters_per_pass);
     Config3D c;
-    c.waves = waves;
-    c.iters_per_pass = iters_per_pass;
+    c.blocks_per_cu = blocks_per_cu;
+    c.max_chunk = max_chunk;
+    BGT_CHECK(check_config3d(c));
     config3d_storage() = c;
     return BGT_OK;
 }

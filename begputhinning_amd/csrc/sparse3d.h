@@ -22,12 +22,15 @@ struct SList {
     int *count;
 };
 
-int sparse3d_grid();
+// Workgroups of the grid-stride step kernel: blocks_per_cu per CU, or the
+// occupancy limit when blocks_per_cu is 0.
+int sparse3d_grid(int blocks_per_cu);
 int sparse3d_build_list(const SItem3D *items, const long *off, int nitems, long total_words,
                         SList list, hipStream_t s);
 // One iteration over `cur` (six subiteration launches, finalize, compaction
 // into `next`, whose count is reset first).  max_entries bounds cur's count.
 int sparse3d_iteration(const SItem3D *items, int nitems, SList cur, SList next, int max_entries,
-                       uint32_t *prot, int *changed, int *iteration, int *live, hipStream_t s);
+                       uint32_t *prot, int *changed, int *iteration, int *live, int grid,
+                       hipStream_t s);
 
 }  // namespace bgt

@@ -189,20 +189,21 @@ __global__ __launch_bounds__(256) void sparse3d_build(const SItem3D *__restrict_
 
 }  // namespace
 
-int sparse3d_grid()
+int sparse3d_grid(int blocks_per_cu)
 {
-    static int g = 0;
-    if (!g) {
-        int dev = 0, cus = 256, per_cu = 4;
+    static int cus = 0, occ = 0;
+    if (!cus) {
+        int dev = 0, c = 256, per_cu = 4;
         if (hipGetDevice(&dev) == hipSuccess) {
-            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            (void)hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)sparse3d_step<0>,
                                                              256, 0) != hipSuccess || per_cu < 1)
                 per_cu = 4;
         }
-        g = std::max(1, cus) * std::max(1, per_cu);
+        occ = std::max(1, per_cu);
+        cus = std::max(1, c);
     }
-    return g;
+    return cus * (blocks_per_cu > 0 ? blocks_per_cu : occ);
 }
 
 int sparse3d_build_list(const SItem3D *items, const long *off, int nitems, long total_words,
@@ -216,9 +217,10 @@ int sparse3d_build_list(const SItem3D *items, const long *off, int nitems, long 
 }
 
 int sparse3d_iteration(const SItem3D *items, int nitems, SList cur, SList next, int max_entries,
-                       uint32_t *prot, int *changed, int *iteration, int *live, hipStream_t s)
+                       uint32_t *prot, int *changed, int *iteration, int *live, int grid,
+                       hipStream_t s)
 {
-    const dim3 g(sparse3d_grid()), b(256);
+    const dim3 g(grid), b(256);
     hipLaunchKernelGGL(sparse3d_step<0>, g, b, 0, s, items, cur, prot, changed);
     hipLaunchKernelGGL(sparse3d_step<1>, g, b, 0, s, items, cur, prot, changed);
     hipLaunchKernelGGL(sparse3d_step<2>, g, b, 0, s, items, cur, prot, changed);

@@ -14,6 +14,7 @@
 
 #include "common.h"
 #include "thin2d.h"
+#include "pack.h"
 #include "thin2d_kernels.h"
 
 namespace bgt {
@@ -47,21 +48,6 @@ __device__ __forceinline__ T one_value()
     return (T)1;
 }
 
-// 4 packed bytes -> 4 bits (bit k set iff byte k is nonzero)
-__device__ __forceinline__ uint32_t nz4(uint32_t x)
-{
-    uint32_t t = x | (x >> 4);
-    t |= t >> 2;
-    t |= t >> 1;
-    t &= 0x01010101u;
-    return (t | (t >> 7) | (t >> 14) | (t >> 21)) & 0xfu;
-}
-// 4 bits -> 4 bytes of 0/1
-__device__ __forceinline__ uint32_t spread4(uint32_t b)
-{
-    return ((b & 0xfu) * 0x00204081u) & 0x01010101u;
-}
-
 // ---------------------------------------------------------------------------
 // pack / unpack
 // ---------------------------------------------------------------------------
@@ -73,26 +59,8 @@ __global__ void pack2d_u8(const uint8_t *__restrict__ in, size_t pitch, int w, i
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)L * h) return;
     const int y = (int)(idx / L), c = (int)(idx % L);
-    const uint8_t *row = in + (size_t)y * pitch;
     const int x0 = c * 64;
-    uint32_t lo = 0, hi = 0;
-    if (x0 + 64 <= w && ((((uintptr_t)(row + x0)) & 15) == 0)) {
-        const uint4 *p = reinterpret_cast<const uint4 *>(row + x0);
-        uint4 a = p[0], b = p[1], cc = p[2], d = p[3];
-        lo = nz4(a.x) | nz4(a.y) << 4 | nz4(a.z) << 8 | nz4(a.w) << 12 | nz4(b.x) << 16 |
-             nz4(b.y) << 20 | nz4(b.z) << 24 | nz4(b.w) << 28;
-        hi = nz4(cc.x) | nz4(cc.y) << 4 | nz4(cc.z) << 8 | nz4(cc.w) << 12 | nz4(d.x) << 16 |
-             nz4(d.y) << 20 | nz4(d.z) << 24 | nz4(d.w) << 28;
-    } else {
-        for (int k = 0; k < 64; ++k) {
-            int x = x0 + k;
-            if (x < w && row[x]) {
-                if (k < 32) lo |= 1u << k;
-                else hi |= 1u << (k - 32);
-            }
-        }
-    }
-    out[idx] = ((uint64_t)hi << 32) | lo;
+    out[idx] = x0 < w ? pack_run_u8(in + (size_t)y * pitch + x0, min(64, w - x0)) : 0ull;
 }
 
 template <typename T>
@@ -121,17 +89,7 @@ __global__ void unpack2d_u8(const Item2D *__restrict__ item, uint8_t *__restrict
     const int x0 = c * 64;
     if (x0 >= it.w) return;
     const uint64_t v = (it.st[ST_PARITY] ? it.buf[1] : it.buf[0])[idx];
-    uint8_t *row = out + (size_t)y * pitch;
-    if (x0 + 64 <= it.w && ((((uintptr_t)(row + x0)) & 15) == 0)) {
-        uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-        uint4 *p = reinterpret_cast<uint4 *>(row + x0);
-        p[0] = make_uint4(spread4(lo), spread4(lo >> 4), spread4(lo >> 8), spread4(lo >> 12));
-        p[1] = make_uint4(spread4(lo >> 16), spread4(lo >> 20), spread4(lo >> 24), spread4(lo >> 28));
-        p[2] = make_uint4(spread4(hi), spread4(hi >> 4), spread4(hi >> 8), spread4(hi >> 12));
-        p[3] = make_uint4(spread4(hi >> 16), spread4(hi >> 20), spread4(hi >> 24), spread4(hi >> 28));
-    } else {
-        for (int k = 0; k < 64 && x0 + k < it.w; ++k) row[x0 + k] = (uint8_t)((v >> k) & 1);
-    }
+    unpack_run_u8(v, out + (size_t)y * pitch + x0, min(64, it.w - x0));
 }
 
 template <typename T>

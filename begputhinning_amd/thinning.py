@@ -176,6 +176,7 @@ def set_config2d(waves=4, rows=16, iters_per_pass=2):
     check(lib().bgt_set_config2d(waves, rows, iters_per_pass))
 
 
-def set_config3d(waves=8, iters_per_pass=1):
-    """Launch shape of the 3D pass kernel for later calls (see bgt.h)."""
-    check(lib().bgt_set_config3d(waves, iters_per_pass))
+def set_config3d(blocks_per_cu=0, max_chunk=16):
+    """Launch shape of the 3D subiteration kernel for later calls: workgroups
+    per CU (0 = occupancy limit) and iterations queued per host poll."""
+    check(lib().bgt_set_config3d(blocks_per_cu, max_chunk))

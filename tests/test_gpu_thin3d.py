@@ -15,9 +15,9 @@ def _gpu():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     import begputhinning_amd as B
-    B.set_config3d(8, 1)
+    B.set_config3d(0, 16)
     yield
-    B.set_config3d(8, 1)
+    B.set_config3d(0, 16)
 
 
 def check(vol, label):
@@ -52,7 +52,7 @@ def test_workload3_256():
     check(synth.workload3(), "tree 256^3")
 
 
-@pytest.mark.parametrize("cfg", [(4, 1), (8, 2), (6, 1)])
+@pytest.mark.parametrize("cfg", [(1, 2), (2, 4), (8, 64)])
 def test_launch_shapes(cfg):
     import begputhinning_amd as B
     from begputhinning_amd import synth
@@ -60,7 +60,7 @@ def test_launch_shapes(cfg):
     try:
         check(synth.tree3d(70, 90, 80, 99), f"config {cfg}")
     finally:
-        B.set_config3d(8, 1)
+        B.set_config3d(0, 16)
 
 
 @pytest.mark.parametrize("dt", [np.uint8, np.int16, np.float32, np.bool_])
