@@ -1,21 +1,26 @@
 // batch.hip -- many images and volumes of uneven size thinned together.
 //
-// All items of a batch live in one device allocation.  Each pass runs from a
-// work queue of active tiles (2D bands, 3D bricks): a tile is in the queue of
-// pass P+1 iff it changed in pass P, or it is non-empty and a neighbouring
-// tile changed in pass P; empty tiles are never queued.  Tiles never queued for
-// a pass hold identical data in both ping-pong buffers (packing writes both),
-// so skipping them is exact.  Queue kernels are launched with a fixed,
-// co-resident grid whose workgroups pull tiles from an atomic counter; a
-// one-workgroup finalize kernel then advances every item (converged when one
+// All items of a batch live in one device allocation.  2D items and 3D items
+// form two independent pipelines on two streams; the host polls their
+// live-item counters once per chunk of passes, and a chunk can be replayed as
+// a captured hipGraph.
+//
+// 2D: each pass runs from a work queue of active bands.  A band is in the
+// queue of pass P+1 iff it changed in pass P, or it is non-empty and a
+// neighbouring band changed in pass P; empty bands are never queued.  Bands
+// never queued for a pass hold identical data in both ping-pong buffers
+// (packing writes both), so skipping them is exact.  The queue kernel runs a
+// fixed, co-resident grid whose workgroups pull bands from an atomic counter;
+// a one-workgroup finalize kernel then advances every item (converged when one
 // iteration of the pass deleted nothing), resets the counters and bumps the
-// pass number.  2D items and 3D items form two independent pipelines on two
-// streams; the host polls their live-item counters once per chunk of passes,
-// and a chunk can be replayed as a captured hipGraph.
+// pass number.
+//
+// 3D: the non-empty-word list engine (sparse3d.hip) over all volumes at once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <vector>
 
@@ -38,13 +43,10 @@ struct IO2D {
     size_t in_pitch;
     uint8_t *out;
     size_t out_pitch;
-    int blk0;
-};
-
-struct IO3D {
-    const uint8_t *in;
-    uint8_t *out;
-    int blk0;
+    int blk0;           // first block of the item in the pack grid
+    int ublk0;          // first block of the item in the unpack grid
+    uint32_t uchunks;   // unpack chunks per segment
+    int flat;           // rows contiguous: the whole image is one segment
 };
 
 // Work queue of one kernel class (all 2D items, or the 3D items of one line
@@ -79,55 +81,17 @@ __global__ __launch_bounds__(256) void batch_pack2d(const Item2D *__restrict__ i
     if (v) occ[it.tile0 + y / it.band_rows] = 1;
 }
 
+// One thread per 16-byte chunk of the caller's image (or of one of its rows).
 __global__ __launch_bounds__(256) void batch_unpack2d(const Item2D *__restrict__ items,
                                                       const IO2D *__restrict__ io,
-                                                      const int *__restrict__ blk_item)
+                                                      const int *__restrict__ ublk)
 {
-    const int i = blk_item[blockIdx.x];
+    const int i = ublk[blockIdx.x];
     const Item2D it = items[i];
     const IO2D f = io[i];
-    const int t = (blockIdx.x - f.blk0) * 256 + threadIdx.x;
-    if (t >= it.h * it.L) return;
-    const int y = t >> it.lgL, x0 = (t & (it.L - 1)) * 64;
-    if (x0 >= it.w) return;
-    const uint64_t v = (it.st[ST_PARITY] ? it.buf[1] : it.buf[0])[t];
-    unpack_run_u8(v, f.out + (size_t)y * f.out_pitch + x0, min(64, it.w - x0));
-}
-
-// One thread per 64 voxels (two 32-bit state words) of an x-line.
-__global__ __launch_bounds__(256) void batch_pack3d(const SItem3D *__restrict__ items,
-                                                    const IO3D *__restrict__ io,
-                                                    const int *__restrict__ blk_item)
-{
-    const int i = blk_item[blockIdx.x];
-    const SItem3D it = items[i];
-    const IO3D f = io[i];
-    const int nq = it.nh >> 1;
-    const long t = (long)(blockIdx.x - f.blk0) * 256 + threadIdx.x;
-    if (t >= (long)it.h * it.d * nq) return;
-    const long line = t / nq;
-    const int x0 = (int)(t - line * nq) * 64;
-    const uint64_t v =
-        x0 < it.w ? pack_run_u8(f.in + (size_t)line * it.w + x0, min(64, it.w - x0)) : 0ull;
-    reinterpret_cast<uint64_t *>(it.buf[0])[t] = v;
-    reinterpret_cast<uint64_t *>(it.buf[1])[t] = v;
-}
-
-__global__ __launch_bounds__(256) void batch_unpack3d(const SItem3D *__restrict__ items,
-                                                      const IO3D *__restrict__ io,
-                                                      const int *__restrict__ blk_item)
-{
-    const int i = blk_item[blockIdx.x];
-    const SItem3D it = items[i];
-    const IO3D f = io[i];
-    const int nq = it.nh >> 1;
-    const long t = (long)(blockIdx.x - f.blk0) * 256 + threadIdx.x;
-    if (t >= (long)it.h * it.d * nq) return;
-    const long line = t / nq;
-    const int x0 = (int)(t - line * nq) * 64;
-    if (x0 >= it.w) return;
-    unpack_run_u8(reinterpret_cast<const uint64_t *>(it.buf[0])[t], f.out + (size_t)line * it.w + x0,
-                  min(64, it.w - x0));
+    const uint32_t t = (uint32_t)(blockIdx.x - f.ublk0) * 256u + threadIdx.x;
+    const uint64_t *buf = it.st[ST_PARITY] ? it.buf[1] : it.buf[0];
+    unpack_image_u8(buf, (uint32_t)it.L, it.w, it.h, f.out, f.out_pitch, f.uchunks, f.flat, t);
 }
 
 // zero states, tile flags and queue control words
@@ -143,12 +107,9 @@ __global__ void batch_init(int *states, long nstate, uint8_t *occ, int *stamp, l
     if (g < nctl) ctl[g] = 0;
 }
 
-__global__ void batch_set_live(int *live2, int n2, int *live3, int n3)
+__global__ void batch_set_live(int *live2, int n2)
 {
-    if (threadIdx.x == 0) {
-        *live2 = n2;
-        *live3 = n3;
-    }
+    if (threadIdx.x == 0) *live2 = n2;
 }
 
 // queue of pass 0 = every occupied tile
@@ -253,7 +214,7 @@ struct Batch {
         int w, h;
     };
     struct In3 {
-        IO3D io;
+        SIO io;
         int w, h, d;
     };
     std::vector<In2> in2;
@@ -266,8 +227,10 @@ struct Batch {
     // 2D pipeline: band work queue
     Item2D *d_items2 = nullptr;
     IO2D *d_io2 = nullptr;
-    int *d_blk2 = nullptr;  // pack/unpack block -> item
+    int *d_blk2 = nullptr;   // pack block -> item
     int nblk2 = 0;
+    int *d_ublk2 = nullptr;  // unpack block -> item
+    int nublk2 = 0;
     Tile2D *d_tiles2 = nullptr;
     Queue q2{};
     Queue *d_q2 = nullptr;
@@ -278,20 +241,12 @@ struct Batch {
     hipGraphExec_t graph2 = nullptr;
     hipStream_t graph2_stream = nullptr;
 
-    // 3D pipeline: non-empty word lists
-    SItem3D *d_items3 = nullptr;
-    IO3D *d_io3 = nullptr;
-    int *d_blk3 = nullptr;
-    int nblk3 = 0;
-    long *d_off3 = nullptr;  // first state word of each item
-    long words3 = 0;
-    SList list3[2]{};
-    uint32_t *d_prot3 = nullptr;
-    int *d_changed3 = nullptr;
-    int *d_iter3 = nullptr;
-    int *d_live3 = nullptr;
-    int *d_states3 = nullptr;
-    hipGraphExec_t graph3 = nullptr;
+    // 3D pipeline: non-empty word list engine
+    Sparse3D e3;
+    bool zeroed3 = false;  // state margins cleared since allocation
+    int bound3 = 0;        // upper bound of the current 3D list's count
+    int cur3 = 0;          // current 3D list
+    std::vector<std::pair<int, hipGraphExec_t>> graphs3;  // by launch bound
     hipStream_t graph3_stream = nullptr;
 
     // control words cleared by one init kernel
@@ -325,9 +280,11 @@ struct Batch {
     {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         events.clear();
+        for (auto &g : graphs3) (void)hipGraphExecDestroy(g.second);
+        graphs3.clear();
         if (graph2) (void)hipGraphExecDestroy(graph2);
-        if (graph3) (void)hipGraphExecDestroy(graph3);
-        graph2 = graph3 = nullptr;
+        graph2 = nullptr;
+        zeroed3 = false;
         if (mem) (void)hipFree(mem);
         mem = nullptr;
         ready = false;
@@ -342,13 +299,16 @@ static int prepare(Batch &b)
     const int n2 = (int)b.in2.size(), n3 = (int)b.in3.size();
     std::vector<Plan2D> p2(n2);
     for (int i = 0; i < n2; ++i) BGT_CHECK(plan2d(b.in2[i].w, b.in2[i].h, b.c2, p2[i]));
-    std::vector<int> nh3(n3);
-    std::vector<long> off3(n3 + 1, 0);
+    BGT_CHECK(check_config3d(b.c3));
+    std::vector<int> w3v(n3), h3v(n3), d3v(n3);
     for (int i = 0; i < n3; ++i) {
-        nh3[i] = 2 * ((b.in3[i].w + 63) / 64);
-        off3[i + 1] = off3[i] + (long)nh3[i] * b.in3[i].h * b.in3[i].d;
+        w3v[i] = b.in3[i].w;
+        h3v[i] = b.in3[i].h;
+        d3v[i] = b.in3[i].d;
     }
-    if (off3[n3] > 0x7fffffffL) return fail(BGT_ERR_UNSUPPORTED, "3D part of the batch too large");
+    Layout3D lay3;
+    if (n3) BGT_CHECK(sparse3d_layout(w3v.data(), h3v.data(), d3v.data(), n3, lay3));
+    const long nwords3 = lay3.nwords, data3 = lay3.data_words;
     const size_t lds2 = (size_t)2 * 2 * b.c2.waves * 64 * sizeof(uint64_t);
 
     std::vector<Tile2D> t2;
@@ -359,21 +319,25 @@ static int prepare(Batch &b)
     }
     b.ntiles2 = (long)t2.size();
     b.nctl = 16;
-    const long w3 = off3[n3];
-    // pack/unpack grids: whole blocks per item, one thread per 64 pixels
-    std::vector<int> blk2, blk3, blk0_2(n2), blk0_3(n3);
+    // pack grid: whole blocks per item, one thread per 64 pixels; unpack
+    // grid: one thread per 16-byte chunk of the image (rows contiguous) or of
+    // each row (pitched)
+    std::vector<int> blk2, blk0_2(n2), ublk2, ublk0_2(n2);
+    std::vector<IO2D> io2(n2);
     for (int i = 0; i < n2; ++i) {
         blk0_2[i] = (int)blk2.size();
         const long nt = (long)p2[i].state_words;
         for (long k = 0; k < (nt + 255) / 256; ++k) blk2.push_back(i);
-    }
-    for (int i = 0; i < n3; ++i) {
-        blk0_3[i] = (int)blk3.size();
-        const long nt = (off3[i + 1] - off3[i]) / 2;
-        for (long k = 0; k < (nt + 255) / 256; ++k) blk3.push_back(i);
+        IO2D &f = io2[i];
+        f = b.in2[i].io;
+        f.blk0 = blk0_2[i];
+        long ut = 0;
+        unpack_image_grid(b.in2[i].w, b.in2[i].h, f.out_pitch, f.flat, f.uchunks, ut);
+        f.ublk0 = (int)ublk2.size();
+        for (long k = 0; k < (ut + 255) / 256; ++k) ublk2.push_back(i);
     }
     b.nblk2 = (int)blk2.size();
-    b.nblk3 = (int)blk3.size();
+    b.nublk2 = (int)ublk2.size();
 
     for (int sweep = 0; sweep < 2; ++sweep) {
         Carver cv{sweep ? (char *)b.mem : nullptr};
@@ -382,6 +346,7 @@ static int prepare(Batch &b)
         b.d_items2 = cv.take<Item2D>(sizeof(Item2D) * n2);
         b.d_io2 = cv.take<IO2D>(sizeof(IO2D) * n2);
         b.d_blk2 = cv.take<int>(sizeof(int) * blk2.size());
+        b.d_ublk2 = cv.take<int>(sizeof(int) * ublk2.size());
         b.d_tiles2 = cv.take<Tile2D>(sizeof(Tile2D) * t2.size());
         b.d_occ = cv.take<uint8_t>((size_t)b.ntiles2);
         b.d_stamp = cv.take<int>(sizeof(int) * b.ntiles2);
@@ -389,19 +354,35 @@ static int prepare(Batch &b)
         b.d_q2 = cv.take<Queue>(sizeof(Queue));
         b.d_states2 = cv.take<int>(sizeof(int) * (size_t)n2 * ST_INTS);
         // 3D
-        b.d_items3 = cv.take<SItem3D>(sizeof(SItem3D) * n3);
-        b.d_io3 = cv.take<IO3D>(sizeof(IO3D) * n3);
-        b.d_off3 = cv.take<long>(sizeof(long) * (n3 + 1));
-        b.d_blk3 = cv.take<int>(sizeof(int) * blk3.size());
+        Sparse3D &e = b.e3;
+        e = Sparse3D{};
+        e.nitems = n3;
+        SItem3D *d_items3 = cv.take<SItem3D>(sizeof(SItem3D) * n3);
+        int2 *d_geo3 = cv.take<int2>(sizeof(int2) * n3);
+        SIO *d_io3 = cv.take<SIO>(sizeof(SIO) * n3);
+        int *d_blk3 = cv.take<int>(sizeof(int) * lay3.blk.size());
+        int *d_ublk3 = cv.take<int>(sizeof(int) * lay3.ublk.size());
+        e.items = d_items3;
+        e.geo = d_geo3;
+        e.io = d_io3;
+        e.blk = d_blk3;
+        e.nblk = (int)lay3.blk.size();
+        e.ublk = d_ublk3;
+        e.nublk = (int)lay3.ublk.size();
+        e.state[0] = cv.take<uint32_t>(sizeof(uint32_t) * nwords3);
+        e.state[1] = cv.take<uint32_t>(sizeof(uint32_t) * nwords3);
+        e.nwords = nwords3;
+        e.ctl = cv.take<int>(sizeof(int) * SC_INTS);
         for (int k = 0; k < 2; ++k) {
-            b.list3[k].item = cv.take<int>(sizeof(int) * w3);
-            b.list3[k].word = cv.take<uint32_t>(sizeof(uint32_t) * w3);
+            e.list[k].item = cv.take<int>(sizeof(int) * data3);
+            e.list[k].word = cv.take<uint32_t>(sizeof(uint32_t) * data3);
+            e.list[k].count = e.ctl ? e.ctl + SC_COUNT0 + k : nullptr;
         }
-        b.d_prot3 = cv.take<uint32_t>(sizeof(uint32_t) * w3);
-        b.d_changed3 = cv.take<int>(sizeof(int) * n3);
-        b.d_states3 = cv.take<int>(sizeof(int) * (size_t)n3 * ST_INTS);
-        // control words: [0,1] 2D cnt, [2] 2D wctr, [3] 2D gpass, [4] 2D live,
-        // [5] 3D iteration, [6] 3D live, [7,8] 3D list counts
+        e.prot = cv.take<uint32_t>(sizeof(uint32_t) * data3);
+        e.capacity = (int)data3;
+        e.states = cv.take<int>(sizeof(int) * (size_t)n3 * ST_INTS);
+        e.changed = cv.take<int>(sizeof(int) * n3);
+        // 2D control words: [0,1] queue counts, [2] next slot, [3] pass, [4] live
         int *ctl = b.d_ctl;
         b.q2.list[0] = lists2;
         b.q2.list[1] = lists2 ? lists2 + b.ntiles2 : nullptr;
@@ -412,10 +393,6 @@ static int prepare(Batch &b)
         b.q2.ntiles = (int)b.ntiles2;
         b.d_gpass2 = ctl ? ctl + 3 : nullptr;
         b.d_live2 = ctl ? ctl + 4 : nullptr;
-        b.d_iter3 = ctl ? ctl + 5 : nullptr;
-        b.d_live3 = ctl ? ctl + 6 : nullptr;
-        b.list3[0].count = ctl ? ctl + 7 : nullptr;
-        b.list3[1].count = ctl ? ctl + 8 : nullptr;
 
         std::vector<Item2D> h2(n2);
         for (int i = 0; i < n2; ++i) {
@@ -429,17 +406,11 @@ static int prepare(Batch &b)
             v.band_rows = p2[i].band_rows;
             v.tile0 = tile0[i];
         }
-        std::vector<SItem3D> h3(n3);
-        std::vector<IO3D> io3(n3);
+        std::vector<SIO> io3(n3);
         for (int i = 0; i < n3; ++i) {
-            SItem3D &v = h3[i];
-            const long words = off3[i + 1] - off3[i];
-            v.buf[0] = cv.take<uint32_t>(words * sizeof(uint32_t));
-            v.buf[1] = cv.take<uint32_t>(words * sizeof(uint32_t));
-            v.st = b.d_states3 ? b.d_states3 + (size_t)i * ST_INTS : nullptr;
-            v.w = b.in3[i].w; v.h = b.in3[i].h; v.d = b.in3[i].d; v.nh = nh3[i];
             io3[i] = b.in3[i].io;
-            io3[i].blk0 = blk0_3[i];
+            io3[i].blk0 = lay3.blk0[i];
+            io3[i].ublk0 = lay3.ublk0[i];
         }
         if (sweep == 0) {
             b.bytes = cv.used;
@@ -451,24 +422,20 @@ static int prepare(Batch &b)
             }
             continue;
         }
-        std::vector<IO2D> io2(n2);
-        for (int i = 0; i < n2; ++i) {
-            io2[i] = b.in2[i].io;
-            io2[i].blk0 = blk0_2[i];
-        }
-        b.words3 = w3;
         if (n2) {
             BGT_HIP(hipMemcpy(b.d_items2, h2.data(), sizeof(Item2D) * n2, hipMemcpyHostToDevice));
             BGT_HIP(hipMemcpy(b.d_io2, io2.data(), sizeof(IO2D) * n2, hipMemcpyHostToDevice));
             BGT_HIP(hipMemcpy(b.d_blk2, blk2.data(), sizeof(int) * blk2.size(), hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(b.d_ublk2, ublk2.data(), sizeof(int) * ublk2.size(), hipMemcpyHostToDevice));
             BGT_HIP(hipMemcpy(b.d_tiles2, t2.data(), sizeof(Tile2D) * t2.size(), hipMemcpyHostToDevice));
             BGT_HIP(hipMemcpy(b.d_q2, &b.q2, sizeof(Queue), hipMemcpyHostToDevice));
         }
         if (n3) {
-            BGT_HIP(hipMemcpy(b.d_items3, h3.data(), sizeof(SItem3D) * n3, hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(b.d_io3, io3.data(), sizeof(IO3D) * n3, hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(b.d_off3, off3.data(), sizeof(long) * (n3 + 1), hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(b.d_blk3, blk3.data(), sizeof(int) * blk3.size(), hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(d_items3, lay3.items.data(), sizeof(SItem3D) * n3, hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(d_geo3, lay3.geo.data(), sizeof(int2) * n3, hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(d_io3, io3.data(), sizeof(SIO) * n3, hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(d_blk3, lay3.blk.data(), sizeof(int) * lay3.blk.size(), hipMemcpyHostToDevice));
+            BGT_HIP(hipMemcpy(d_ublk3, lay3.ublk.data(), sizeof(int) * lay3.ublk.size(), hipMemcpyHostToDevice));
         }
     }
     if (n2)
@@ -519,13 +486,11 @@ static int launch_pass2d(Batch &b, hipStream_t s)
     return BGT_OK;
 }
 
-// one 3D iteration from list `cur` into list cur^1
-static int launch_iter3d(Batch &b, int cur, hipStream_t s)
+// one 3D iteration from list `cur` into list cur^1; `bound` caps the list count
+static int launch_iter3d(Batch &b, int cur, int bound, hipStream_t s)
 {
     BGT_CHECK(timed_mark(b, s, 1, true));
-    BGT_CHECK(sparse3d_iteration(b.d_items3, (int)b.in3.size(), b.list3[cur], b.list3[cur ^ 1],
-                                 (int)b.words3, b.d_prot3, b.d_changed3, b.d_iter3, b.d_live3,
-                                 sparse3d_grid(b.c3.blocks_per_cu), s));
+    BGT_CHECK(sparse3d_iteration(b.e3, cur, bound, sparse3d_grid(b.c3.blocks_per_cu), s));
     BGT_CHECK(timed_mark(b, s, 1, false));
     return BGT_OK;
 }
@@ -534,32 +499,60 @@ static int launch_iter3d(Batch &b, int cur, hipStream_t s)
 // on the same queue parity and can be captured once and replayed.
 static const int kChunk = 4;
 
+// 3D launch sizes follow the list-count bound rounded up to a power of two
+// of workgroups, so that a few captured graphs cover a whole run.
+static int bound_blocks3(int bound) { return pow2_ceil(std::max(1, (bound + 255) / 256)); }
+
+static int capture(hipGraphExec_t &graph, const std::function<int(hipStream_t)> &body)
+{
+    graph = nullptr;
+    hipStream_t cap = nullptr;
+    BGT_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    BGT_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    const int rc = body(cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cap, &g);
+    (void)hipStreamDestroy(cap);
+    if (rc != BGT_OK) return rc;
+    if (e != hipSuccess) return fail(BGT_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        graph = nullptr;
+        return fail(BGT_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    }
+    return BGT_OK;
+}
+
 static int launch_chunk(Batch &b, bool three_d, bool use_graph, hipStream_t s)
 {
+    const int bound3 = bound_blocks3(b.bound3) * 256;
     auto body = [&](hipStream_t st) -> int {
         for (int i = 0; i < kChunk; ++i)
-            BGT_CHECK(three_d ? launch_iter3d(b, i & 1, st) : launch_pass2d(b, st));
+            BGT_CHECK(three_d ? launch_iter3d(b, i & 1, bound3, st) : launch_pass2d(b, st));
         return BGT_OK;
     };
     if (!use_graph) return body(s);
-    hipGraphExec_t &graph = three_d ? b.graph3 : b.graph2;
-    hipStream_t &gs = three_d ? b.graph3_stream : b.graph2_stream;
-    if (!graph || gs != s) {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        graph = nullptr;
-        hipStream_t cap = nullptr;
-        BGT_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-        BGT_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-        const int rc = body(cap);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(cap, &g);
-        (void)hipStreamDestroy(cap);
-        if (rc != BGT_OK) return rc;
-        if (e != hipSuccess) return fail(BGT_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-        e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return fail(BGT_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
-        gs = s;
+    if (!three_d) {
+        if (!b.graph2 || b.graph2_stream != s) {
+            if (b.graph2) (void)hipGraphExecDestroy(b.graph2);
+            BGT_CHECK(capture(b.graph2, body));
+            b.graph2_stream = s;
+        }
+        BGT_HIP(hipGraphLaunch(b.graph2, s));
+        return BGT_OK;
+    }
+    if (b.graph3_stream != s) {
+        for (auto &g : b.graphs3) (void)hipGraphExecDestroy(g.second);
+        b.graphs3.clear();
+        b.graph3_stream = s;
+    }
+    hipGraphExec_t graph = nullptr;
+    for (auto &g : b.graphs3)
+        if (g.first == bound3) graph = g.second;
+    if (!graph) {
+        BGT_CHECK(capture(graph, body));
+        b.graphs3.push_back({bound3, graph});
     }
     BGT_HIP(hipGraphLaunch(graph, s));
     return BGT_OK;
@@ -594,22 +587,19 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     hipLaunchKernelGGL(batch_init, dim3((unsigned)((init_n + 255) / 256)), dim3(256), 0, s,
                        b.d_states2, (long)n2 * ST_INTS, b.d_occ, b.d_stamp, b.ntiles2, b.d_ctl,
                        b.nctl);
-    if (n3)
-        hipLaunchKernelGGL(batch_init, dim3((unsigned)((init_n + 255) / 256)), dim3(256), 0, s,
-                           b.d_states3, (long)n3 * ST_INTS, (uint8_t *)nullptr, (int *)nullptr,
-                           0L, b.d_changed3, n3);
-    hipLaunchKernelGGL(batch_set_live, dim3(1), dim3(64), 0, s, b.d_live2, n2, b.d_live3, n3);
+    hipLaunchKernelGGL(batch_set_live, dim3(1), dim3(64), 0, s, b.d_live2, n2);
     if (b.nblk2)
         hipLaunchKernelGGL(batch_pack2d, dim3(b.nblk2), dim3(256), 0, s, b.d_items2, b.d_io2,
                            b.d_blk2, b.d_occ);
     if (n2)
         hipLaunchKernelGGL(batch_build_list, dim3((unsigned)((b.ntiles2 + 255) / 256)), dim3(256),
                            0, s, b.q2);
-    if (b.nblk3)
-        hipLaunchKernelGGL(batch_pack3d, dim3(b.nblk3), dim3(256), 0, s, b.d_items3, b.d_io3,
-                           b.d_blk3);
     BGT_HIP(hipGetLastError());
-    if (n3) BGT_CHECK(sparse3d_build_list(b.d_items3, b.d_off3, n3, b.words3, b.list3[0], s));
+    if (n3) {
+        BGT_CHECK(sparse3d_begin(b.e3, !b.zeroed3, s));
+        b.zeroed3 = true;
+        b.bound3 = b.e3.capacity;
+    }
 
     // 3D pipeline on the side stream, forked from and joined into `s`
     BGT_HIP(hipEventRecord(b.ev_fork, s));
@@ -625,20 +615,23 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         if (live3) BGT_CHECK(launch_chunk(b, true, use_graph, b.side));
         if (live2) BGT_HIP(hipMemcpyAsync(b.h_live, b.d_live2, sizeof(int), hipMemcpyDeviceToHost, s));
         if (live3)
-            BGT_HIP(hipMemcpyAsync(b.h_live + 1, b.d_live3, sizeof(int), hipMemcpyDeviceToHost, b.side));
+            BGT_HIP(hipMemcpyAsync(b.h_live + 8, b.e3.ctl, SC_INTS * sizeof(int),
+                                   hipMemcpyDeviceToHost, b.side));
         if (live2) BGT_HIP(hipStreamSynchronize(s));
         if (live3) BGT_HIP(hipStreamSynchronize(b.side));
         if (live2 && b.h_live[0] <= 0) live2 = false;
-        if (live3 && b.h_live[1] <= 0) live3 = false;
+        if (live3) {
+            // chunks hold an even number of iterations: list 0 is current
+            b.bound3 = b.h_live[8 + SC_COUNT0];
+            if (b.h_live[8 + SC_LIVE] <= 0) live3 = false;
+        }
         if (++rounds > max_rounds) return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
     }
-    if (b.nblk3)
-        hipLaunchKernelGGL(batch_unpack3d, dim3(b.nblk3), dim3(256), 0, b.side, b.d_items3, b.d_io3,
-                           b.d_blk3);
+    if (n3) BGT_CHECK(sparse3d_end(b.e3, b.side));
     BGT_HIP(hipEventRecord(b.ev_join, b.side));
-    if (b.nblk2)
-        hipLaunchKernelGGL(batch_unpack2d, dim3(b.nblk2), dim3(256), 0, s, b.d_items2, b.d_io2,
-                           b.d_blk2);
+    if (b.nublk2)
+        hipLaunchKernelGGL(batch_unpack2d, dim3(b.nublk2), dim3(256), 0, s, b.d_items2, b.d_io2,
+                           b.d_ublk2);
     BGT_HIP(hipGetLastError());
     BGT_HIP(hipStreamWaitEvent(s, b.ev_join, 0));
     BGT_HIP(hipStreamSynchronize(s));
@@ -663,7 +656,7 @@ int batch_iterations(Batch &b, int *out)
         for (int i = 0; i < n2; ++i) out[i] = st[(size_t)i * ST_INTS + ST_ITERS];
     }
     if (n3) {
-        BGT_HIP(hipMemcpy(st.data(), b.d_states3, (size_t)n3 * ST_INTS * sizeof(int), hipMemcpyDeviceToHost));
+        BGT_HIP(hipMemcpy(st.data(), b.e3.states, (size_t)n3 * ST_INTS * sizeof(int), hipMemcpyDeviceToHost));
         for (int i = 0; i < n3; ++i) out[n2 + i] = st[(size_t)i * ST_INTS + ST_ITERS];
     }
     return BGT_OK;

@@ -70,3 +70,96 @@ __device__ __forceinline__ void unpack_run_u8(uint64_t v, uint8_t *p, int n)
 }
 
 }  // namespace bgt
+
+namespace bgt {
+
+// Coalesced unpacking: the caller's bytes of an item are cut into 16-byte
+// aligned chunks, one per thread, so the lanes of a wave store consecutive
+// uint4s.  A segment is `n` consecutive bytes at `base` holding pixels
+// f = 0..n-1 of lines of width w (pixel f is line f / w, x = f % w): a whole
+// contiguous item, or one row of a pitched image.  Chunks that lie inside the
+// segment and inside one line take one uint4 store; the rest (line ends,
+// segment ends, lines narrower than 16) store their in-segment bytes one by
+// one.  Bytes outside the segment are never written.
+//
+// Src provides   uint32_t bits16(line, x)  (16 pixels from x, x + 16 <= w)
+//        and     uint32_t bit(line, x).
+template <class Src>
+__device__ __forceinline__ void unpack_chunk_u8(uint8_t *base, uint32_t n, uint32_t w, uint32_t c,
+                                                const Src &src)
+{
+    uint8_t *C = reinterpret_cast<uint8_t *>(reinterpret_cast<uintptr_t>(base) & ~(uintptr_t)15) +
+                 (size_t)c * 16;
+    const long f0 = (long)(C - base);
+    if (f0 >= (long)n || f0 + 16 <= 0) return;
+    if (f0 >= 0 && f0 + 16 <= (long)n) {
+        const uint32_t line = (uint32_t)f0 / w, x0 = (uint32_t)f0 - line * w;
+        if (x0 + 16 <= w) {
+            const uint32_t b = src.bits16(line, x0);
+            *reinterpret_cast<uint4 *>(C) =
+                make_uint4(spread4(b), spread4(b >> 4), spread4(b >> 8), spread4(b >> 12));
+            return;
+        }
+    }
+    const uint32_t fa = (uint32_t)(f0 < 0 ? 0 : f0);
+    const uint32_t fb = (uint32_t)(f0 + 16 < (long)n ? f0 + 16 : (long)n);
+    uint32_t line = fa / w, x = fa - line * w;
+    for (uint32_t f = fa; f < fb; ++f) {
+        C[f - f0] = (uint8_t)src.bit(line, x);
+        if (++x == w) {
+            x = 0;
+            ++line;
+        }
+    }
+}
+
+// Chunks a segment of n bytes at base can touch.
+__host__ __device__ inline uint32_t unpack_chunks(uint32_t n) { return n / 16 + 2; }
+
+// Pixels of an image's rows in its packed state (64-bit words, L per row).
+struct RowBits {
+    const uint64_t *row0;
+    uint32_t L;
+    __device__ uint32_t bits16(uint32_t y, uint32_t x) const
+    {
+        const uint64_t *row = row0 + (size_t)y * L;
+        const uint32_t i = x >> 6, s = x & 63;
+        uint64_t v = row[i] >> s;
+        if (s > 48) v |= row[i + 1] << (64 - s);  // x + 16 <= width: word i+1 is in the row
+        return (uint32_t)v & 0xffffu;
+    }
+    __device__ uint32_t bit(uint32_t y, uint32_t x) const
+    {
+        return (uint32_t)(row0[(size_t)y * L + (x >> 6)] >> (x & 63)) & 1u;
+    }
+};
+
+// Chunk t of a w x h uint8 image at `out` (row pitch `pitch`) from its packed
+// rows: the whole image is one segment when `flat` (pitch == w), else each row
+// is a segment of `uchunks` chunks.
+__device__ __forceinline__ void unpack_image_u8(const uint64_t *buf, uint32_t L, int w, int h,
+                                                uint8_t *out, size_t pitch, uint32_t uchunks,
+                                                int flat, uint32_t t)
+{
+    if (flat) {
+        if (t >= uchunks) return;
+        unpack_chunk_u8(out, (uint32_t)w * (uint32_t)h, (uint32_t)w, t, RowBits{buf, L});
+    } else {
+        const uint32_t y = t / uchunks, c = t - y * uchunks;
+        if (y >= (uint32_t)h) return;
+        unpack_chunk_u8(out + (size_t)y * pitch, (uint32_t)w, (uint32_t)w, c,
+                        RowBits{buf + (size_t)y * L, L});
+    }
+}
+
+// Unpack grid of an image: segment mode and chunks per segment.
+inline void unpack_image_grid(int w, int h, size_t pitch, int &flat, uint32_t &uchunks,
+                              long &threads)
+{
+    const long px = (long)w * h;
+    flat = pitch == (size_t)w && px < 0x7fffffffL;
+    uchunks = unpack_chunks(flat ? (uint32_t)px : (uint32_t)w);
+    threads = flat ? (long)uchunks : (long)uchunks * h;
+}
+
+}  // namespace bgt

@@ -2,29 +2,29 @@
 //
 // Volumes on this path are sparse (a few percent foreground), so the work is
 // organised around the words (32 voxels of an x-line) that hold foreground:
-// a compacted list of (item, word) entries, kept in raster order inside each
-// block of the compaction so neighbouring lanes touch neighbouring words.
+// a compacted list of (item, word) entries, in raster order inside each
+// workgroup of the compaction so neighbouring lanes touch neighbouring words.
 // A subiteration is one launch over the list; every entry gathers its 3x3x3
-// word neighbourhood from the previous subiteration's buffer (the state is a
-// few tens of MB and stays in the caches), evaluates the rule and writes the
-// other buffer.  Empty words are zero in both buffers and can never change
-// (subiterations only clear bits), so the list only ever shrinks: after each
-// iteration, entries whose word became empty or whose item converged are
-// compacted away.  No tile, halo or neighbour bookkeeping is needed, and each
-// subiteration is synchronous over the whole volume, as in the reference's
-// semantics (3D/CUDA_ThinningKernel3D_Linear.cuh:739-782).
+// word neighbourhood from the previous subiteration's state array (the state
+// of a batch is a few tens of MB and stays in the caches), evaluates the rule
+// and writes the other array.  Empty words are zero in both arrays and can
+// never change (subiterations only clear bits), so the list only ever
+// shrinks: after each iteration, entries whose word became empty or whose item
+// converged are compacted away.  No tile, halo or neighbour bookkeeping is
+// needed, and each subiteration is synchronous over the whole volume, as in
+// the reference's semantics (3D/CUDA_ThinningKernel3D_Linear.cuh:739-782).
 //
 // An iteration is six launches, U (with the protection pre-test of all six
 // directions, 3D/CUDA_ThinningKernel3D_Linear.cuh:690-702), N, E, D, S, W,
-// alternating buffers 0 -> 1 -> 0 ..., so it ends in buffer 0, followed by a
-// one-workgroup finalize (per-item convergence) and the list compaction.
+// alternating arrays 0 -> 1 -> 0 ..., so it ends in array 0, and a compaction
+// launch that builds the next list and records per-item convergence.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cstring>
 #include <vector>
 
 #include "common.h"
+#include "pack.h"
 #include "sparse3d.h"
 #include "thin3d_kernels.h"
 
@@ -32,40 +32,143 @@ namespace bgt {
 
 namespace {
 
-__global__ void sparse3d_reset_count(int *count) { *count = 0; }
-
-__device__ __forceinline__ uint32_t fetch(const uint32_t *buf, const SItem3D &it, int y, int z,
-                                          int j)
+// Appends the words of a workgroup whose flags are set to `out`, keeping
+// lane order, with one atomic per workgroup.  Every thread of the block must
+// call it (flags may be 0).  Each thread offers up to two words (w0, w0 + 1).
+__device__ __forceinline__ void append2(bool f0, bool f1, int item, uint32_t w0, SList out)
 {
-    if ((unsigned)y >= (unsigned)it.h || (unsigned)z >= (unsigned)it.d || (unsigned)j >= (unsigned)it.nh)
-        return 0u;
-    return buf[((size_t)z * it.h + y) * it.nh + j];
+    __shared__ int s_cnt[4];
+    __shared__ int s_base;
+    const uint64_t m0 = __ballot(f0), m1 = __ballot(f1);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int pre = __popcll(m0 & lt) + __popcll(m1 & lt);
+    if (lane == 0) s_cnt[wv] = __popcll(m0) + __popcll(m1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int sum = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) sum += s_cnt[k];
+        s_base = sum ? atomicAdd(out.count, sum) : 0;
+    }
+    __syncthreads();
+    int pos = s_base + pre;
+    for (int k = 0; k < wv; ++k) pos += s_cnt[k];
+    if (f0) {
+        out.item[pos] = item;
+        out.word[pos] = w0;
+        ++pos;
+    }
+    if (f1) {
+        out.item[pos] = item;
+        out.word[pos] = w0 + 1;
+    }
 }
 
+__global__ void sparse3d_init(int *states, int *changed, int *ctl, int nitems)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nitems * ST_INTS) states[t] = 0;
+    if (t < nitems) changed[t] = 0;
+    if (t < SC_INTS) ctl[t] = t == SC_LIVE ? nitems : 0;
+}
+
+__global__ void sparse3d_zero(uint32_t *a, uint32_t *b, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        a[t] = 0u;
+        b[t] = 0u;
+    }
+}
+
+// One thread per 64 voxels (two data words) of an x-line: writes both state
+// arrays and lists the non-empty words (list 0).
+__global__ __launch_bounds__(256) void sparse3d_pack(const SItem3D *__restrict__ items,
+                                                     const SIO *__restrict__ io,
+                                                     const int *__restrict__ blk, uint32_t *s0,
+                                                     uint32_t *s1, SList out)
+{
+    const int i = blk ? blk[blockIdx.x] : 0;
+    const SItem3D it = items[i];
+    const SIO f = io[i];
+    const uint32_t nq = (uint32_t)it.nh >> 1;
+    const uint32_t t = (uint32_t)(blockIdx.x - f.blk0) * 256u + threadIdx.x;
+    uint32_t lo = 0, hi = 0, g = 0;
+    if (t < (uint32_t)it.h * (uint32_t)it.d * nq) {
+        const uint32_t line = t / nq;
+        const int x0 = (int)(t - line * nq) * 64;
+        const uint64_t v =
+            x0 < it.w ? pack_run_u8(f.in + (size_t)line * it.w + x0, min(64, it.w - x0)) : 0ull;
+        lo = (uint32_t)v;
+        hi = (uint32_t)(v >> 32);
+        const uint32_t y = line % (uint32_t)it.h, z = line / (uint32_t)it.h;
+        g = (uint32_t)it.base + ((z + 1) * (uint32_t)(it.h + 2) + y + 1) * (uint32_t)it.P + 1 +
+            (uint32_t)x0 / 32;
+        s0[g] = lo;
+        s0[g + 1] = hi;
+        s1[g] = lo;
+        s1[g + 1] = hi;
+    }
+    append2(lo != 0, hi != 0, i, g, out);
+}
+
+// Pixels of one z-plane of an item in state array 0.
+struct PlaneBits {
+    const uint32_t *s0;
+    uint32_t g0;  // word 0 of line y = 0
+    uint32_t P;
+    __device__ uint32_t bits16(uint32_t y, uint32_t x) const
+    {
+        const uint32_t j = g0 + y * P + (x >> 5);
+        const uint64_t v = ((uint64_t)s0[j] | ((uint64_t)s0[j + 1] << 32)) >> (x & 31);
+        return (uint32_t)v & 0xffffu;
+    }
+    __device__ uint32_t bit(uint32_t y, uint32_t x) const
+    {
+        return (s0[g0 + y * P + (x >> 5)] >> (x & 31)) & 1u;
+    }
+};
+
+// One thread per 16-byte chunk of a z-plane of the caller's buffer.
+__global__ __launch_bounds__(256) void sparse3d_unpack(const SItem3D *__restrict__ items,
+                                                       const SIO *__restrict__ io,
+                                                       const int *__restrict__ ublk,
+                                                       const uint32_t *__restrict__ s0)
+{
+    const int i = ublk ? ublk[blockIdx.x] : 0;
+    const SItem3D it = items[i];
+    const SIO f = io[i];
+    const uint32_t plane = (uint32_t)it.w * (uint32_t)it.h;
+    const uint32_t cpp = unpack_chunks(plane);
+    const uint32_t t = (uint32_t)(blockIdx.x - f.ublk0) * 256u + threadIdx.x;
+    const uint32_t z = t / cpp, c = t - z * cpp;
+    if (z >= (uint32_t)it.d) return;
+    const PlaneBits src{s0, (uint32_t)it.base + ((z + 1) * (uint32_t)(it.h + 2) + 1) * (uint32_t)it.P + 1,
+                        (uint32_t)it.P};
+    unpack_chunk_u8(f.out + (size_t)z * plane, plane, (uint32_t)it.w, c, src);
+}
+
+// Subiteration D (0..5 = U, N, E, D, S, W) over the list: src -> dst.
 template <int D>
-__global__ __launch_bounds__(256) void sparse3d_step(const SItem3D *__restrict__ items,
-                                                     const SList list, uint32_t *__restrict__ prot,
+__global__ __launch_bounds__(256) void sparse3d_step(const uint32_t *__restrict__ src,
+                                                     uint32_t *__restrict__ dst,
+                                                     const int2 *__restrict__ geo, const SList list,
+                                                     uint32_t *__restrict__ prot,
                                                      int *__restrict__ changed)
 {
     const int n = *list.count;
     const int stride = gridDim.x * blockDim.x;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
         const int ii = list.item[k];
-        const uint32_t w = list.word[k];
-        const SItem3D it = items[ii];
-        const uint32_t *src = (D & 1) ? it.buf[1] : it.buf[0];
-        uint32_t *dst = (D & 1) ? it.buf[0] : it.buf[1];
-        const int j = (int)(w % (uint32_t)it.nh);
-        const uint32_t line = w / (uint32_t)it.nh;
-        const int y = (int)(line % (uint32_t)it.h), z = (int)(line / (uint32_t)it.h);
+        const uint32_t g = list.word[k];
+        const int2 gp = geo[ii];
         uint32_t nb[27];
 #pragma unroll
         for (int dz = 0; dz < 3; ++dz)
 #pragma unroll
             for (int dy = 0; dy < 3; ++dy) {
-                const uint32_t a = fetch(src, it, y + dy - 1, z + dz - 1, j - 1);
-                const uint32_t b = fetch(src, it, y + dy - 1, z + dz - 1, j);
-                const uint32_t c = fetch(src, it, y + dy - 1, z + dz - 1, j + 1);
+                const uint32_t *r = src + (g + (uint32_t)((dz - 1) * gp.y + (dy - 1) * gp.x));
+                const uint32_t a = r[-1], b = r[0], c = r[1];
                 const int q = dz * 9 + dy * 3;
                 nb[q + 0] = __builtin_amdgcn_alignbit(b, a, 31);  // x-1
                 nb[q + 1] = b;
@@ -82,112 +185,113 @@ __global__ __launch_bounds__(256) void sparse3d_step(const SItem3D *__restrict__
         } else {
             del = rule3<D>(nb) & v & ~prot[k];
         }
-        dst[w] = v ^ del;
+        dst[g] = v ^ del;
         if (del) changed[ii] = 1;
     }
 }
 
-// Per-item convergence after an iteration: an item whose iteration deleted
-// nothing is done; its iteration count includes that last iteration.
-__global__ void sparse3d_finalize(const SItem3D *__restrict__ items, int nitems, int *changed,
-                                  int *iteration, int *live)
+// Keep the entries whose word is non-empty and whose item changed in this
+// iteration (an item whose iteration deleted nothing has converged).  A word
+// emptied by the last subiteration (W) still holds its bits from before that
+// step in array 1; once off the list nothing would rewrite it, and the next
+// iteration's odd steps read array 1, so it is cleared here.
+//
+// The last workgroup to finish then advances the items: each live item whose
+// iteration deleted nothing is done, with that iteration counted.  It reads
+// only what earlier launches wrote, and it clears `changed` and the input
+// list's count only after every other workgroup has finished reading them
+// (their block-counter increments follow their reads), so no fence is needed.
+__global__ __launch_bounds__(256) void sparse3d_compact(const uint32_t *__restrict__ s0,
+                                                        uint32_t *__restrict__ s1,
+                                                        int *__restrict__ changed, const SList in,
+                                                        SList out, int *__restrict__ states,
+                                                        int *__restrict__ ctl, int nitems)
 {
-    const int itn = *iteration;
+    __shared__ int s_last;
+    const int n = *in.count;
+    // Workgroups past the list take no part (the grid is sized by an upper
+    // bound of n); the others count themselves in to elect the last one.
+    const int active = n > 0 ? (n + 255) / 256 : 1;
+    if ((int)blockIdx.x >= active) return;
+    {
+        const int k = blockIdx.x * blockDim.x + threadIdx.x;
+        bool keep = false;
+        int ii = 0;
+        uint32_t g = 0;
+        if (k < n) {
+            ii = in.item[k];
+            g = in.word[k];
+            const bool live = changed[ii] != 0;
+            keep = live && s0[g] != 0u;
+            if (live && !keep) s1[g] = 0u;
+        }
+        append2(keep, false, ii, g, out);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(ctl + SC_BLOCKS, 1) == active - 1;
+    __syncthreads();
+    if (!s_last) return;
+    const int itn = ctl[SC_ITER];
     for (int i = threadIdx.x; i < nitems; i += blockDim.x) {
-        int *st = items[i].st;
-        if (st[ST_DONE]) continue;
-        if (!changed[i]) {
+        int *st = states + (size_t)i * ST_INTS;
+        if (!st[ST_DONE] && !changed[i]) {
             st[ST_DONE] = 1;
             st[ST_ITERS] = itn + 1;
             st[ST_PARITY] = 0;
-            atomicSub(live, 1);
+            atomicSub(ctl + SC_LIVE, 1);
         }
         changed[i] = 0;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) *iteration = itn + 1;
-}
-
-// Keep the entries whose word is non-empty and whose item is not done.  Each
-// block keeps the order of its 256 entries; blocks take output ranges with one
-// atomic each.
-__global__ __launch_bounds__(256) void sparse3d_compact(const SItem3D *__restrict__ items,
-                                                        const SList in, SList out)
-{
-    __shared__ int s_scan[256];
-    __shared__ int s_base;
-    const int n = *in.count;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x * blockDim.x >= n) return;
-    int keep = 0, ii = 0;
-    uint32_t w = 0;
-    if (k < n) {
-        ii = in.item[k];
-        w = in.word[k];
-        const SItem3D it = items[ii];
-        const bool live = !it.st[ST_DONE];
-        keep = live && it.buf[0][w] != 0u;
-        // A word emptied by the last subiteration (W) still holds its bits from
-        // before that step in buffer 1; once off the list nothing rewrites it,
-        // and the next iteration's odd steps read buffer 1.
-        if (live && !keep) it.buf[1][w] = 0u;
-    }
-    s_scan[threadIdx.x] = keep;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const int v = threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0;
-        __syncthreads();
-        s_scan[threadIdx.x] += v;
-        __syncthreads();
-    }
-    if (threadIdx.x == 255) s_base = s_scan[255] ? atomicAdd(out.count, s_scan[255]) : 0;
-    __syncthreads();
-    if (keep) {
-        const int pos = s_base + s_scan[threadIdx.x] - 1;
-        out.item[pos] = ii;
-        out.word[pos] = w;
-    }
-}
-
-// Initial list: every non-empty word of every item (grid over all words).
-__global__ __launch_bounds__(256) void sparse3d_build(const SItem3D *__restrict__ items,
-                                                      const long *__restrict__ off, int nitems,
-                                                      long total, SList out)
-{
-    __shared__ int s_scan[256];
-    __shared__ int s_base;
-    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int keep = 0, ii = 0;
-    uint32_t w = 0;
-    if (g < total) {
-        int lo = 0, hi = nitems - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (off[mid] <= g) lo = mid;
-            else hi = mid - 1;
-        }
-        ii = lo;
-        w = (uint32_t)(g - off[ii]);
-        keep = items[ii].buf[0][w] != 0u;
-    }
-    s_scan[threadIdx.x] = keep;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        const int v = threadIdx.x >= o ? s_scan[threadIdx.x - o] : 0;
-        __syncthreads();
-        s_scan[threadIdx.x] += v;
-        __syncthreads();
-    }
-    if (threadIdx.x == 255) s_base = s_scan[255] ? atomicAdd(out.count, s_scan[255]) : 0;
-    __syncthreads();
-    if (keep) {
-        const int pos = s_base + s_scan[threadIdx.x] - 1;
-        out.item[pos] = ii;
-        out.word[pos] = w;
+    if (threadIdx.x == 0) {
+        ctl[SC_ITER] = itn + 1;
+        ctl[SC_BLOCKS] = 0;
+        *in.count = 0;  // the next iteration compacts into this list
     }
 }
 
 }  // namespace
+
+int sparse3d_layout(const int *w, const int *h, const int *d, int n, Layout3D &lay)
+{
+    lay = Layout3D{};
+    lay.items.assign(n, SItem3D{});
+    lay.geo.assign(n, int2{0, 0});
+    lay.blk0.assign(n, 0);
+    lay.ublk0.assign(n, 0);
+    long base = 0, data = 0;
+    for (int i = 0; i < n; ++i) {
+        if (w[i] <= 0 || h[i] <= 0 || d[i] <= 0)
+            return fail(BGT_ERR_INVALID, "3D size must be >= 1 (got %d x %d x %d)", w[i], h[i], d[i]);
+        if ((long)w[i] * h[i] >= 0x7fffffffL)
+            return fail(BGT_ERR_UNSUPPORTED, "3D plane of %d x %d voxels exceeds 2^31", w[i], h[i]);
+        SItem3D &it = lay.items[i];
+        it.w = w[i];
+        it.h = h[i];
+        it.d = d[i];
+        it.nh = 2 * ((w[i] + 63) / 64);
+        const long P = it.nh + 2, plane = P * (h[i] + 2), total = plane * (d[i] + 2);
+        if (base + total >= 0x7fffffffL || plane >= 0x7fffffffL)
+            return fail(BGT_ERR_UNSUPPORTED, "3D state of %ld words exceeds 2^31", base + total);
+        it.P = (int)P;
+        it.plane = (int)plane;
+        it.base = base;
+        lay.geo[i] = int2{it.P, it.plane};
+        base += total;
+        data += (long)h[i] * d[i] * it.nh;
+        const long threads = (long)h[i] * d[i] * (it.nh / 2);
+        lay.blk0[i] = (int)lay.blk.size();
+        for (long k = 0; k < (threads + 255) / 256; ++k) lay.blk.push_back(i);
+        const long uthreads = (long)d[i] * unpack_chunks((uint32_t)(w[i] * h[i]));
+        lay.ublk0[i] = (int)lay.ublk.size();
+        for (long k = 0; k < (uthreads + 255) / 256; ++k) lay.ublk.push_back(i);
+    }
+    if (data >= 0x7fffffffL) return fail(BGT_ERR_UNSUPPORTED, "3D data exceeds 2^31 words");
+    if (lay.blk.size() >= 0x7fffffffUL || lay.ublk.size() >= 0x7fffffffUL)
+        return fail(BGT_ERR_UNSUPPORTED, "3D grid too large");
+    lay.nwords = base;
+    lay.data_words = data;
+    return BGT_OK;
+}
 
 int sparse3d_grid(int blocks_per_cu)
 {
@@ -206,32 +310,44 @@ int sparse3d_grid(int blocks_per_cu)
     return cus * (blocks_per_cu > 0 ? blocks_per_cu : occ);
 }
 
-int sparse3d_build_list(const SItem3D *items, const long *off, int nitems, long total_words,
-                        SList list, hipStream_t s)
+int sparse3d_begin(const Sparse3D &e, bool zero, hipStream_t s)
 {
-    if (total_words <= 0) return BGT_OK;
-    hipLaunchKernelGGL(sparse3d_build, dim3((unsigned)((total_words + 255) / 256)), dim3(256), 0, s,
-                       items, off, nitems, total_words, list);
+    const int ninit = std::max(e.nitems * ST_INTS, (int)SC_INTS);
+    hipLaunchKernelGGL(sparse3d_init, dim3((ninit + 255) / 256), dim3(256), 0, s, e.states,
+                       e.changed, e.ctl, e.nitems);
+    if (zero && e.nwords > 0)
+        hipLaunchKernelGGL(sparse3d_zero, dim3((unsigned)((e.nwords + 255) / 256)), dim3(256), 0, s,
+                           e.state[0], e.state[1], e.nwords);
+    if (e.nblk > 0)
+        hipLaunchKernelGGL(sparse3d_pack, dim3(e.nblk), dim3(256), 0, s, e.items, e.io, e.blk,
+                           e.state[0], e.state[1], e.list[0]);
     BGT_HIP(hipGetLastError());
     return BGT_OK;
 }
 
-int sparse3d_iteration(const SItem3D *items, int nitems, SList cur, SList next, int max_entries,
-                       uint32_t *prot, int *changed, int *iteration, int *live, int grid,
-                       hipStream_t s)
+int sparse3d_iteration(const Sparse3D &e, int cur, int bound, int grid, hipStream_t s)
 {
-    const dim3 g(grid), b(256);
-    hipLaunchKernelGGL(sparse3d_step<0>, g, b, 0, s, items, cur, prot, changed);
-    hipLaunchKernelGGL(sparse3d_step<1>, g, b, 0, s, items, cur, prot, changed);
-    hipLaunchKernelGGL(sparse3d_step<2>, g, b, 0, s, items, cur, prot, changed);
-    hipLaunchKernelGGL(sparse3d_step<3>, g, b, 0, s, items, cur, prot, changed);
-    hipLaunchKernelGGL(sparse3d_step<4>, g, b, 0, s, items, cur, prot, changed);
-    hipLaunchKernelGGL(sparse3d_step<5>, g, b, 0, s, items, cur, prot, changed);
-    hipLaunchKernelGGL(sparse3d_finalize, dim3(1), dim3(256), 0, s, items, nitems, changed,
-                       iteration, live);
-    hipLaunchKernelGGL(sparse3d_reset_count, dim3(1), dim3(1), 0, s, next.count);
-    hipLaunchKernelGGL(sparse3d_compact, dim3((unsigned)((max_entries + 255) / 256)), b, 0, s,
-                       items, cur, next);
+    const SList L = e.list[cur], N = e.list[cur ^ 1];
+    const int need = std::max(1, (bound + 255) / 256);
+    const dim3 g(std::min(grid, need)), b(256);
+    uint32_t *s0 = e.state[0], *s1 = e.state[1];
+    hipLaunchKernelGGL(sparse3d_step<0>, g, b, 0, s, s0, s1, e.geo, L, e.prot, e.changed);
+    hipLaunchKernelGGL(sparse3d_step<1>, g, b, 0, s, s1, s0, e.geo, L, e.prot, e.changed);
+    hipLaunchKernelGGL(sparse3d_step<2>, g, b, 0, s, s0, s1, e.geo, L, e.prot, e.changed);
+    hipLaunchKernelGGL(sparse3d_step<3>, g, b, 0, s, s1, s0, e.geo, L, e.prot, e.changed);
+    hipLaunchKernelGGL(sparse3d_step<4>, g, b, 0, s, s0, s1, e.geo, L, e.prot, e.changed);
+    hipLaunchKernelGGL(sparse3d_step<5>, g, b, 0, s, s1, s0, e.geo, L, e.prot, e.changed);
+    hipLaunchKernelGGL(sparse3d_compact, dim3(need), b, 0, s, s0, s1, e.changed, L, N, e.states,
+                       e.ctl, e.nitems);
+    BGT_HIP(hipGetLastError());
+    return BGT_OK;
+}
+
+int sparse3d_end(const Sparse3D &e, hipStream_t s)
+{
+    if (e.nublk > 0)
+        hipLaunchKernelGGL(sparse3d_unpack, dim3(e.nublk), dim3(256), 0, s, e.items, e.io, e.ublk,
+                           e.state[0]);
     BGT_HIP(hipGetLastError());
     return BGT_OK;
 }

@@ -78,18 +78,28 @@ __global__ void pack2d_t(const T *__restrict__ in, int w, int h, int L, uint64_t
     out[idx] = v;
 }
 
-// packed words (from the buffer the item's final parity selects) -> uint8 0/1
-__global__ void unpack2d_u8(const Item2D *__restrict__ item, uint8_t *__restrict__ out,
-                            size_t pitch)
+// packed words (from the buffer the item's final parity selects) -> uint8 0/1,
+// one thread per 16-byte chunk of the output
+__global__ __launch_bounds__(256) void unpack2d_u8(const Item2D *__restrict__ item,
+                                                   uint8_t *__restrict__ out, size_t pitch,
+                                                   uint32_t uchunks, int flat)
 {
     const Item2D it = *item;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)it.L * it.h) return;
-    const int y = (int)(idx / it.L), c = (int)(idx % it.L);
-    const int x0 = c * 64;
-    if (x0 >= it.w) return;
-    const uint64_t v = (it.st[ST_PARITY] ? it.buf[1] : it.buf[0])[idx];
-    unpack_run_u8(v, out + (size_t)y * pitch + x0, min(64, it.w - x0));
+    const uint64_t *buf = it.st[ST_PARITY] ? it.buf[1] : it.buf[0];
+    unpack_image_u8(buf, (uint32_t)it.L, it.w, it.h, out, pitch, uchunks, flat,
+                    blockIdx.x * 256u + threadIdx.x);
+}
+
+// launch of unpack2d_u8 for a w x h image with row pitch `pitch`
+static void launch_unpack2d_u8(const Item2D *d_item, uint8_t *out, int w, int h, size_t pitch,
+                               hipStream_t s)
+{
+    int flat = 0;
+    uint32_t uchunks = 0;
+    long threads = 0;
+    unpack_image_grid(w, h, pitch, flat, uchunks, threads);
+    hipLaunchKernelGGL(unpack2d_u8, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s,
+                       d_item, out, pitch, uchunks, flat);
 }
 
 template <typename T>
@@ -324,8 +334,7 @@ int thin2d_device_u8(const uint8_t *d_in, size_t in_pitch, int w, int h, uint8_t
                        in_pitch, w, h, pl.L, hv.buf[0]);
     BGT_HIP(hipGetLastError());
     BGT_CHECK(run_passes2d(cfg, pl, d_item, d_ctr, hv.st, s, iterations));
-    hipLaunchKernelGGL(unpack2d_u8, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, d_item,
-                       d_out, out_pitch);
+    launch_unpack2d_u8(d_item, d_out, w, h, out_pitch, s);
     BGT_HIP(hipGetLastError());
     BGT_HIP(hipStreamSynchronize(s));
     return BGT_OK;
@@ -358,7 +367,7 @@ static int thin2d_host_t(int w, int h, const T *in, OutT *out, const Config2D &c
     BGT_HIP(hipGetLastError());
     BGT_CHECK(run_passes2d(cfg, pl, d_item, d_ctr, hv.st, s, nullptr));
     if (std::is_same<OutT, uint8_t>::value)
-        hipLaunchKernelGGL(unpack2d_u8, g, b, 0, s, d_item, (uint8_t *)io.ptr, (size_t)w);
+        launch_unpack2d_u8(d_item, (uint8_t *)io.ptr, w, h, (size_t)w, s);
     else
         hipLaunchKernelGGL(unpack2d_t<OutT>, g, b, 0, s, d_item, (OutT *)io.ptr);
     BGT_HIP(hipGetLastError());

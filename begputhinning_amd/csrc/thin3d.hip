@@ -38,76 +38,33 @@ __device__ __forceinline__ bool fg3d<bool>(bool v)
 // per-volume path on the non-empty-word list engine
 // ---------------------------------------------------------------------------
 
-// one thread per 64 voxels (two state words) of an x-line
-__global__ void pack3d_both(const uint8_t *__restrict__ in, int w, long lines, int nh,
-                            uint32_t *__restrict__ b0, uint32_t *__restrict__ b1)
+// Thinning3D<T> input -> uint8 0/1 (v > 0 in T), and back (0/1 in T).
+template <typename T>
+__global__ void mask3d_from(const T *__restrict__ in, uint8_t *__restrict__ m, long n)
 {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int nq = nh >> 1;
-    if (idx >= lines * nq) return;
-    const long line = idx / nq;
-    const int x0 = (int)(idx - line * nq) * 64;
-    const uint64_t v = x0 < w ? pack_run_u8(in + (size_t)line * w + x0, min(64, w - x0)) : 0ull;
-    reinterpret_cast<uint64_t *>(b0)[idx] = v;
-    reinterpret_cast<uint64_t *>(b1)[idx] = v;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) m[t] = fg3d<T>(in[t]) ? 1 : 0;
 }
 
 template <typename T>
-__global__ void pack3d_both_t(const T *__restrict__ in, int w, long lines, int nh,
-                              uint32_t *__restrict__ b0, uint32_t *__restrict__ b1)
+__global__ void mask3d_to(const uint8_t *__restrict__ m, T *__restrict__ out, long n)
 {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= lines * nh) return;
-    const long line = idx / nh;
-    const int j = (int)(idx % nh);
-    const T *row = in + (size_t)line * w;
-    uint32_t v = 0;
-    for (int k = 0; k < 32 && j * 32 + k < w; ++k)
-        if (fg3d<T>(row[j * 32 + k])) v |= 1u << k;
-    b0[idx] = v;
-    b1[idx] = v;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = m[t] ? (T)1 : (T)0;
 }
 
-template <typename T>
-__global__ void unpack3d_sparse(const uint32_t *__restrict__ b0, int w, long lines, int nh,
-                                T *__restrict__ out)
+__global__ void init_one3d(SItem3D *item, int2 *geo, SIO *io, SItem3D v, SIO f)
 {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= lines * nh) return;
-    const long line = idx / nh;
-    const int j = (int)(idx % nh);
-    const uint32_t v = b0[idx];
-    T *row = out + (size_t)line * w;
-    for (int k = 0; k < 32 && j * 32 + k < w; ++k) row[j * 32 + k] = ((v >> k) & 1) ? (T)1 : (T)0;
-}
-
-// uint8 output: one thread per 64 voxels of an x-line
-__global__ void unpack3d_u8(const uint32_t *__restrict__ b0, int w, long lines, int nh,
-                            uint8_t *__restrict__ out)
-{
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int nq = nh >> 1;
-    if (idx >= lines * nq) return;
-    const long line = idx / nq;
-    const int x0 = (int)(idx - line * nq) * 64;
-    if (x0 >= w) return;
-    unpack_run_u8(reinterpret_cast<const uint64_t *>(b0)[idx], out + (size_t)line * w + x0,
-                  min(64, w - x0));
-}
-
-__global__ void init_sparse3d(SItem3D *item, SItem3D val, int *ctl)
-{
-    // ctl: [0] changed, [1] iteration, [2] live, [3] count A, [4] count B
-    const int t = threadIdx.x;
-    if (t == 0) *item = val;
-    if (t < ST_INTS) val.st[t] = 0;
-    if (t < 5) ctl[t] = t == 2 ? 1 : 0;
+    if (threadIdx.x == 0) {
+        *item = v;
+        *geo = int2{v.P, v.plane};
+        *io = f;
+    }
 }
 
 int check_config3d(const Config3D &c)
 {
-    if (c.blocks_per_cu < 0 || c.blocks_per_cu > 32 || c.max_chunk < 2 || c.max_chunk > 64 ||
-        (c.max_chunk & 1))
+    if (c.blocks_per_cu < 0 || c.blocks_per_cu > 32 || c.max_chunk < 1 || c.max_chunk > 64)
         return fail(BGT_ERR_INVALID, "bad 3D config (blocks_per_cu=%d max_chunk=%d)",
                     c.blocks_per_cu, c.max_chunk);
     return BGT_OK;
@@ -117,97 +74,99 @@ static int *poll_buffer3()
 {
     thread_local int *p = nullptr;
     if (!p) {
-        if (hipHostMalloc((void **)&p, 64, hipHostMallocDefault) != hipSuccess) p = nullptr;
+        if (hipHostMalloc((void **)&p, 256, hipHostMallocDefault) != hipSuccess) p = nullptr;
     }
     return p;
 }
 
-struct SparseVol {
-    SItem3D *d_item;
-    SItem3D host;
-    SList list[2];
-    uint32_t *prot;
-    int *ctl;
-    long words;
-    int capacity;
+// Engine for one volume in a workspace: [item|geo|io] [states+ctl] [changed]
+// [state 0] [state 1] [list 0 item|word] [list 1 item|word] [prot].
+struct OneVolume {
+    Sparse3D e;
+    SItem3D item;
 };
 
-static size_t sparse_bytes(long words)
+static size_t one_volume_bytes(long nwords, long data)
 {
-    return round_up(sizeof(SItem3D), 256) + round_up(ST_INTS * sizeof(int), 256) +
-           round_up(8 * sizeof(int), 256) + 2 * round_up(words * sizeof(uint32_t), 256) +
-           2 * (round_up(words * sizeof(int), 256) + round_up(words * sizeof(uint32_t), 256)) +
-           round_up(words * sizeof(uint32_t), 256) + round_up(sizeof(long) * 2, 256);
+    return 3 * 256 + round_up((ST_INTS + SC_INTS) * sizeof(int), 256) + 256 +
+           2 * round_up(nwords * sizeof(uint32_t), 256) +
+           4 * round_up(data * sizeof(uint32_t), 256) + round_up(data * sizeof(uint32_t), 256);
 }
 
-static int sparse_setup(int w, int h, int d, void *ws, hipStream_t s, SparseVol &sv)
+static int one_volume_setup(int w, int h, int d, const uint8_t *in, uint8_t *out, Workspace &ws,
+                            hipStream_t s, OneVolume &ov)
 {
-    const int nh = 2 * ((w + 63) / 64);
-    const long words = (long)nh * h * d;
-    if (words > 0x7fffffffL) return fail(BGT_ERR_UNSUPPORTED, "3D volume too large");
-    char *p = (char *)ws;
+    Layout3D lay;
+    BGT_CHECK(sparse3d_layout(&w, &h, &d, 1, lay));
+    const long nwords = lay.nwords, data = lay.data_words;
+    BGT_CHECK(workspace_reserve(ws, one_volume_bytes(nwords, data)));
+    char *p = (char *)ws.ptr;
     auto take = [&](size_t bytes) {
         char *r = p;
-        p += round_up(bytes, 256);
+        p += round_up(bytes ? bytes : 1, 256);
         return r;
     };
-    sv.d_item = (SItem3D *)take(sizeof(SItem3D));
-    int *st = (int *)take(ST_INTS * sizeof(int));
-    sv.ctl = (int *)take(8 * sizeof(int));
-    SItem3D v{};
-    v.buf[0] = (uint32_t *)take(words * sizeof(uint32_t));
-    v.buf[1] = (uint32_t *)take(words * sizeof(uint32_t));
-    v.st = st;
-    v.w = w; v.h = h; v.d = d; v.nh = nh;
+    Sparse3D &e = ov.e;
+    e = Sparse3D{};
+    e.nitems = 1;
+    SItem3D *d_item = (SItem3D *)take(sizeof(SItem3D));
+    int2 *d_geo = (int2 *)take(sizeof(int2));
+    SIO *d_io = (SIO *)take(sizeof(SIO));
+    e.items = d_item;
+    e.geo = d_geo;
+    e.io = d_io;
+    e.blk = nullptr;  // every block belongs to item 0
+    e.nblk = (int)lay.blk.size();
+    e.ublk = nullptr;
+    e.nublk = (int)lay.ublk.size();
+    e.states = (int *)take((ST_INTS + SC_INTS) * sizeof(int));
+    e.ctl = e.states + ST_INTS;  // polled together with the item state
+    e.changed = (int *)take(sizeof(int));
+    e.state[0] = (uint32_t *)take(nwords * sizeof(uint32_t));
+    e.state[1] = (uint32_t *)take(nwords * sizeof(uint32_t));
+    e.nwords = nwords;
     for (int k = 0; k < 2; ++k) {
-        sv.list[k].item = (int *)take(words * sizeof(int));
-        sv.list[k].word = (uint32_t *)take(words * sizeof(uint32_t));
-        sv.list[k].count = sv.ctl + 3 + k;
+        e.list[k].item = (int *)take(data * sizeof(int));
+        e.list[k].word = (uint32_t *)take(data * sizeof(uint32_t));
+        e.list[k].count = e.ctl + SC_COUNT0 + k;
     }
-    sv.prot = (uint32_t *)take(words * sizeof(uint32_t));
-    long *off = (long *)take(sizeof(long) * 2);
-    (void)off;
-    sv.host = v;
-    sv.words = words;
-    sv.capacity = (int)words;
-    hipLaunchKernelGGL(init_sparse3d, dim3(1), dim3(64), 0, s, sv.d_item, v, sv.ctl);
+    e.prot = (uint32_t *)take(data * sizeof(uint32_t));
+    e.capacity = (int)data;
+    ov.item = lay.items[0];
+    hipLaunchKernelGGL(init_one3d, dim3(1), dim3(64), 0, s, d_item, d_geo, d_io, lay.items[0],
+                       SIO{in, out, 0, 0});
     BGT_HIP(hipGetLastError());
     return BGT_OK;
 }
 
-__global__ void set_offsets(long *off, long total)
+// Iterate one volume to convergence, polling the item state every chunk of
+// iterations (2, 4, ... up to max_chunk).
+static int one_volume_run(OneVolume &ov, const Config3D &cfg, hipStream_t s, int *iterations)
 {
-    off[0] = 0;
-    off[1] = total;
-}
-
-static int sparse_run(SparseVol &sv, const Config3D &cfg, hipStream_t s, int *iterations)
-{
-    // offsets for the list builder (one item)
-    long *off = (long *)((char *)sv.prot + round_up(sv.words * sizeof(uint32_t), 256));
-    hipLaunchKernelGGL(set_offsets, dim3(1), dim3(1), 0, s, off, sv.words);
-    BGT_CHECK(sparse3d_build_list(sv.d_item, off, 1, sv.words, sv.list[0], s));
+    const Sparse3D &e = ov.e;
+    BGT_CHECK(sparse3d_begin(e, true, s));
     int *hp = poll_buffer3();
     if (!hp) return fail(BGT_ERR_NOMEM, "pinned poll buffer allocation failed");
-    const long max_iter = (long)sv.host.w * sv.host.h * sv.host.d + 2;
-    long done_iters = 0;
+    const long max_iter = (long)ov.item.w * ov.item.h * ov.item.d + 2;
     const int grid = sparse3d_grid(cfg.blocks_per_cu);
-    int cur = 0, chunk = 2;
+    long done_iters = 0;
+    int cur = 0, chunk = std::min(2, cfg.max_chunk), bound = e.capacity;
     for (;;) {
         for (int i = 0; i < chunk; ++i) {
-            BGT_CHECK(sparse3d_iteration(sv.d_item, 1, sv.list[cur], sv.list[cur ^ 1], sv.capacity,
-                                         sv.prot, sv.ctl + 0, sv.ctl + 1, sv.ctl + 2, grid, s));
+            BGT_CHECK(sparse3d_iteration(e, cur, bound, grid, s));
             cur ^= 1;
         }
         done_iters += chunk;
-        BGT_HIP(hipMemcpyAsync(hp, sv.host.st, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
+        BGT_HIP(hipMemcpyAsync(hp, e.states, (ST_INTS + SC_INTS) * sizeof(int),
+                               hipMemcpyDeviceToHost, s));
         BGT_HIP(hipStreamSynchronize(s));
         if (hp[ST_DONE]) break;
         if (done_iters > max_iter) return fail(BGT_ERR_NOT_CONVERGED, "3D thinning did not converge");
+        bound = hp[ST_INTS + SC_COUNT0 + cur];
         chunk = std::min(chunk * 2, cfg.max_chunk);
     }
     if (iterations) *iterations = hp[ST_ITERS];
-    return BGT_OK;
+    return sparse3d_end(e, s);
 }
 
 int thin3d_device_u8(const uint8_t *d_in, int w, int h, int d, uint8_t *d_out, hipStream_t s,
@@ -217,19 +176,9 @@ int thin3d_device_u8(const uint8_t *d_in, int w, int h, int d, uint8_t *d_out, h
     if (!d_in || !d_out) return fail(BGT_ERR_INVALID, "null device pointer");
     if (w <= 0 || h <= 0 || d <= 0)
         return fail(BGT_ERR_INVALID, "3D size must be >= 1 (got %d x %d x %d)", w, h, d);
-    const long words = (long)(2 * ((w + 63) / 64)) * h * d;
-    Workspace &ws = thread_workspace(2);
-    BGT_CHECK(workspace_reserve(ws, sparse_bytes(words)));
-    SparseVol sv;
-    BGT_CHECK(sparse_setup(w, h, d, ws.ptr, s, sv));
-    const long lines = (long)h * d;
-    const dim3 g((unsigned)((words / 2 + 255) / 256)), b(256);
-    hipLaunchKernelGGL(pack3d_both, g, b, 0, s, d_in, w, lines, sv.host.nh, sv.host.buf[0],
-                       sv.host.buf[1]);
-    BGT_HIP(hipGetLastError());
-    BGT_CHECK(sparse_run(sv, cfg, s, iterations));
-    hipLaunchKernelGGL(unpack3d_u8, g, b, 0, s, sv.host.buf[0], w, lines, sv.host.nh, d_out);
-    BGT_HIP(hipGetLastError());
+    OneVolume ov;
+    BGT_CHECK(one_volume_setup(w, h, d, d_in, d_out, thread_workspace(2), s, ov));
+    BGT_CHECK(one_volume_run(ov, cfg, s, iterations));
     BGT_HIP(hipStreamSynchronize(s));
     return BGT_OK;
 }
@@ -240,24 +189,22 @@ static int thin3d_host_t(int w, int h, int d, const T *in, T *out, const Config3
     BGT_CHECK(check_config3d(cfg));
     if (w <= 0 || h <= 0 || d <= 0)
         return fail(BGT_ERR_INVALID, "3D size must be >= 1 (got %d x %d x %d)", w, h, d);
-    const size_t bytes = (size_t)w * h * d * sizeof(T);
-    const long words = (long)(2 * ((w + 63) / 64)) * h * d;
-    Workspace &ws = thread_workspace(2);
+    const long n = (long)w * h * d;
+    const size_t bytes = (size_t)n * sizeof(T);
+    // uint8 and bool inputs are their own mask (nonzero == v > 0)
+    const bool direct = std::is_same<T, uint8_t>::value || std::is_same<T, bool>::value;
     Workspace &io = thread_workspace(3);
-    BGT_CHECK(workspace_reserve(ws, sparse_bytes(words)));
-    BGT_CHECK(workspace_reserve(io, bytes));
+    BGT_CHECK(workspace_reserve(io, round_up(bytes, 256) + (direct ? 0 : (size_t)n)));
     hipStream_t s = nullptr;
     BGT_HIP(hipMemcpyAsync(io.ptr, in, bytes, hipMemcpyHostToDevice, s));
-    SparseVol sv;
-    BGT_CHECK(sparse_setup(w, h, d, ws.ptr, s, sv));
-    const long lines = (long)h * d;
-    const dim3 g((unsigned)((words + 255) / 256)), b(256);
-    hipLaunchKernelGGL(pack3d_both_t<T>, g, b, 0, s, (const T *)io.ptr, w, lines, sv.host.nh,
-                       sv.host.buf[0], sv.host.buf[1]);
+    uint8_t *mask = direct ? (uint8_t *)io.ptr : (uint8_t *)io.ptr + round_up(bytes, 256);
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+    if (!direct) hipLaunchKernelGGL(mask3d_from<T>, g, b, 0, s, (const T *)io.ptr, mask, n);
     BGT_HIP(hipGetLastError());
-    BGT_CHECK(sparse_run(sv, cfg, s, nullptr));
-    hipLaunchKernelGGL(unpack3d_sparse<T>, g, b, 0, s, sv.host.buf[0], w, lines, sv.host.nh,
-                       (T *)io.ptr);
+    OneVolume ov;
+    BGT_CHECK(one_volume_setup(w, h, d, mask, mask, thread_workspace(2), s, ov));
+    BGT_CHECK(one_volume_run(ov, cfg, s, nullptr));
+    if (!direct) hipLaunchKernelGGL(mask3d_to<T>, g, b, 0, s, mask, (T *)io.ptr, n);
     BGT_HIP(hipGetLastError());
     BGT_HIP(hipMemcpyAsync(out, io.ptr, bytes, hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));

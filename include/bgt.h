@@ -44,10 +44,12 @@ int bgt_version(void);
 /* Launch shapes (no reference counterpart: the reference hard-codes them,
  * 2D/LibThinning2D.cu:59-60, 3D/LibThinning3D.cu:50).  2D: waves per
  * workgroup (1-4), register rows per lane (8 or 16), iterations fused per
- * HBM pass (1-16).  3D: waves per workgroup (1-8), iterations per pass (1-5).
- * Applies to later calls of the calling process. */
+ * HBM pass (1-16).  3D: workgroups per CU of the subiteration kernel (0-32,
+ * 0 = the occupancy limit) and the most iterations queued between two host
+ * convergence polls (1-64).  Applies to later calls of the calling
+ * process and to batches created afterwards. */
 int bgt_set_config2d(int waves, int rows, int iters_per_pass);
-int bgt_set_config3d(int waves, int iters_per_pass);
+int bgt_set_config3d(int blocks_per_cu, int max_chunk);
 
 /* ------------------------------------------------------------------------- */
 /* Drop-in thinning on host buffers.                                          */
