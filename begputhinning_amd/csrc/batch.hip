@@ -256,7 +256,8 @@ struct Batch {
     int *d_stamp = nullptr;
     long ntiles2 = 0;
 
-    int *h_live = nullptr;  // pinned: [0] 2D live, [1] 3D live
+    int *h_live = nullptr;  // pinned, two slots of 32: [0] 2D live, [8..] 3D control words
+    hipEvent_t ev_poll2[2] = {nullptr, nullptr}, ev_poll3[2] = {nullptr, nullptr};
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
@@ -274,6 +275,10 @@ struct Batch {
         if (side) (void)hipStreamDestroy(side);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        for (int k = 0; k < 2; ++k) {
+            if (ev_poll2[k]) (void)hipEventDestroy(ev_poll2[k]);
+            if (ev_poll3[k]) (void)hipEventDestroy(ev_poll3[k]);
+        }
         if (h_live) (void)hipHostFree(h_live);
     }
     void release()
@@ -442,11 +447,15 @@ static int prepare(Batch &b)
         b.grid2 = occupancy_grid(b.c2.rows == 8 ? (const void *)thin2d_queue_pass<8>
                                                 : (const void *)thin2d_queue_pass<16>,
                                  b.c2.waves * 64, lds2, (int)b.ntiles2);
-    if (!b.h_live && hipHostMalloc((void **)&b.h_live, 64, hipHostMallocDefault) != hipSuccess)
+    if (!b.h_live && hipHostMalloc((void **)&b.h_live, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(BGT_ERR_NOMEM, "pinned allocation failed");
     if (!b.side) BGT_HIP(hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking));
     if (!b.ev_fork) BGT_HIP(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
     if (!b.ev_join) BGT_HIP(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
+    for (int k = 0; k < 2; ++k) {
+        if (!b.ev_poll2[k]) BGT_HIP(hipEventCreateWithFlags(&b.ev_poll2[k], hipEventDisableTiming));
+        if (!b.ev_poll3[k]) BGT_HIP(hipEventCreateWithFlags(&b.ev_poll3[k], hipEventDisableTiming));
+    }
     b.ready = true;
     return BGT_OK;
 }
@@ -610,21 +619,40 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     for (auto &v : b.in3) max_vox = std::max<long>(max_vox, (long)v.w * v.h * v.d);
     const long max_rounds = max_vox / kChunk + 2;
     long rounds = 0;
+    // Chunks are pipelined: chunk k+1 is queued before the counters copied
+    // after chunk k are read, so the GPU does not wait for the host.  A chunk
+    // queued after the last item converged finds empty queues and lists.
+    int slot = 0;
+    bool prev2 = false, prev3 = false;  // a poll of the previous chunk is pending
     while (live2 || live3) {
-        if (live2) BGT_CHECK(launch_chunk(b, false, use_graph, s));
-        if (live3) BGT_CHECK(launch_chunk(b, true, use_graph, b.side));
-        if (live2) BGT_HIP(hipMemcpyAsync(b.h_live, b.d_live2, sizeof(int), hipMemcpyDeviceToHost, s));
-        if (live3)
-            BGT_HIP(hipMemcpyAsync(b.h_live + 8, b.e3.ctl, SC_INTS * sizeof(int),
-                                   hipMemcpyDeviceToHost, b.side));
-        if (live2) BGT_HIP(hipStreamSynchronize(s));
-        if (live3) BGT_HIP(hipStreamSynchronize(b.side));
-        if (live2 && b.h_live[0] <= 0) live2 = false;
-        if (live3) {
-            // chunks hold an even number of iterations: list 0 is current
-            b.bound3 = b.h_live[8 + SC_COUNT0];
-            if (b.h_live[8 + SC_LIVE] <= 0) live3 = false;
+        int *hs = b.h_live + slot * 32;
+        if (live2) {
+            BGT_CHECK(launch_chunk(b, false, use_graph, s));
+            BGT_HIP(hipMemcpyAsync(hs, b.d_live2, sizeof(int), hipMemcpyDeviceToHost, s));
+            BGT_HIP(hipEventRecord(b.ev_poll2[slot], s));
         }
+        if (live3) {
+            BGT_CHECK(launch_chunk(b, true, use_graph, b.side));
+            BGT_HIP(hipMemcpyAsync(hs + 8, b.e3.ctl, SC_INTS * sizeof(int), hipMemcpyDeviceToHost,
+                                   b.side));
+            BGT_HIP(hipEventRecord(b.ev_poll3[slot], b.side));
+        }
+        const int prev = slot ^ 1;
+        const int *hp = b.h_live + prev * 32;
+        const bool launched2 = live2, launched3 = live3;
+        if (live2 && prev2) {
+            BGT_HIP(hipEventSynchronize(b.ev_poll2[prev]));
+            if (hp[0] <= 0) live2 = false;
+        }
+        if (live3 && prev3) {
+            BGT_HIP(hipEventSynchronize(b.ev_poll3[prev]));
+            // chunks hold an even number of iterations: list 0 is current
+            b.bound3 = hp[8 + SC_COUNT0];
+            if (hp[8 + SC_LIVE] <= 0) live3 = false;
+        }
+        prev2 = launched2;
+        prev3 = launched3;
+        slot = prev;
         if (++rounds > max_rounds) return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
     }
     if (n3) BGT_CHECK(sparse3d_end(b.e3, b.side));
