@@ -4,12 +4,12 @@ Workload (default): the mixed batch -- 512 images with sides U{128..2048}
 (polyline content, 60 polylines of width 3-15 px, 0.05% salt) and 16 volumes
 with sides U{64..384} (vessel trees + 20 ellipsoids, ~3% foreground), seeded,
 uint8 {0,1}, all resident in HBM before timing.  One step = thinning the whole
-batch to convergence (pack, fused passes, unpack) with one batched launch
-sequence.  Reported beside it: the same items looped through the plain
-per-image drop-in path, p50/p99 latency of 1024x1024 tree frames through the
-per-image path, the dominant kernel's roofline, and the CPU baseline (the
-oracle, the reference's rules restated word-parallel in C, one core) on the
-whole workload, which also checks every skeleton bit for bit.
+batch to convergence (pack, passes, unpack) with one batched launch sequence.
+Reported beside it: the same items looped through the plain per-image
+drop-in path, p50/p99 latency of 1024x1024 tree frames through the per-image
+path, the dominant kernel's roofline, and the CPU baseline (the oracle, the
+reference's rules restated word-parallel in C, one core) on the whole
+workload, which also checks every skeleton bit for bit.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
@@ -17,6 +17,7 @@ import argparse
 import hashlib
 import json
 import os
+import re
 import statistics
 import sys
 import time
@@ -28,6 +29,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+DOMINANT = "thin2d_queue_pass"
+# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by a copy of known size (see the file for the runs)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "thin2d_queue_pass.mixed_batch.pmc.txt")
 
 
 def log(*a):
@@ -47,6 +52,15 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()[:16]
 
 
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary, or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(TRAFFIC_FILE).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,10 +69,17 @@ def main():
     ap.add_argument("--frames", type=int, default=200, help="1024x1024 frames for p50/p99")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", type=int, default=0, help="replay pass chunks as a hipGraph")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="time the batched step only (no per-image loop, latency or CPU leg)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a kernel of "
+                         "known traffic, to scale hardware byte counters)")
     args = ap.parse_args()
+    if args.only_batch:
+        args.no_cpu_baseline = True
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
-              "(items shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+              "(items shard across GPUs with no communication)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -76,8 +97,16 @@ def main():
     ins = [torch.from_numpy(a).to(dev) for a in items]
     outs = [torch.empty_like(t) for t in ins]
     torch.cuda.synchronize()
+    if args.copy_probe_mib:
+        src = torch.ones(args.copy_probe_mib << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mib} MiB device-to-device")
+        del src, dst
 
-    # ---- batched path ------------------------------------------------------
+    # ---- batched path: the timed steps -------------------------------------
     batch = ThinningBatch()
     for t, o in zip(ins, outs):
         if t.dim() == 2:
@@ -87,7 +116,6 @@ def main():
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         batch.run(stream, use_graph=bool(args.graph))
-    torch.cuda.synchronize()
     step_ms = []
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -107,7 +135,8 @@ def main():
     log(f"batched: {ms_step:.2f} ms/step (events median {statistics.median(step_ms):.2f} ms), "
         f"{value:.0f} Mvoxel/s")
 
-    # kernel timing (HIP events around every pass launch on the run's stream)
+    # ---- kernel timing: HIP events around every launch of the 2D pass kernel
+    # and every 3D iteration, on the streams they run on, in one more step
     batch.set_timing(True)
     ev0.record(stream)
     batch.run(stream, use_graph=False)
@@ -119,57 +148,64 @@ def main():
     n2d_vox = [a.size for a in items[:n2]]
     n3d_vox = [a.size for a in items[n2:]]
     it2, it3 = iters[:n2], iters[n2:]
+    # algorithmic bytes: every iteration reads and writes the packed state once
     bytes_iter_2d = sum(2 * ((n + 7) // 8) * int(i) for n, i in zip(n2d_vox, it2))
     bytes_iter_3d = sum(2 * ((n + 7) // 8) * int(i) for n, i in zip(n3d_vox, it3))
     b_alg = 2 * total_vox + bytes_iter_2d + bytes_iter_3d
-    dom = "thin2d_pass" if ms2d >= ms3d else "thin3d_pass"
-    dom_bytes, dom_ms, dom_launches = ((bytes_iter_2d, ms2d, l2d) if dom == "thin2d_pass"
-                                       else (bytes_iter_3d, ms3d, l3d))
-    achieved = dom_bytes / (dom_ms / 1e3) if dom_ms > 0 else 0.0
-    log(f"kernels: 2D pass {ms2d:.2f} ms / {l2d} launches, 3D pass {ms3d:.2f} ms / {l3d} "
-        f"launches; timed step {timed_step_ms:.2f} ms")
+    avg_launch_s = ms2d / 1e3 / max(l2d, 1)
+    bytes_per_launch = bytes_iter_2d / max(l2d, 1)
+    achieved = bytes_per_launch / avg_launch_s if avg_launch_s > 0 else 0.0
+    log(f"kernels: {DOMINANT} {ms2d:.2f} ms / {l2d} launches, 3D iterations {ms3d:.2f} ms / "
+        f"{l3d}; timed step {timed_step_ms:.2f} ms")
 
-    # ---- per-image drop-in path on the same items ---------------------------
-    outs_pi = [torch.empty_like(t) for t in ins]
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for t, o in zip(ins, outs_pi):
-        if t.dim() == 2:
-            B.thin2d_device(t, out=o)
-        else:
-            B.thin3d_device(t, out=o)
-    torch.cuda.synchronize()
-    per_image_s = time.perf_counter() - t1
-    per_image_value = total_vox / per_image_s / 1e6
-    log(f"per-image loop: {per_image_s * 1e3:.1f} ms, {per_image_value:.0f} Mvoxel/s")
-
-    # ---- latency of 1024x1024 frames through the per-image path -----------
-    frames = [torch.from_numpy(synth.workload2(i)).to(dev) for i in range(min(args.frames, 64))]
-    fout = torch.empty_like(frames[0])
-    torch.cuda.synchronize()
-    lat = []
-    for k in range(args.frames):
-        f = frames[k % len(frames)]
-        t2 = time.perf_counter()
-        B.thin2d_device(f, out=fout)
+    per_image = latency = None
+    outs_pi = None
+    if not args.only_batch:
+        # ---- per-image drop-in path on the same items -----------------------
+        outs_pi = [torch.empty_like(t) for t in ins]
         torch.cuda.synchronize()
-        lat.append((time.perf_counter() - t2) * 1e3)
-    p50 = float(np.percentile(lat, 50))
-    p99 = float(np.percentile(lat, 99))
-    log(f"1024x1024 frame latency p50 {p50:.3f} ms p99 {p99:.3f} ms")
+        t1 = time.perf_counter()
+        for t, o in zip(ins, outs_pi):
+            if t.dim() == 2:
+                B.thin2d_device(t, out=o)
+            else:
+                B.thin3d_device(t, out=o)
+        torch.cuda.synchronize()
+        per_image_s = time.perf_counter() - t1
+        per_image_value = total_vox / per_image_s / 1e6
+        log(f"per-image loop: {per_image_s * 1e3:.1f} ms, {per_image_value:.0f} Mvoxel/s")
+        per_image = {"value": round(per_image_value, 1), "unit": "Mvoxel/s",
+                     "ratio_batched_over_per_image": round(value / per_image_value, 2)}
+
+        # ---- latency of 1024x1024 frames through the per-image path ---------
+        frames = [torch.from_numpy(synth.workload2(i)).to(dev) for i in range(min(args.frames, 64))]
+        fout = torch.empty_like(frames[0])
+        torch.cuda.synchronize()
+        lat = []
+        for k in range(args.frames):
+            f = frames[k % len(frames)]
+            t2 = time.perf_counter()
+            B.thin2d_device(f, out=fout)
+            torch.cuda.synchronize()
+            lat.append((time.perf_counter() - t2) * 1e3)
+        p50 = float(np.percentile(lat, 50))
+        p99 = float(np.percentile(lat, 99))
+        log(f"1024x1024 frame latency p50 {p50:.3f} ms p99 {p99:.3f} ms")
+        latency = {"p50": round(p50, 3), "p99": round(p99, 3), "frames": args.frames,
+                   "path": "per-image device-resident"}
 
     # ---- results back; bit-exactness vs the oracle (the CPU baseline leg) --
     got = [o.cpu().numpy() for o in outs]
-    got_pi = [o.cpu().numpy() for o in outs_pi]
-    same_pi = all(np.array_equal(a, b) for a, b in zip(got, got_pi))
+    same_pi = (all(np.array_equal(a, o.cpu().numpy()) for a, o in zip(got, outs_pi))
+               if outs_pi is not None else None)
     gold_path = os.path.join(ROOT, "tests", "golden", "workload4.json")
     gold = json.load(open(gold_path)) if os.path.exists(gold_path) else None
     hashes = [sha(a) for a in got]
     gold_ok = None
     if gold:
-        gold_ok = sum(h == g for h, g in zip(hashes, gold["skeleton_sha"])) \
-            if len(gold["skeleton_sha"]) == len(hashes) else 0
-        gold_ok = gold_ok == len(hashes) and list(map(int, iters)) == gold["iterations"]
+        gold_ok = (len(gold["skeleton_sha"]) == len(hashes)
+                   and all(h == g for h, g in zip(hashes, gold["skeleton_sha"]))
+                   and list(map(int, iters)) == gold["iterations"])
 
     cpu = None
     exact = None
@@ -189,6 +225,7 @@ def main():
                          f"{cpu_s:.1f} s on one core"}
         log(f"cpu baseline {cpu['value']} Mvoxel/s; bit-exact items {ok}/{len(items)}")
 
+    traffic = measured_traffic()
     line = {
         "metric": "bit-exact thinned Mvoxel/s, batched vs per-image, 1 MI355X; p50/p99 ms",
         "value": round(value, 1),
@@ -205,17 +242,18 @@ def main():
         "config": {"workload": "mixed batch: 512 images U{128..2048}^2 + 16 volumes "
                                "U{64..384}^3, uint8, seeded", "items": len(items),
                    "mvoxel": round(total_vox / 1e6, 1), "graph": bool(args.graph)},
-        "per_image": {"value": round(per_image_value, 1), "unit": "Mvoxel/s",
-                      "ratio_batched_over_per_image": round(value / per_image_value, 2)},
-        "latency_1024x1024_ms": {"p50": round(p50, 3), "p99": round(p99, 3),
-                                 "frames": args.frames, "path": "per-image device-resident"},
+        "per_image": per_image,
+        "latency_1024x1024_ms": latency,
         "bit_exact": {"vs_oracle": exact, "vs_committed_hashes": gold_ok,
                       "batched_equals_per_image": same_pi},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1),
+        "roofline": {"bound": "hbm", "kernel": DOMINANT, "achieved": round(achieved / 1e9, 1),
                      "peak": 8000.0, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": None, "launches": dom_launches,
-                     "avg_launch_ms": round(dom_ms / max(dom_launches, 1), 4),
-                     "bytes_per_launch": int(dom_bytes / max(dom_launches, 1))},
+                     "traffic": traffic, "launches": l2d,
+                     "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                     "bytes_per_launch": int(bytes_per_launch),
+                     "timing": "HIP events around each launch on its stream, one extra step"},
+        "three_d_iterations": {"launch_groups": l3d, "ms": round(ms3d, 3),
+                               "avg_iteration_ms": round(ms3d / max(l3d, 1), 4)},
         "b_alg": {"bytes": int(b_alg), "frac_of_hbm_peak_per_step": round(
             b_alg / (ms_step / 1e3) / HBM_PEAK, 4)},
         "cpu_baseline": cpu,
