@@ -3,9 +3,11 @@
 // State layout: every volume is stored bit-packed with a margin of zero words
 // around it, in two state arrays shared by all volumes of a run.  An x-line
 // holds nh = 2*ceil(w/64) data words (bit k of word j is voxel x = 32j + k)
-// between one zero word on each side: P = nh + 2 words per padded line.  A
-// padded plane is h + 2 lines (a zero line above and below), the volume d + 2
-// planes.  Interior word (j, y, z) of an item is at
+// after one zero word: P = nh + 1 words per padded line (the next line's zero
+// word is the right-hand neighbour of a line's last word).  A padded plane is
+// h + 2 lines (a zero line above and below), the volume d + 2 planes and one
+// trailing zero word (the right-hand neighbour of the last line's last word).
+// Interior word (j, y, z) of an item is at
 //     base + ((z + 1) * (h + 2) + (y + 1)) * P + (j + 1),
 // so its 26 neighbour words are at constant offsets +-1, +-P, +-plane and
 // their sums: no bounds checks, and the margins stay zero for good because

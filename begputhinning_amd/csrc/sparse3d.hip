@@ -156,9 +156,15 @@ __global__ __launch_bounds__(256) void sparse3d_step(const uint32_t *__restrict_
                                                      uint32_t *__restrict__ prot,
                                                      int *__restrict__ changed)
 {
+    // The list is cut into one contiguous range per XCD (workgroups are dealt
+    // to the 8 XCDs round-robin; the grid is a multiple of 8 or smaller than
+    // 8), so that neighbouring entries, which share most of their 3x3x3
+    // neighbourhoods, are gathered through the same L2.
     const int n = *list.count;
-    const int stride = gridDim.x * blockDim.x;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const int nx = gridDim.x >= 8 ? 8 : 1;
+    const int xcd = blockIdx.x % nx, local = blockIdx.x / nx, per = gridDim.x / nx;
+    const int lo = (int)((long)n * xcd / nx), hi = (int)((long)n * (xcd + 1) / nx);
+    for (int k = lo + local * 256 + threadIdx.x; k < hi; k += per * 256) {
         const int ii = list.item[k];
         const uint32_t g = list.word[k];
         const int2 gp = geo[ii];
@@ -269,7 +275,9 @@ int sparse3d_layout(const int *w, const int *h, const int *d, int n, Layout3D &l
         it.h = h[i];
         it.d = d[i];
         it.nh = 2 * ((w[i] + 63) / 64);
-        const long P = it.nh + 2, plane = P * (h[i] + 2), total = plane * (d[i] + 2);
+        // + 1: the trailing zero word is the +x neighbour of the last word of
+        // the last margin line, which the corner interior word gathers
+        const long P = it.nh + 1, plane = P * (h[i] + 2), total = plane * (d[i] + 2) + 1;
         if (base + total >= 0x7fffffffL || plane >= 0x7fffffffL)
             return fail(BGT_ERR_UNSUPPORTED, "3D state of %ld words exceeds 2^31", base + total);
         it.P = (int)P;
@@ -329,7 +337,9 @@ int sparse3d_iteration(const Sparse3D &e, int cur, int bound, int grid, hipStrea
 {
     const SList L = e.list[cur], N = e.list[cur ^ 1];
     const int need = std::max(1, (bound + 255) / 256);
-    const dim3 g(std::min(grid, need)), b(256);
+    int steps = std::min(grid, need);
+    if (steps >= 8) steps &= ~7;  // whole XCD ranges
+    const dim3 g(steps), b(256);
     uint32_t *s0 = e.state[0], *s1 = e.state[1];
     hipLaunchKernelGGL(sparse3d_step<0>, g, b, 0, s, s0, s1, e.geo, L, e.prot, e.changed);
     hipLaunchKernelGGL(sparse3d_step<1>, g, b, 0, s, s1, s0, e.geo, L, e.prot, e.changed);

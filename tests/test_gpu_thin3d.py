@@ -41,6 +41,17 @@ def test_random_shapes(w, h, d):
     check(vol, f"random {w}x{h}x{d}")
 
 
+@pytest.mark.parametrize("w,h,d", [(32, 3, 3), (64, 5, 4), (96, 6, 7), (256, 9, 5)])
+def test_full_last_word_corner(w, h, d):
+    """Widths that fill the last state word, with foreground up to the far
+    corner (x = w-1, y = h-1, z = d-1): the corner word's +x neighbours lie in
+    the zero margins after the last line."""
+    rng = np.random.default_rng(w + h + d)
+    vol = (rng.random((d, h, w)) < 0.8).astype(np.uint8)
+    vol[-2:, -2:, -40:] = 1
+    check(vol, f"corner {w}x{h}x{d}")
+
+
 @pytest.mark.parametrize("side", [64, 96])
 def test_tree(side):
     from begputhinning_amd import synth
