@@ -78,8 +78,31 @@ def build_oracle():
         subprocess.run([make, "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
 
 
+def build_dropin_drivers():
+    """Test drivers written against the drop-in C++ headers (include/2D,
+    include/3D), linked to libbgt.so: tests/dropin/dropin2d and dropin3d."""
+    src = os.path.join(ROOT, "tests", "dropin", "dropin_main.cpp")
+    if not os.path.exists(src):
+        return []
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if not cxx:
+        raise RuntimeError("no C++ compiler for the drop-in drivers")
+    outs = []
+    for dim in (2, 3):
+        exe = os.path.join(ROOT, "tests", "dropin", f"dropin{dim}d")
+        cmd = [cxx, "-std=c++14", "-O1", "-Wall", "-Wextra", f"-DDIM={dim}",
+               "-I" + os.path.join(ROOT, "include"), src, "-L" + HERE, "-lbgt",
+               "-Wl,-rpath,$ORIGIN/../../begputhinning_amd", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode:
+            raise RuntimeError("drop-in driver build failed:\n" + r.stdout + r.stderr)
+        outs.append(exe)
+    return outs
+
+
 def main(argv):
     build_library(verbose="-v" in argv)
+    build_dropin_drivers()
     if "--all" in argv:
         build_oracle()
     print(LIB)

@@ -257,3 +257,25 @@ def ref_extract3d(vol):
     assert ns >= 0, "capacity too small"
     m = npts.value
     return {"x": xs[:m], "y": ys[:m], "z": zs[:m], "start": st[:ns + 1], "is_end": ie[:ns]}
+
+
+def ref_extract3d_plain(vol):
+    """The reference's ExtractSegments (point lists, no end flags)."""
+    L = ctypes.CDLL(os.path.join(REF_DIR, "libref3d.so"))
+    L.ref3d_extract_plain.restype = ctypes.c_int
+    L.ref3d_extract_plain.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, _i16p, _i16p, _i16p, _u32p,
+                                      ctypes.POINTER(ctypes.c_int)]
+    a = np.ascontiguousarray(vol, dtype=np.uint8)
+    d, h, w = a.shape
+    cnt = int(a.astype(bool).sum())
+    cap_p, cap_s = cnt * 60 + 16, cnt * 30 + 16
+    xs = np.zeros(cap_p, np.int16); ys = np.zeros(cap_p, np.int16); zs = np.zeros(cap_p, np.int16)
+    st = np.zeros(cap_s + 1, np.uint32)
+    npts = ctypes.c_int(0)
+    ns = L.ref3d_extract_plain(_ptr(a, _u8p), w, h, d, cap_p, cap_s, _ptr(xs, _i16p),
+                               _ptr(ys, _i16p), _ptr(zs, _i16p), _ptr(st, _u32p),
+                               ctypes.byref(npts))
+    assert ns >= 0, "capacity too small"
+    m = npts.value
+    return {"x": xs[:m], "y": ys[:m], "z": zs[:m], "start": st[:ns + 1]}
