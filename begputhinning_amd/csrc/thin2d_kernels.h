@@ -47,15 +47,16 @@ struct Tile2D {
 
 // ---- lane exchange -------------------------------------------------------
 
-// lane i receives lane i-1's value (lane 0 receives 0)
+// lane i receives lane i-1's value (lane 0 receives 0: DPP wave_shr:1 with
+// bound_ctrl, which writes 0 where the source lane is out of range)
 __device__ __forceinline__ uint32_t from_lower_lane(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, true);
 }
-// lane i receives lane i+1's value (lane 63 receives 0)
+// lane i receives lane i+1's value (lane 63 receives 0: DPP wave_shl:1)
 __device__ __forceinline__ uint32_t from_upper_lane(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, true);
 }
 
 struct W64 {
