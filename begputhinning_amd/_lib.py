@@ -75,6 +75,14 @@ SIGNATURES = {
     "bgt_batch_set_timing": (_i, [_vp, _i]),
     "bgt_batch_kernel_time": (_i, [_vp, ctypes.POINTER(ctypes.c_double), _ip,
                                    ctypes.POINTER(ctypes.c_double), _ip]),
+    "bgt_server_create": (_i, [ctypes.POINTER(ctypes.c_void_p), _sz, _i, _i, _i]),
+    "bgt_server_submit2d": (_i, [_vp, _vp, _sz, _i, _i, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)]),
+    "bgt_server_submit3d": (_i, [_vp, _vp, _i, _i, _i, _vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "bgt_server_wait": (_i, [_vp, ctypes.c_uint64, _ip, ctypes.POINTER(ctypes.c_double)]),
+    "bgt_server_poll": (_i, [_vp, ctypes.c_uint64, _ip]),
+    "bgt_server_stats": (_i, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                              ctypes.POINTER(ctypes.c_size_t)]),
+    "bgt_server_destroy": (_i, [_vp]),
     "bgt_synth_batch_dims": (_i, [ctypes.c_uint64, _i, _i, _i, _ip, _i, _i, _i, _ip]),
     "bgt_synth_arrivals": (_i, [ctypes.c_uint64, _i, ctypes.c_double,
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),

@@ -24,5 +24,12 @@ size_t batch_device_bytes(Batch &b);
 int batch_size(const Batch &b);
 void batch_set_timing(Batch &b, bool on);
 void batch_kernel_time(const Batch &b, double *ms2d, int *n2d, double *ms3d, int *n3d);
+// Device bytes a run of the current items needs (no allocation).
+int batch_plan_bytes(Batch &b, size_t *bytes);
+// Lay later runs out in caller-owned device memory instead of an allocation
+// of the batch's own (mem == nullptr restores that).
+void batch_set_arena(Batch &b, void *mem, size_t bytes);
+// Remove all items (keeps streams, events and the arena).
+void batch_clear(Batch &b);
 
 }  // namespace bgt

@@ -223,6 +223,9 @@ struct Batch {
     bool ready = false;
     void *mem = nullptr;
     size_t bytes = 0;
+    bool owns_mem = true;       // false when `mem` is the caller's arena
+    void *arena = nullptr;      // optional caller-owned device memory
+    size_t arena_bytes = 0;
 
     // 2D pipeline: band work queue
     Item2D *d_items2 = nullptr;
@@ -290,15 +293,19 @@ struct Batch {
         if (graph2) (void)hipGraphExecDestroy(graph2);
         graph2 = nullptr;
         zeroed3 = false;
-        if (mem) (void)hipFree(mem);
+        if (mem && owns_mem) (void)hipFree(mem);
         mem = nullptr;
         ready = false;
     }
 };
 
-static int prepare(Batch &b)
+// Lays the batch out in device memory (its own allocation, or the caller's
+// arena); with size_only, only computes b.bytes (the sizing sweep sets the
+// device pointers from a null base, so a batch that is not ready is laid out
+// again by its next run).
+static int prepare(Batch &b, bool size_only = false)
 {
-    if (b.ready) return BGT_OK;
+    if (b.ready) return BGT_OK;  // b.bytes is that of the current layout
     if (b.in2.empty() && b.in3.empty()) return fail(BGT_ERR_INVALID, "empty batch");
     BGT_CHECK(check_config2d(b.c2));
     const int n2 = (int)b.in2.size(), n3 = (int)b.in3.size();
@@ -419,12 +426,22 @@ static int prepare(Batch &b)
         }
         if (sweep == 0) {
             b.bytes = cv.used;
+            if (size_only) return BGT_OK;
+            if (b.arena) {
+                if (b.bytes > b.arena_bytes)
+                    return fail(BGT_ERR_NOMEM, "batch needs %zu bytes, arena holds %zu", b.bytes,
+                                b.arena_bytes);
+                b.mem = b.arena;
+                b.owns_mem = false;
+                continue;
+            }
             hipError_t e = hipMalloc(&b.mem, b.bytes);
             if (e != hipSuccess) {
                 (void)hipGetLastError();
                 b.mem = nullptr;
                 return fail(BGT_ERR_NOMEM, "batch allocation of %zu bytes failed", b.bytes);
             }
+            b.owns_mem = true;
             continue;
         }
         if (n2) {
@@ -725,6 +742,27 @@ int batch_set_config(Batch &b, const Config2D &c2, const Config3D &c3)
 }
 
 size_t batch_device_bytes(Batch &b) { return b.ready ? b.bytes : 0; }
+
+int batch_plan_bytes(Batch &b, size_t *bytes)
+{
+    BGT_CHECK(prepare(b, true));
+    *bytes = b.bytes;
+    return BGT_OK;
+}
+
+void batch_set_arena(Batch &b, void *mem, size_t bytes)
+{
+    b.release();
+    b.arena = mem;
+    b.arena_bytes = mem ? bytes : 0;
+}
+
+void batch_clear(Batch &b)
+{
+    b.release();
+    b.in2.clear();
+    b.in3.clear();
+}
 
 int batch_size(const Batch &b) { return (int)(b.in2.size() + b.in3.size()); }
 

@@ -12,6 +12,7 @@
 #include "batch.h"
 #include "common.h"
 #include "segments.h"
+#include "server.h"
 #include "thin2d.h"
 #include "thin3d.h"
 
@@ -242,6 +243,66 @@ int bgt_batch_destroy(bgt_batch b)
     if (!b) return BGT_OK;
     batch_delete(b->b);
     delete b;
+    return BGT_OK;
+}
+
+struct bgt_server_s {
+    Server *s;
+};
+
+int bgt_server_create(bgt_server *out, size_t hbm_budget, int workers2d, int workers3d,
+                      int max_items)
+{
+    if (!out) return fail(BGT_ERR_INVALID, "null pointer");
+    ServerConfig cfg;
+    cfg.budget = hbm_budget;
+    cfg.workers2d = workers2d;
+    cfg.workers3d = workers3d;
+    cfg.max_items = max_items;
+    Server *s = nullptr;
+    BGT_CHECK(server_create(cfg, default_config2d(), default_config3d(), &s));
+    *out = new bgt_server_s{s};
+    return BGT_OK;
+}
+
+int bgt_server_submit2d(bgt_server sv, const uint8_t *d_in, size_t in_pitch, int width, int height,
+                        uint8_t *d_out, size_t out_pitch, uint64_t *ticket)
+{
+    if (!sv) return fail(BGT_ERR_INVALID, "null server");
+    return server_submit(sv->s, 0, d_in, in_pitch, d_out, out_pitch, width, height, 1, ticket);
+}
+
+int bgt_server_submit3d(bgt_server sv, const uint8_t *d_in, int width, int height, int depth,
+                        uint8_t *d_out, uint64_t *ticket)
+{
+    if (!sv) return fail(BGT_ERR_INVALID, "null server");
+    return server_submit(sv->s, 1, d_in, 0, d_out, 0, width, height, depth, ticket);
+}
+
+int bgt_server_wait(bgt_server sv, uint64_t ticket, int *iterations, double *latency_ms)
+{
+    if (!sv) return fail(BGT_ERR_INVALID, "null server");
+    return server_wait(sv->s, ticket, iterations, latency_ms);
+}
+
+int bgt_server_poll(bgt_server sv, uint64_t ticket, int *done)
+{
+    if (!sv || !done) return fail(BGT_ERR_INVALID, "null pointer");
+    return server_poll(sv->s, ticket, done);
+}
+
+int bgt_server_stats(bgt_server sv, uint64_t *batches, uint64_t *completed, size_t *peak_bytes)
+{
+    if (!sv || !batches || !completed || !peak_bytes) return fail(BGT_ERR_INVALID, "null pointer");
+    server_stats(sv->s, batches, completed, peak_bytes);
+    return BGT_OK;
+}
+
+int bgt_server_destroy(bgt_server sv)
+{
+    if (!sv) return BGT_OK;
+    server_destroy(sv->s);
+    delete sv;
     return BGT_OK;
 }
 

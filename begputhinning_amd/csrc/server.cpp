@@ -1,0 +1,277 @@
+// server.cpp -- request loop with continuous batching over the batch engine.
+//
+// Requests (device-resident images and volumes, caller-owned input and output
+// buffers) are queued per class (2D, 3D).  Worker threads -- each with its
+// own stream, batch engine and grow-only device arena, so no allocation or
+// free happens per batch -- take every queued request of their class that
+// fits the HBM budget (the engine memory of all admitted requests stays below
+// it), thin them as one batch and release their memory.  A request arriving
+// while a batch runs is taken by the next idle worker at once, so its work
+// overlaps the running batch on the device; frames never wait behind volumes
+// because the classes have their own workers.  Each result equals the
+// single-item result (the batch engine's per-item convergence), whatever the
+// mix it ran in.
+//
+// No counterpart in the reference (it thins one image per blocking call,
+// 2D/LibThinning2D.cu:39-130).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch.h"
+#include "common.h"
+#include "server.h"
+
+namespace bgt {
+
+namespace {
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    uint64_t id = 0;
+    int cls = 0;  // 0 = 2D, 1 = 3D
+    const uint8_t *in = nullptr;
+    size_t in_pitch = 0;
+    uint8_t *out = nullptr;
+    size_t out_pitch = 0;
+    int w = 0, h = 0, d = 1;
+    size_t bytes = 0;  // admission footprint (engine memory of the item alone)
+    double t_submit = 0, t_done = 0;
+    int iterations = 0;
+    int status = BGT_OK;
+    std::string error;  // message of a failed batch (set on the worker thread)
+    bool done = false;
+};
+
+}  // namespace
+
+struct Server {
+    ServerConfig cfg;
+    Config2D c2;
+    Config3D c3;
+    int device = 0;
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> queue[2];
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    size_t in_use = 0;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::vector<std::thread> threads;
+    // statistics
+    uint64_t batches = 0, completed = 0;
+    size_t peak_in_use = 0;
+
+    void worker(int cls);
+};
+
+void Server::worker(int cls)
+{
+    (void)hipSetDevice(device);
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    std::unique_ptr<Batch, void (*)(Batch *)> b(batch_new(), batch_delete);
+    (void)batch_set_config(*b, c2, c3);
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    std::vector<Request *> take;
+    std::vector<int> iters;
+    for (;;) {
+        take.clear();
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv_work.wait(lk, [&] {
+                return stop || (!queue[cls].empty() && in_use + queue[cls].front()->bytes <= cfg.budget);
+            });
+            if (stop) break;
+            while (!queue[cls].empty() && (int)take.size() < cfg.max_items &&
+                   in_use + queue[cls].front()->bytes <= cfg.budget) {
+                Request *r = queue[cls].front();
+                queue[cls].pop_front();
+                in_use += r->bytes;
+                take.push_back(r);
+            }
+            peak_in_use = std::max(peak_in_use, in_use);
+        }
+        int rc = s ? BGT_OK : fail(BGT_ERR_HIP, "worker stream creation failed");
+        batch_clear(*b);
+        for (Request *r : take) {
+            if (rc != BGT_OK) break;
+            rc = r->cls == 0 ? batch_add2d(*b, r->in, r->in_pitch, r->w, r->h, r->out, r->out_pitch)
+                             : batch_add3d(*b, r->in, r->w, r->h, r->d, r->out);
+        }
+        size_t need = 0;
+        if (rc == BGT_OK) rc = batch_plan_bytes(*b, &need);
+        if (rc == BGT_OK && need > arena_bytes) {
+            batch_set_arena(*b, nullptr, 0);
+            if (arena) (void)hipFree(arena);
+            arena = nullptr;
+            arena_bytes = 0;
+            const size_t grow = need + need / 4;
+            if (hipMalloc(&arena, grow) == hipSuccess) {
+                arena_bytes = grow;
+            } else {
+                (void)hipGetLastError();
+                arena = nullptr;
+                rc = fail(BGT_ERR_NOMEM, "server arena of %zu bytes failed", grow);
+            }
+        }
+        if (rc == BGT_OK) {
+            batch_set_arena(*b, arena, arena_bytes);
+            rc = batch_run(*b, s, false);
+        }
+        iters.assign(take.size(), 0);
+        if (rc == BGT_OK) rc = batch_iterations(*b, iters.data());
+        const double t = now_ms();
+        const std::string err = rc == BGT_OK ? std::string() : std::string(bgt_last_error());
+        {
+            std::lock_guard<std::mutex> lk(m);
+            for (size_t k = 0; k < take.size(); ++k) {
+                Request *r = take[k];
+                r->status = rc;
+                r->error = err;
+                r->iterations = iters[k];
+                r->t_done = t;
+                r->done = true;
+                in_use -= r->bytes;
+            }
+            completed += take.size();
+            ++batches;
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+    batch_set_arena(*b, nullptr, 0);
+    b.reset();
+    if (arena) (void)hipFree(arena);
+    if (s) (void)hipStreamDestroy(s);
+}
+
+int server_create(const ServerConfig &cfg, const Config2D &c2, const Config3D &c3, Server **out)
+{
+    if (cfg.budget == 0 || cfg.workers2d < 0 || cfg.workers3d < 0 ||
+        cfg.workers2d + cfg.workers3d < 1 || cfg.max_items < 1)
+        return fail(BGT_ERR_INVALID, "bad server configuration");
+    BGT_CHECK(check_config2d(c2));
+    BGT_CHECK(check_config3d(c3));
+    std::unique_ptr<Server> sv(new Server());
+    sv->cfg = cfg;
+    sv->c2 = c2;
+    sv->c3 = c3;
+    BGT_HIP(hipGetDevice(&sv->device));
+    for (int k = 0; k < cfg.workers2d; ++k) sv->threads.emplace_back(&Server::worker, sv.get(), 0);
+    for (int k = 0; k < cfg.workers3d; ++k) sv->threads.emplace_back(&Server::worker, sv.get(), 1);
+    *out = sv.release();
+    return BGT_OK;
+}
+
+void server_destroy(Server *sv)
+{
+    if (!sv) return;
+    {
+        // requests not yet taken by a worker fail; batches in flight finish
+        std::lock_guard<std::mutex> lk(sv->m);
+        sv->stop = true;
+        for (auto &q : sv->queue)
+            for (Request *r : q) {
+                r->status = BGT_ERR_INVALID;
+                r->error = "server destroyed before the request ran";
+                r->done = true;
+            }
+        sv->queue[0].clear();
+        sv->queue[1].clear();
+    }
+    sv->cv_work.notify_all();
+    sv->cv_done.notify_all();
+    for (auto &t : sv->threads) t.join();
+    delete sv;
+}
+
+static int item_bytes(const Request &r, const Config2D &c2, const Config3D &c3, size_t *bytes)
+{
+    std::unique_ptr<Batch, void (*)(Batch *)> b(batch_new(), batch_delete);
+    BGT_CHECK(batch_set_config(*b, c2, c3));
+    BGT_CHECK(r.cls == 0 ? batch_add2d(*b, r.in, r.in_pitch, r.w, r.h, r.out, r.out_pitch)
+                         : batch_add3d(*b, r.in, r.w, r.h, r.d, r.out));
+    return batch_plan_bytes(*b, bytes);
+}
+
+int server_submit(Server *sv, int cls, const uint8_t *in, size_t in_pitch, uint8_t *out,
+                  size_t out_pitch, int w, int h, int d, uint64_t *ticket)
+{
+    if (!sv || !ticket) return fail(BGT_ERR_INVALID, "null server or ticket");
+    if ((cls == 0 ? sv->cfg.workers2d : sv->cfg.workers3d) == 0)
+        return fail(BGT_ERR_INVALID, "the server has no %s workers", cls == 0 ? "2D" : "3D");
+    std::unique_ptr<Request> r(new Request());
+    r->cls = cls;
+    r->in = in;
+    r->in_pitch = in_pitch;
+    r->out = out;
+    r->out_pitch = out_pitch;
+    r->w = w;
+    r->h = h;
+    r->d = d;
+    BGT_CHECK(item_bytes(*r, sv->c2, sv->c3, &r->bytes));
+    if (r->bytes > sv->cfg.budget)
+        return fail(BGT_ERR_ADMISSION, "request needs %zu bytes of device memory, budget is %zu",
+                    r->bytes, sv->cfg.budget);
+    {
+        std::lock_guard<std::mutex> lk(sv->m);
+        r->id = sv->next_id++;
+        r->t_submit = now_ms();
+        *ticket = r->id;
+        sv->queue[cls].push_back(r.get());
+        sv->reqs.emplace(r->id, std::move(r));
+    }
+    sv->cv_work.notify_all();
+    return BGT_OK;
+}
+
+int server_wait(Server *sv, uint64_t ticket, int *iterations, double *latency_ms)
+{
+    std::unique_lock<std::mutex> lk(sv->m);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return fail(BGT_ERR_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
+    Request *r = it->second.get();
+    sv->cv_done.wait(lk, [&] { return r->done; });
+    const int rc = r->status;
+    const std::string err = r->error;
+    if (iterations) *iterations = r->iterations;
+    if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+    sv->reqs.erase(it);
+    if (rc != BGT_OK) set_error(rc, "request failed: " + err);
+    return rc;
+}
+
+int server_poll(Server *sv, uint64_t ticket, int *done)
+{
+    std::lock_guard<std::mutex> lk(sv->m);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return fail(BGT_ERR_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
+    *done = it->second->done ? 1 : 0;
+    return BGT_OK;
+}
+
+void server_stats(Server *sv, uint64_t *batches, uint64_t *completed, size_t *peak_bytes)
+{
+    std::lock_guard<std::mutex> lk(sv->m);
+    *batches = sv->batches;
+    *completed = sv->completed;
+    *peak_bytes = sv->peak_in_use;
+}
+
+}  // namespace bgt

@@ -151,6 +151,36 @@ int bgt_batch_kernel_time(bgt_batch b, double *ms2d, int *launches2d, double *ms
 int bgt_batch_destroy(bgt_batch b);
 
 /* ------------------------------------------------------------------------- */
+/* Serving (no reference counterpart: the reference thins one image per       */
+/* blocking call, 2D/LibThinning2D.cu:39-130).  Device-resident requests are  */
+/* admitted by the device memory their thinning needs (at most hbm_budget     */
+/* bytes in use at once), batched as they arrive and thinned by worker        */
+/* threads (workers2d for images, workers3d for volumes; each its own stream  */
+/* and batch engine), so requests arriving while others run overlap them on  */
+/* the device.  A result equals the single-item result whatever it ran with.  */
+/* Inputs must be ready on the device when submitted; outputs are complete    */
+/* when bgt_server_wait returns.                                              */
+/* ------------------------------------------------------------------------- */
+typedef struct bgt_server_s *bgt_server;
+
+/* Launch shapes are those current at this call (bgt_set_config2d/3d). */
+int bgt_server_create(bgt_server *out, size_t hbm_budget, int workers2d, int workers3d,
+                      int max_items);
+/* BGT_ERR_ADMISSION when the request alone needs more than the budget. */
+int bgt_server_submit2d(bgt_server s, const uint8_t *d_in, size_t in_pitch, int width, int height,
+                        uint8_t *d_out, size_t out_pitch, uint64_t *ticket);
+int bgt_server_submit3d(bgt_server s, const uint8_t *d_in, int width, int height, int depth,
+                        uint8_t *d_out, uint64_t *ticket);
+/* Blocks until the request is done and releases its ticket; latency_ms is
+ * submit-to-done on the host's steady clock.  Either pointer may be NULL. */
+int bgt_server_wait(bgt_server s, uint64_t ticket, int *iterations, double *latency_ms);
+int bgt_server_poll(bgt_server s, uint64_t ticket, int *done);
+int bgt_server_stats(bgt_server s, uint64_t *batches, uint64_t *completed, size_t *peak_bytes);
+/* Requests not yet started fail; batches in flight finish first.  Call after
+ * every wait has returned. */
+int bgt_server_destroy(bgt_server s);
+
+/* ------------------------------------------------------------------------- */
 /* Synthetic inputs (seeded, deterministic) for tests and benchmarks.         */
 /* ------------------------------------------------------------------------- */
 uint64_t bgt_synth_item_seed(int config, int item);
