@@ -216,6 +216,7 @@ int server_submit(Server *sv, int cls, const uint8_t *in, size_t in_pitch, uint8
     if (!sv || !ticket) return fail(BGT_ERR_INVALID, "null server or ticket");
     if ((cls == 0 ? sv->cfg.workers2d : sv->cfg.workers3d) == 0)
         return fail(BGT_ERR_INVALID, "the server has no %s workers", cls == 0 ? "2D" : "3D");
+    const double t_submit = now_ms();  // latency includes admission
     std::unique_ptr<Request> r(new Request());
     r->cls = cls;
     r->in = in;
@@ -232,7 +233,7 @@ int server_submit(Server *sv, int cls, const uint8_t *in, size_t in_pitch, uint8
     {
         std::lock_guard<std::mutex> lk(sv->m);
         r->id = sv->next_id++;
-        r->t_submit = now_ms();
+        r->t_submit = t_submit;
         *ticket = r->id;
         sv->queue[cls].push_back(r.get());
         sv->reqs.emplace(r->id, std::move(r));

@@ -61,6 +61,107 @@ def measured_traffic():
     return float(m.group(1)) if m else None
 
 
+def serve_pool(synth):
+    """Distinct inputs the serving leg draws requests from: frames with sides
+    U{256..1024} (polyline or tree content, half each) including 1024x1024
+    ones, and volumes with sides U{64..512} (tree content)."""
+    rng = np.random.default_rng(0xBE6E0004)
+    specs = []
+    for i in range(40):
+        w, h = (int(v) for v in rng.integers(256, 1025, 2))
+        specs.append(("poly" if i % 2 else "tree", w, h, 0))
+    specs += [("tree", 1024, 1024, 0)] * 4 + [("poly", 1024, 1024, 0)] * 4
+    for i in range(12):
+        w, h, d = (int(v) for v in rng.integers(64, 513, 3))
+        specs.append(("tree3d", w, h, d))
+
+    def make(k):
+        kind, w, h, d = specs[k]
+        seed = synth.item_seed(4, 10000 + k)
+        if kind == "poly":
+            return synth.polylines(w, h, seed)
+        if kind == "tree":
+            return synth.tree2d(w, h, seed)
+        return synth.tree3d(w, h, d, seed)
+
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 4)) as ex:
+        return list(ex.map(make, range(len(specs))))
+
+
+def serve_leg(args, torch, B, synth, dev):
+    """Poisson arrivals at a fixed rate of 80% frames / 20% volumes drawn from
+    the pool, submitted open-loop to the server; latency is submit-to-done on
+    the server's clock.  Every served output is compared with the per-image
+    device result of its input."""
+    import threading
+    from begputhinning_amd.serve import ThinningServer
+    pool = serve_pool(synth)
+    ins = [torch.from_numpy(a).to(dev) for a in pool]
+    single = []
+    for t in ins:
+        single.append((B.thin2d_device(t) if t.dim() == 2 else B.thin3d_device(t))[0])
+    torch.cuda.synchronize()
+    frames = [k for k, a in enumerate(pool) if a.ndim == 2]
+    vols = [k for k, a in enumerate(pool) if a.ndim == 3]
+    n = args.serve_requests
+    gaps, mix = synth.arrivals(n, args.serve_qps, 0xBE6E0005)
+    rng = np.random.default_rng(0xBE6E0006)
+    pick = [int(frames[rng.integers(len(frames))]) if m < 0.8 else int(vols[rng.integers(len(vols))])
+            for m in mix]
+    outs = [torch.empty_like(ins[k]) for k in pick]
+    torch.cuda.synchronize()
+    srv = ThinningServer(budget_bytes=32 << 30, workers2d=2, workers3d=2, max_items=64)
+    tickets = [None] * n
+    lat = [0.0] * n
+    submitted = threading.Semaphore(0)
+    results = {}
+
+    def collect():
+        for i in range(n):
+            submitted.acquire()
+            _, lat[i] = srv.wait(tickets[i])
+        results["t_end"] = time.perf_counter()
+
+    col = threading.Thread(target=collect)
+    col.start()
+    t0 = time.perf_counter()
+    due = t0
+    for i in range(n):
+        due += gaps[i]
+        while True:
+            left = due - time.perf_counter()
+            if left <= 0:
+                break
+            if left > 2e-4:
+                time.sleep(left - 1e-4)
+        tickets[i] = srv.submit(ins[pick[i]], outs[i])
+        submitted.release()
+    col.join()
+    span = results["t_end"] - t0
+    st = srv.stats()
+    srv.close()
+    same = all(torch.equal(o, single[k]) for o, k in zip(outs, pick))
+    vox = sum(pool[k].size for k in pick)
+    is2 = np.array([pool[k].ndim == 2 for k in pick])
+    big = np.array([pool[k].ndim == 2 and pool[k].shape == (1024, 1024) for k in pick])
+    lat = np.array(lat)
+
+    def pct(mask):
+        return ({"p50": round(float(np.percentile(lat[mask], 50)), 3),
+                 "p99": round(float(np.percentile(lat[mask], 99)), 3), "n": int(mask.sum())}
+                if mask.any() else None)
+
+    out = {"qps": args.serve_qps, "requests": n, "mix": "80% 2D U{256..1024}, 20% 3D U{64..512}",
+           "latency_ms_2d": pct(is2), "latency_ms_1024x1024": pct(big),
+           "latency_ms_3d": pct(~is2), "mvoxel_per_s": round(vox / span / 1e6, 1),
+           "batches": st["batches"], "served_equals_single_item": same,
+           "workers": "2D x2, 3D x2", "budget_bytes": 32 << 30}
+    log(f"serving {args.serve_qps:.0f} QPS: 2D p50/p99 {out['latency_ms_2d']}, 1024^2 "
+        f"{out['latency_ms_1024x1024']}, 3D {out['latency_ms_3d']}, {out['mvoxel_per_s']} Mvoxel/s, "
+        f"{st['batches']} batches, equal {same}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,12 +172,16 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="replay pass chunks as a hipGraph")
     ap.add_argument("--only-batch", action="store_true",
                     help="time the batched step only (no per-image loop, latency or CPU leg)")
+    ap.add_argument("--serve-qps", type=float, default=1000.0,
+                    help="Poisson arrival rate of the serving leg (0 = skip it)")
+    ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, to scale hardware byte counters)")
     args = ap.parse_args()
     if args.only_batch:
         args.no_cpu_baseline = True
+        args.serve_qps = 0
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
               "(items shard across GPUs with no communication)", file=sys.stderr)
@@ -194,6 +299,10 @@ def main():
         latency = {"p50": round(p50, 3), "p99": round(p99, 3), "frames": args.frames,
                    "path": "per-image device-resident"}
 
+    serving = None
+    if args.serve_qps > 0:
+        serving = serve_leg(args, torch, B, synth, dev)
+
     # ---- results back; bit-exactness vs the oracle (the CPU baseline leg) --
     got = [o.cpu().numpy() for o in outs]
     same_pi = (all(np.array_equal(a, o.cpu().numpy()) for a, o in zip(got, outs_pi))
@@ -244,6 +353,7 @@ def main():
                    "mvoxel": round(total_vox / 1e6, 1), "graph": bool(args.graph)},
         "per_image": per_image,
         "latency_1024x1024_ms": latency,
+        "serving": serving,
         "bit_exact": {"vs_oracle": exact, "vs_committed_hashes": gold_ok,
                       "batched_equals_per_image": same_pi},
         "roofline": {"bound": "hbm", "kernel": DOMINANT, "achieved": round(achieved / 1e9, 1),
