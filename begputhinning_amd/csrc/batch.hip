@@ -221,6 +221,7 @@ struct Batch {
     std::vector<In3> in3;
 
     bool ready = false;
+    bool ran = false;  // a run completed since the layout was made
     void *mem = nullptr;
     size_t bytes = 0;
     bool owns_mem = true;       // false when `mem` is the caller's arena
@@ -260,6 +261,10 @@ struct Batch {
     long ntiles2 = 0;
 
     int *h_live = nullptr;  // pinned, two slots of 32: [0] 2D live, [8..] 3D control words
+    char *h_stage = nullptr;   // pinned staging of the tables uploaded by prepare
+    size_t h_stage_bytes = 0;
+    int *h_states = nullptr;   // pinned copy of the item states after a run
+    size_t h_states_ints = 0;
     hipEvent_t ev_poll2[2] = {nullptr, nullptr}, ev_poll3[2] = {nullptr, nullptr};
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -283,6 +288,8 @@ struct Batch {
             if (ev_poll3[k]) (void)hipEventDestroy(ev_poll3[k]);
         }
         if (h_live) (void)hipHostFree(h_live);
+        if (h_stage) (void)hipHostFree(h_stage);
+        if (h_states) (void)hipHostFree(h_states);
     }
     void release()
     {
@@ -296,6 +303,7 @@ struct Batch {
         if (mem && owns_mem) (void)hipFree(mem);
         mem = nullptr;
         ready = false;
+        ran = false;
     }
 };
 
@@ -303,8 +311,20 @@ struct Batch {
 // arena); with size_only, only computes b.bytes (the sizing sweep sets the
 // device pointers from a null base, so a batch that is not ready is laid out
 // again by its next run).
-static int prepare(Batch &b, bool size_only = false)
+static int prepare(Batch &b, hipStream_t s, bool size_only = false)
 {
+    // host tables, uploaded together through pinned memory on the run stream
+    struct Staged {
+        void *dst;
+        size_t off, n;
+    };
+    std::vector<Staged> staged;
+    std::vector<char> host_stage;
+    auto stage = [&](void *dst, const void *src, size_t n) {
+        if (!n) return;
+        staged.push_back({dst, host_stage.size(), n});
+        host_stage.insert(host_stage.end(), (const char *)src, (const char *)src + n);
+    };
     if (b.ready) return BGT_OK;  // b.bytes is that of the current layout
     if (b.in2.empty() && b.in3.empty()) return fail(BGT_ERR_INVALID, "empty batch");
     BGT_CHECK(check_config2d(b.c2));
@@ -445,21 +465,32 @@ static int prepare(Batch &b, bool size_only = false)
             continue;
         }
         if (n2) {
-            BGT_HIP(hipMemcpy(b.d_items2, h2.data(), sizeof(Item2D) * n2, hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(b.d_io2, io2.data(), sizeof(IO2D) * n2, hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(b.d_blk2, blk2.data(), sizeof(int) * blk2.size(), hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(b.d_ublk2, ublk2.data(), sizeof(int) * ublk2.size(), hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(b.d_tiles2, t2.data(), sizeof(Tile2D) * t2.size(), hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(b.d_q2, &b.q2, sizeof(Queue), hipMemcpyHostToDevice));
+            stage(b.d_items2, h2.data(), sizeof(Item2D) * n2);
+            stage(b.d_io2, io2.data(), sizeof(IO2D) * n2);
+            stage(b.d_blk2, blk2.data(), sizeof(int) * blk2.size());
+            stage(b.d_ublk2, ublk2.data(), sizeof(int) * ublk2.size());
+            stage(b.d_tiles2, t2.data(), sizeof(Tile2D) * t2.size());
+            stage(b.d_q2, &b.q2, sizeof(Queue));
         }
         if (n3) {
-            BGT_HIP(hipMemcpy(d_items3, lay3.items.data(), sizeof(SItem3D) * n3, hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(d_geo3, lay3.geo.data(), sizeof(int2) * n3, hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(d_io3, io3.data(), sizeof(SIO) * n3, hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(d_blk3, lay3.blk.data(), sizeof(int) * lay3.blk.size(), hipMemcpyHostToDevice));
-            BGT_HIP(hipMemcpy(d_ublk3, lay3.ublk.data(), sizeof(int) * lay3.ublk.size(), hipMemcpyHostToDevice));
+            stage(d_items3, lay3.items.data(), sizeof(SItem3D) * n3);
+            stage(d_geo3, lay3.geo.data(), sizeof(int2) * n3);
+            stage(d_io3, io3.data(), sizeof(SIO) * n3);
+            stage(d_blk3, lay3.blk.data(), sizeof(int) * lay3.blk.size());
+            stage(d_ublk3, lay3.ublk.data(), sizeof(int) * lay3.ublk.size());
         }
     }
+    if (host_stage.size() > b.h_stage_bytes) {
+        if (b.h_stage) (void)hipHostFree(b.h_stage);
+        b.h_stage = nullptr;
+        b.h_stage_bytes = 0;
+        if (hipHostMalloc((void **)&b.h_stage, host_stage.size(), hipHostMallocDefault) != hipSuccess)
+            return fail(BGT_ERR_NOMEM, "pinned staging of %zu bytes failed", host_stage.size());
+        b.h_stage_bytes = host_stage.size();
+    }
+    if (!host_stage.empty()) std::memcpy(b.h_stage, host_stage.data(), host_stage.size());
+    for (const Staged &t : staged)
+        BGT_HIP(hipMemcpyAsync(t.dst, b.h_stage + t.off, t.n, hipMemcpyHostToDevice, s));
     if (n2)
         b.grid2 = occupancy_grid(b.c2.rows == 8 ? (const void *)thin2d_queue_pass<8>
                                                 : (const void *)thin2d_queue_pass<16>,
@@ -601,7 +632,7 @@ static int collect_timing(Batch &b)
 
 int batch_run(Batch &b, hipStream_t s, bool use_graph)
 {
-    BGT_CHECK(prepare(b));
+    BGT_CHECK(prepare(b, s));
     const int n2 = (int)b.in2.size(), n3 = (int)b.in3.size();
     if (b.timing) use_graph = false;
     b.ev_marks.clear();
@@ -679,7 +710,23 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
                            b.d_ublk2);
     BGT_HIP(hipGetLastError());
     BGT_HIP(hipStreamWaitEvent(s, b.ev_join, 0));
+    const size_t nst = (size_t)(n2 + n3) * ST_INTS;
+    if (nst > b.h_states_ints) {
+        if (b.h_states) (void)hipHostFree(b.h_states);
+        b.h_states = nullptr;
+        b.h_states_ints = 0;
+        if (hipHostMalloc((void **)&b.h_states, nst * sizeof(int), hipHostMallocDefault) != hipSuccess)
+            return fail(BGT_ERR_NOMEM, "pinned state copy failed");
+        b.h_states_ints = nst;
+    }
+    if (n2)
+        BGT_HIP(hipMemcpyAsync(b.h_states, b.d_states2, (size_t)n2 * ST_INTS * sizeof(int),
+                               hipMemcpyDeviceToHost, s));
+    if (n3)
+        BGT_HIP(hipMemcpyAsync(b.h_states + (size_t)n2 * ST_INTS, b.e3.states,
+                               (size_t)n3 * ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
+    b.ran = true;
     if (b.timing) BGT_CHECK(collect_timing(b));
     return BGT_OK;
 }
@@ -693,17 +740,9 @@ void batch_kernel_time(const Batch &b, double *ms2d, int *n2d, double *ms3d, int
 
 int batch_iterations(Batch &b, int *out)
 {
-    if (!b.ready) return fail(BGT_ERR_INVALID, "batch has not run");
-    const int n2 = (int)b.in2.size(), n3 = (int)b.in3.size();
-    std::vector<int> st((size_t)std::max(n2, n3) * ST_INTS);
-    if (n2) {
-        BGT_HIP(hipMemcpy(st.data(), b.d_states2, (size_t)n2 * ST_INTS * sizeof(int), hipMemcpyDeviceToHost));
-        for (int i = 0; i < n2; ++i) out[i] = st[(size_t)i * ST_INTS + ST_ITERS];
-    }
-    if (n3) {
-        BGT_HIP(hipMemcpy(st.data(), b.e3.states, (size_t)n3 * ST_INTS * sizeof(int), hipMemcpyDeviceToHost));
-        for (int i = 0; i < n3; ++i) out[n2 + i] = st[(size_t)i * ST_INTS + ST_ITERS];
-    }
+    if (!b.ready || !b.ran) return fail(BGT_ERR_INVALID, "batch has not run");
+    const int n = (int)(b.in2.size() + b.in3.size());
+    for (int i = 0; i < n; ++i) out[i] = b.h_states[(size_t)i * ST_INTS + ST_ITERS];
     return BGT_OK;
 }
 
@@ -745,7 +784,7 @@ size_t batch_device_bytes(Batch &b) { return b.ready ? b.bytes : 0; }
 
 int batch_plan_bytes(Batch &b, size_t *bytes)
 {
-    BGT_CHECK(prepare(b, true));
+    BGT_CHECK(prepare(b, nullptr, true));
     *bytes = b.bytes;
     return BGT_OK;
 }

@@ -30,6 +30,8 @@
 #include "batch.h"
 #include "common.h"
 #include "server.h"
+#include "thin2d.h"
+#include "thin3d.h"
 
 namespace bgt {
 
@@ -77,7 +79,32 @@ struct Server {
     size_t peak_in_use = 0;
 
     void worker(int cls);
+    void finish(const std::vector<Request *> &take, const std::vector<int> &iters, int rc);
 };
+
+// Marks a batch's requests done (with the status and, on failure, the
+// worker thread's error message) and releases their memory.
+void Server::finish(const std::vector<Request *> &take, const std::vector<int> &iters, int rc)
+{
+    const double t = now_ms();
+    const std::string err = rc == BGT_OK ? std::string() : std::string(bgt_last_error());
+    {
+        std::lock_guard<std::mutex> lk(m);
+        for (size_t k = 0; k < take.size(); ++k) {
+            Request *r = take[k];
+            r->status = rc;
+            r->error = err;
+            r->iterations = iters[k];
+            r->t_done = t;
+            r->done = true;
+            in_use -= r->bytes;
+        }
+        completed += take.size();
+        ++batches;
+    }
+    cv_done.notify_all();
+    cv_work.notify_all();
+}
 
 void Server::worker(int cls)
 {
@@ -108,6 +135,17 @@ void Server::worker(int cls)
             peak_in_use = std::max(peak_in_use, in_use);
         }
         int rc = s ? BGT_OK : fail(BGT_ERR_HIP, "worker stream creation failed");
+        iters.assign(take.size(), 0);
+        if (rc == BGT_OK && take.size() == 1) {
+            // a batch of one takes the per-item path (no batch layout, no
+            // table uploads); its result is the same
+            Request *r = take[0];
+            rc = r->cls == 0 ? thin2d_device_u8(r->in, r->in_pitch, r->w, r->h, r->out,
+                                                r->out_pitch, s, &iters[0], c2)
+                             : thin3d_device_u8(r->in, r->w, r->h, r->d, r->out, s, &iters[0], c3);
+            finish(take, iters, rc);
+            continue;
+        }
         batch_clear(*b);
         for (Request *r : take) {
             if (rc != BGT_OK) break;
@@ -134,26 +172,8 @@ void Server::worker(int cls)
             batch_set_arena(*b, arena, arena_bytes);
             rc = batch_run(*b, s, false);
         }
-        iters.assign(take.size(), 0);
         if (rc == BGT_OK) rc = batch_iterations(*b, iters.data());
-        const double t = now_ms();
-        const std::string err = rc == BGT_OK ? std::string() : std::string(bgt_last_error());
-        {
-            std::lock_guard<std::mutex> lk(m);
-            for (size_t k = 0; k < take.size(); ++k) {
-                Request *r = take[k];
-                r->status = rc;
-                r->error = err;
-                r->iterations = iters[k];
-                r->t_done = t;
-                r->done = true;
-                in_use -= r->bytes;
-            }
-            completed += take.size();
-            ++batches;
-        }
-        cv_done.notify_all();
-        cv_work.notify_all();
+        finish(take, iters, rc);
     }
     batch_set_arena(*b, nullptr, 0);
     b.reset();
