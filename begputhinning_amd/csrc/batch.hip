@@ -497,7 +497,6 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
                                  b.c2.waves * 64, lds2, (int)b.ntiles2);
     if (!b.h_live && hipHostMalloc((void **)&b.h_live, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(BGT_ERR_NOMEM, "pinned allocation failed");
-    if (!b.side) BGT_HIP(hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking));
     if (!b.ev_fork) BGT_HIP(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
     if (!b.ev_join) BGT_HIP(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
     for (int k = 0; k < 2; ++k) {
@@ -658,9 +657,17 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         b.bound3 = b.e3.capacity;
     }
 
-    // 3D pipeline on the side stream, forked from and joined into `s`
-    BGT_HIP(hipEventRecord(b.ev_fork, s));
-    BGT_HIP(hipStreamWaitEvent(b.side, b.ev_fork, 0));
+    // With both kinds of items, the 3D pipeline runs on a side stream forked
+    // from and joined into `s`; a batch of one kind uses `s` alone (no extra
+    // stream, so a caller with one stream per worker keeps one hardware queue
+    // per worker).
+    const bool fork = n2 > 0 && n3 > 0;
+    if (fork && !b.side) BGT_HIP(hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking));
+    hipStream_t s3 = fork ? b.side : s;
+    if (fork) {
+        BGT_HIP(hipEventRecord(b.ev_fork, s));
+        BGT_HIP(hipStreamWaitEvent(b.side, b.ev_fork, 0));
+    }
     bool live2 = n2 > 0, live3 = n3 > 0;
     long max_vox = 0;
     for (auto &v : b.in2) max_vox = std::max<long>(max_vox, (long)v.w * v.h);
@@ -680,10 +687,10 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
             BGT_HIP(hipEventRecord(b.ev_poll2[slot], s));
         }
         if (live3) {
-            BGT_CHECK(launch_chunk(b, true, use_graph, b.side));
+            BGT_CHECK(launch_chunk(b, true, use_graph, s3));
             BGT_HIP(hipMemcpyAsync(hs + 8, b.e3.ctl, SC_INTS * sizeof(int), hipMemcpyDeviceToHost,
-                                   b.side));
-            BGT_HIP(hipEventRecord(b.ev_poll3[slot], b.side));
+                                   s3));
+            BGT_HIP(hipEventRecord(b.ev_poll3[slot], s3));
         }
         const int prev = slot ^ 1;
         const int *hp = b.h_live + prev * 32;
@@ -703,13 +710,13 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         slot = prev;
         if (++rounds > max_rounds) return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
     }
-    if (n3) BGT_CHECK(sparse3d_end(b.e3, b.side));
-    BGT_HIP(hipEventRecord(b.ev_join, b.side));
+    if (n3) BGT_CHECK(sparse3d_end(b.e3, s3));
+    if (fork) BGT_HIP(hipEventRecord(b.ev_join, s3));
     if (b.nublk2)
         hipLaunchKernelGGL(batch_unpack2d, dim3(b.nublk2), dim3(256), 0, s, b.d_items2, b.d_io2,
                            b.d_ublk2);
     BGT_HIP(hipGetLastError());
-    BGT_HIP(hipStreamWaitEvent(s, b.ev_join, 0));
+    if (fork) BGT_HIP(hipStreamWaitEvent(s, b.ev_join, 0));
     const size_t nst = (size_t)(n2 + n3) * ST_INTS;
     if (nst > b.h_states_ints) {
         if (b.h_states) (void)hipHostFree(b.h_states);
