@@ -72,6 +72,7 @@ SIGNATURES = {
     "bgt_batch_iterations": (_i, [_vp, _ip]),
     "bgt_batch_device_bytes": (_i, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     "bgt_batch_destroy": (_i, [_vp]),
+    "bgt_batch_autotune": (_i, [_vp, _vp, _i, _ip, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_double)]),
     "bgt_batch_set_timing": (_i, [_vp, _i]),
     "bgt_batch_kernel_time": (_i, [_vp, ctypes.POINTER(ctypes.c_double), _ip,
                                    ctypes.POINTER(ctypes.c_double), _ip]),

@@ -47,6 +47,18 @@ class ThinningBatch:
         s = stream if stream is not None else torch.cuda.current_stream()
         check(lib().bgt_batch_run(self._h, ctypes.c_void_p(s.cuda_stream), 1 if use_graph else 0))
 
+    def autotune(self, stream=None, reps=2):
+        """Pick the fastest launch shapes for this batch (the outputs hold a
+        complete result afterwards); returns the chosen shapes and time."""
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream()
+        v = [ctypes.c_int(0) for _ in range(4)]
+        ms = ctypes.c_double(0.0)
+        check(lib().bgt_batch_autotune(self._h, ctypes.c_void_p(st.cuda_stream), reps,
+                                       *[ctypes.byref(x) for x in v], ctypes.byref(ms)))
+        return {"waves": v[0].value, "rows": v[1].value, "iters_per_pass": v[2].value,
+                "blocks_per_cu_3d": v[3].value, "ms": ms.value}
+
     def iterations(self):
         out = np.zeros(self.n, np.int32)
         check(lib().bgt_batch_iterations(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))

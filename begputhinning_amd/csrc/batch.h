@@ -31,5 +31,8 @@ int batch_plan_bytes(Batch &b, size_t *bytes);
 void batch_set_arena(Batch &b, void *mem, size_t bytes);
 // Remove all items (keeps streams, events and the arena).
 void batch_clear(Batch &b);
+// Time the batch under candidate launch shapes and keep the fastest.
+int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D *best3,
+                   double *best_ms, int *tried);
 
 }  // namespace bgt

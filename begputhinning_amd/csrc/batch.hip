@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <functional>
 #include <numeric>
@@ -784,6 +785,76 @@ int batch_set_config(Batch &b, const Config2D &c2, const Config3D &c3)
     b.release();
     b.c2 = c2;
     b.c3 = c3;
+    return BGT_OK;
+}
+
+// Launch-shape autotuner: runs the batch under candidate 2D shapes (waves,
+// register rows, iterations per pass) with the current 3D shape, then under
+// candidate 3D step grids with the best 2D shape, and keeps the fastest.
+// Every shape gives the same results, so only time is compared; shapes that
+// do not fit an item (a band thinner than its halo) are skipped.  Each
+// candidate is run once to lay the batch out and then `reps` times, taking
+// the fastest wall time of a whole run.  The batch's current shape is timed
+// first and is replaced only by a shape at least 3% faster, so that the
+// choice does not flip between near-equal shapes from run to run.
+int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D *best3,
+                   double *best_ms, int *tried)
+{
+    if (b.in2.empty() && b.in3.empty()) return fail(BGT_ERR_INVALID, "empty batch");
+    reps = std::max(1, reps);
+    static const int k2[][3] = {{4, 16, 4}, {4, 16, 3}, {4, 16, 2}, {4, 16, 6}, {4, 8, 2},
+                                {4, 8, 3},  {4, 8, 4},  {2, 16, 3}, {2, 16, 4}, {4, 16, 8}};
+    static const int k3[] = {0, 2, 4, 8, 16};
+    Config2D c2 = b.c2;
+    Config3D c3 = b.c3;
+    double best = 1e30;
+    int n = 0;
+    auto trial = [&](const Config2D &t2, const Config3D &t3) {
+        if (n && t2.waves == c2.waves && t2.rows == c2.rows && t2.iters_per_pass == c2.iters_per_pass &&
+            t3.blocks_per_cu == c3.blocks_per_cu)
+            return;  // the current shape, already timed
+        if (batch_set_config(b, t2, t3) != BGT_OK) return;
+        if (batch_run(b, s, false) != BGT_OK) return;  // lays out; shape may not fit
+        double t = 1e30;
+        for (int r = 0; r < reps; ++r) {
+            const auto t0 = std::chrono::steady_clock::now();
+            if (batch_run(b, s, false) != BGT_OK) return;
+            const auto t1 = std::chrono::steady_clock::now();
+            t = std::min(t, std::chrono::duration<double, std::milli>(t1 - t0).count());
+        }
+        const bool first = n == 0;
+        ++n;
+        if (first || t < best * 0.97) {
+            best = t;
+            c2 = t2;
+            c3 = t3;
+        }
+    };
+    const Config3D start3 = b.c3;
+    trial(b.c2, start3);  // the current shape first
+    if (!b.in2.empty()) {
+        for (const auto &k : k2) {
+            Config2D t2;
+            t2.waves = k[0];
+            t2.rows = k[1];
+            t2.iters_per_pass = k[2];
+            trial(t2, start3);
+        }
+    }
+    if (!b.in3.empty()) {
+        const Config2D base2 = c2;
+        for (int bpc : k3) {
+            Config3D t3 = start3;
+            t3.blocks_per_cu = bpc;
+            trial(base2, t3);
+        }
+    }
+    if (!n) return fail(BGT_ERR_INVALID, "no launch shape fits this batch");
+    BGT_CHECK(batch_set_config(b, c2, c3));
+    *best2 = c2;
+    *best3 = c3;
+    *best_ms = best;
+    if (tried) *tried = n;
     return BGT_OK;
 }
 

@@ -238,6 +238,22 @@ int bgt_batch_kernel_time(bgt_batch b, double *ms2d, int *launches2d, double *ms
     return BGT_OK;
 }
 
+int bgt_batch_autotune(bgt_batch b, void *stream, int reps, int *waves, int *rows,
+                       int *iters_per_pass, int *blocks_per_cu, double *best_ms)
+{
+    if (!b) return fail(BGT_ERR_INVALID, "null batch");
+    Config2D c2;
+    Config3D c3;
+    double ms = 0;
+    BGT_CHECK(batch_autotune(*b->b, (hipStream_t)stream, reps, &c2, &c3, &ms, nullptr));
+    if (waves) *waves = c2.waves;
+    if (rows) *rows = c2.rows;
+    if (iters_per_pass) *iters_per_pass = c2.iters_per_pass;
+    if (blocks_per_cu) *blocks_per_cu = c3.blocks_per_cu;
+    if (best_ms) *best_ms = ms;
+    return BGT_OK;
+}
+
 int bgt_batch_destroy(bgt_batch b)
 {
     if (!b) return BGT_OK;

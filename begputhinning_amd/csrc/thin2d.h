@@ -15,7 +15,7 @@ struct Tile2D;
 struct Config2D {
     int waves = 4;           // wave64s per workgroup
     int rows = 16;           // register rows per lane (8, 16 or 32)
-    int iters_per_pass = 4;  // K: iterations fused per HBM pass (halo 4K rows)
+    int iters_per_pass = 3;  // K: iterations fused per HBM pass (halo 4K rows)
 };
 
 struct Plan2D {

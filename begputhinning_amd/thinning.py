@@ -171,7 +171,7 @@ def thin3d_device(vol, out=None, stream=None):
     return out, it.value
 
 
-def set_config2d(waves=4, rows=16, iters_per_pass=4):
+def set_config2d(waves=4, rows=16, iters_per_pass=3):
     """Launch shape of the 2D pass kernel for later calls (see bgt.h)."""
     check(lib().bgt_set_config2d(waves, rows, iters_per_pass))
 

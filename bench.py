@@ -53,12 +53,15 @@ def sha(a):
 
 
 def measured_traffic():
-    """HBM bytes per launch of the dominant kernel from the committed counter
-    summary, or None."""
+    """(HBM bytes per launch of the dominant kernel, launch shape it was
+    measured with) from the committed counter summary, or (None, None)."""
     if not os.path.exists(TRAFFIC_FILE):
-        return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(TRAFFIC_FILE).read(), re.M)
-    return float(m.group(1)) if m else None
+        return None, None
+    text = open(TRAFFIC_FILE).read()
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
+    sh = re.search(r"^launch_shape:\s*waves=(\d+) rows=(\d+) iters_per_pass=(\d+)", text, re.M)
+    shape = tuple(int(v) for v in sh.groups()) if sh else None
+    return (float(m.group(1)) if m else None), shape
 
 
 def serve_pool(synth):
@@ -170,6 +173,8 @@ def main():
     ap.add_argument("--frames", type=int, default=200, help="1024x1024 frames for p50/p99")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", type=int, default=0, help="replay pass chunks as a hipGraph")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of tuning them on the batch")
     ap.add_argument("--only-batch", action="store_true",
                     help="time the batched step only (no per-image loop, latency or CPU leg)")
     ap.add_argument("--serve-qps", type=float, default=1000.0,
@@ -219,6 +224,11 @@ def main():
         else:
             batch.add3d(t, o)
     stream = torch.cuda.current_stream()
+    tuned = None
+    if not args.no_autotune:
+        t_tune = time.perf_counter()
+        tuned = batch.autotune(stream, reps=2)  # untimed; results do not depend on it
+        log(f"autotune: {tuned} in {time.perf_counter() - t_tune:.2f}s")
     for _ in range(args.warmup):
         batch.run(stream, use_graph=bool(args.graph))
     step_ms = []
@@ -334,7 +344,13 @@ def main():
                          f"{cpu_s:.1f} s on one core"}
         log(f"cpu baseline {cpu['value']} Mvoxel/s; bit-exact items {ok}/{len(items)}")
 
-    traffic = measured_traffic()
+    traffic, traffic_shape = measured_traffic()
+    shape = ((tuned["waves"], tuned["rows"], tuned["iters_per_pass"]) if tuned
+             else (4, 16, 3))
+    if traffic is not None and traffic_shape != shape:
+        log(f"traffic figure was measured at launch shape {traffic_shape}, this run uses "
+            f"{shape}: not reported")
+        traffic = None
     line = {
         "metric": "bit-exact thinned Mvoxel/s, batched vs per-image, 1 MI355X; p50/p99 ms",
         "value": round(value, 1),
@@ -350,7 +366,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "mixed batch: 512 images U{128..2048}^2 + 16 volumes "
                                "U{64..384}^3, uint8, seeded", "items": len(items),
-                   "mvoxel": round(total_vox / 1e6, 1), "graph": bool(args.graph)},
+                   "mvoxel": round(total_vox / 1e6, 1), "graph": bool(args.graph),
+                   "launch_shapes": tuned or "defaults"},
         "per_image": per_image,
         "latency_1024x1024_ms": latency,
         "serving": serving,

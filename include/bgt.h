@@ -148,6 +148,15 @@ int bgt_batch_device_bytes(bgt_batch b, size_t *bytes);
 int bgt_batch_set_timing(bgt_batch b, int on);
 int bgt_batch_kernel_time(bgt_batch b, double *ms2d, int *launches2d, double *ms3d,
                           int *launches3d);
+/* Launch-shape autotuner (no reference counterpart; the reference hard-codes
+ * its shapes, 2D/LibThinning2D.cu:59-60, 3D/LibThinning3D.cu:50): runs the
+ * batch under candidate 2D shapes (waves, rows, iterations per pass) and 3D
+ * step grids, `reps` timed runs each after one layout run, keeps the fastest
+ * for the batch and reports it (any output pointer may be NULL).  Results do
+ * not depend on the shape.  The batch's outputs hold a complete result
+ * afterwards. */
+int bgt_batch_autotune(bgt_batch b, void *stream, int reps, int *waves, int *rows,
+                       int *iters_per_pass, int *blocks_per_cu, double *best_ms);
 int bgt_batch_destroy(bgt_batch b);
 
 /* ------------------------------------------------------------------------- */

@@ -72,3 +72,27 @@ def test_batch_timing_and_single_item_equality():
         single, _ = (B.thin2d_device(t) if t.dim() == 2 else B.thin3d_device(t))
         assert torch.equal(single, o)
     b.close()
+
+
+def test_autotune_keeps_results():
+    """The autotuner tries launch shapes on the batch itself; every shape gives
+    the same skeletons, and the batch keeps the fastest one."""
+    from begputhinning_amd.batch import ThinningBatch
+    items = make_items(3, n2=12, n3=3)
+    ins = [torch.from_numpy(a).cuda() for a in items]
+    outs = [torch.full_like(t, 9) for t in ins]
+    b = ThinningBatch()
+    for t, o in zip(ins, outs):
+        (b.add2d if t.dim() == 2 else b.add3d)(t, o)
+    best = b.autotune(reps=1)
+    print(f"[batch] autotune -> {best}", flush=True)
+    assert best["waves"] in (1, 2, 3, 4) and best["rows"] in (8, 16)
+    assert 1 <= best["iters_per_pass"] <= 16 and best["ms"] > 0
+    b.run(use_graph=False)
+    torch.cuda.synchronize()
+    its = b.iterations()
+    for k, (a, o) in enumerate(zip(items, outs)):
+        ref, rit = O.thin2d(a) if a.ndim == 2 else O.thin3d(a)
+        assert np.array_equal(o.cpu().numpy(), ref), k
+        assert its[k] == rit, k
+    b.close()

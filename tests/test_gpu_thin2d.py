@@ -15,9 +15,9 @@ def _gpu():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     import begputhinning_amd as B
-    B.set_config2d(4, 16, 4)
+    B.set_config2d(4, 16, 3)
     yield
-    B.set_config2d(4, 16, 4)
+    B.set_config2d(4, 16, 3)
 
 
 def thin_gpu(img):
@@ -60,7 +60,7 @@ def test_full_and_empty():
     check(np.zeros((77, 150), np.uint8), "all zeros")
 
 
-@pytest.mark.parametrize("cfg", [(1, 8, 1), (2, 8, 3), (4, 8, 4), (1, 16, 1), (3, 16, 2), (4, 16, 2),
+@pytest.mark.parametrize("cfg", [(1, 8, 1), (2, 8, 3), (4, 8, 4), (1, 16, 1), (3, 16, 2), (4, 16, 2), (4, 16, 4),
                                  (4, 16, 5)])
 def test_launch_shapes(cfg):
     import begputhinning_amd as B
@@ -72,7 +72,7 @@ def test_launch_shapes(cfg):
         rng = np.random.default_rng(5)
         check((rng.random((300, 200)) < 0.7).astype(np.uint8), f"config {cfg} random")
     finally:
-        B.set_config2d(4, 16, 4)
+        B.set_config2d(4, 16, 3)
 
 
 def test_pitched_input():
