@@ -116,6 +116,57 @@ __device__ __forceinline__ void unpack_chunk_u8(uint8_t *base, uint32_t n, uint3
 // Chunks a segment of n bytes at base can touch.
 __host__ __device__ inline uint32_t unpack_chunks(uint32_t n) { return n / 16 + 2; }
 
+// The same with one thread per aligned 64-byte block: one division per block,
+// the block's 64 pixels gathered from at most two lines, four uint4 stores.
+// Blocks at a segment end, or crossing more than one line end (lines
+// narrower than 64), store their in-segment bytes one by one.  Src also
+// provides uint64_t bits64(line, x): 64 pixels from x of the line (pixels
+// past the line's width may hold anything; they are masked here).
+template <class Src>
+__device__ __forceinline__ void unpack_block64_u8(uint8_t *base, uint32_t n, uint32_t w, uint32_t c,
+                                                  const Src &src)
+{
+    uint8_t *B = reinterpret_cast<uint8_t *>(reinterpret_cast<uintptr_t>(base) & ~(uintptr_t)63) +
+                 (size_t)c * 64;
+    const long f0 = (long)(B - base);
+    if (f0 >= (long)n || f0 + 64 <= 0) return;
+    if (f0 >= 0 && f0 + 64 <= (long)n) {
+        const uint32_t line = (uint32_t)f0 / w, x = (uint32_t)f0 - line * w;
+        bool fast = true;
+        uint64_t v;
+        if (x + 64 <= w) {
+            v = src.bits64(line, x);
+        } else if (w >= 64) {
+            const uint32_t k1 = w - x;  // 1..63 pixels left on this line
+            v = (src.bits64(line, x) & ((1ull << k1) - 1ull)) | (src.bits64(line + 1, 0) << k1);
+        } else {
+            fast = false;
+        }
+        if (fast) {
+            const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+            uint4 *q = reinterpret_cast<uint4 *>(B);
+            q[0] = make_uint4(spread4(lo), spread4(lo >> 4), spread4(lo >> 8), spread4(lo >> 12));
+            q[1] = make_uint4(spread4(lo >> 16), spread4(lo >> 20), spread4(lo >> 24), spread4(lo >> 28));
+            q[2] = make_uint4(spread4(hi), spread4(hi >> 4), spread4(hi >> 8), spread4(hi >> 12));
+            q[3] = make_uint4(spread4(hi >> 16), spread4(hi >> 20), spread4(hi >> 24), spread4(hi >> 28));
+            return;
+        }
+    }
+    const uint32_t fa = (uint32_t)(f0 < 0 ? 0 : f0);
+    const uint32_t fb = (uint32_t)(f0 + 64 < (long)n ? f0 + 64 : (long)n);
+    uint32_t line = fa / w, x = fa - line * w;
+    for (uint32_t f = fa; f < fb; ++f) {
+        B[f - f0] = (uint8_t)src.bit(line, x);
+        if (++x == w) {
+            x = 0;
+            ++line;
+        }
+    }
+}
+
+// Blocks a segment of n bytes at base can touch.
+__host__ __device__ inline uint32_t unpack_blocks(uint32_t n) { return n / 64 + 2; }
+
 // Pixels of an image's rows in its packed state (64-bit words, L per row).
 struct RowBits {
     const uint64_t *row0;
@@ -132,23 +183,32 @@ struct RowBits {
     {
         return (uint32_t)(row0[(size_t)y * L + (x >> 6)] >> (x & 63)) & 1u;
     }
+    // 64 pixels from x of row y; pixels past the row's last word read as 0
+    __device__ uint64_t bits64(uint32_t y, uint32_t x) const
+    {
+        const uint64_t *row = row0 + (size_t)y * L;
+        const uint32_t i = x >> 6, s = x & 63;
+        uint64_t v = row[i] >> s;
+        if (s && i + 1 < L) v |= row[i + 1] << (64 - s);
+        return v;
+    }
 };
 
-// Chunk t of a w x h uint8 image at `out` (row pitch `pitch`) from its packed
-// rows: the whole image is one segment when `flat` (pitch == w), else each row
-// is a segment of `uchunks` chunks.
+// Block t (64 bytes) of a w x h uint8 image at `out` (row pitch `pitch`) from
+// its packed rows: the whole image is one segment when `flat` (pitch == w),
+// else each row is a segment of `uchunks` blocks.
 __device__ __forceinline__ void unpack_image_u8(const uint64_t *buf, uint32_t L, int w, int h,
                                                 uint8_t *out, size_t pitch, uint32_t uchunks,
                                                 int flat, uint32_t t)
 {
     if (flat) {
         if (t >= uchunks) return;
-        unpack_chunk_u8(out, (uint32_t)w * (uint32_t)h, (uint32_t)w, t, RowBits{buf, L});
+        unpack_block64_u8(out, (uint32_t)w * (uint32_t)h, (uint32_t)w, t, RowBits{buf, L});
     } else {
         const uint32_t y = t / uchunks, c = t - y * uchunks;
         if (y >= (uint32_t)h) return;
-        unpack_chunk_u8(out + (size_t)y * pitch, (uint32_t)w, (uint32_t)w, c,
-                        RowBits{buf + (size_t)y * L, L});
+        unpack_block64_u8(out + (size_t)y * pitch, (uint32_t)w, (uint32_t)w, c,
+                          RowBits{buf + (size_t)y * L, L});
     }
 }
 
@@ -158,7 +218,7 @@ inline void unpack_image_grid(int w, int h, size_t pitch, int &flat, uint32_t &u
 {
     const long px = (long)w * h;
     flat = pitch == (size_t)w && px < 0x7fffffffL;
-    uchunks = unpack_chunks(flat ? (uint32_t)px : (uint32_t)w);
+    uchunks = unpack_blocks(flat ? (uint32_t)px : (uint32_t)w);
     threads = flat ? (long)uchunks : (long)uchunks * h;
 }
 

@@ -127,9 +127,17 @@ struct PlaneBits {
     {
         return (s0[g0 + y * P + (x >> 5)] >> (x & 31)) & 1u;
     }
+    // 64 voxels from x of line y (three words; the words after a line's data
+    // belong to the next line's margin or data: the caller masks them)
+    __device__ uint64_t bits64(uint32_t y, uint32_t x) const
+    {
+        const uint32_t j = g0 + y * P + (x >> 5), s = x & 31;
+        const uint64_t a = (uint64_t)s0[j] | ((uint64_t)s0[j + 1] << 32);
+        return s ? (a >> s) | ((uint64_t)s0[j + 2] << (64 - s)) : a;
+    }
 };
 
-// One thread per 16-byte chunk of a z-plane of the caller's buffer.
+// One thread per 64-byte block of a z-plane of the caller's buffer.
 __global__ __launch_bounds__(256) void sparse3d_unpack(const SItem3D *__restrict__ items,
                                                        const SIO *__restrict__ io,
                                                        const int *__restrict__ ublk,
@@ -139,13 +147,13 @@ __global__ __launch_bounds__(256) void sparse3d_unpack(const SItem3D *__restrict
     const SItem3D it = items[i];
     const SIO f = io[i];
     const uint32_t plane = (uint32_t)it.w * (uint32_t)it.h;
-    const uint32_t cpp = unpack_chunks(plane);
+    const uint32_t cpp = unpack_blocks(plane);
     const uint32_t t = (uint32_t)(blockIdx.x - f.ublk0) * 256u + threadIdx.x;
     const uint32_t z = t / cpp, c = t - z * cpp;
     if (z >= (uint32_t)it.d) return;
     const PlaneBits src{s0, (uint32_t)it.base + ((z + 1) * (uint32_t)(it.h + 2) + 1) * (uint32_t)it.P + 1,
                         (uint32_t)it.P};
-    unpack_chunk_u8(f.out + (size_t)z * plane, plane, (uint32_t)it.w, c, src);
+    unpack_block64_u8(f.out + (size_t)z * plane, plane, (uint32_t)it.w, c, src);
 }
 
 // Subiteration D (0..5 = U, N, E, D, S, W) over the list: src -> dst.
@@ -289,7 +297,7 @@ int sparse3d_layout(const int *w, const int *h, const int *d, int n, Layout3D &l
         const long threads = (long)h[i] * d[i] * (it.nh / 2);
         lay.blk0[i] = (int)lay.blk.size();
         for (long k = 0; k < (threads + 255) / 256; ++k) lay.blk.push_back(i);
-        const long uthreads = (long)d[i] * unpack_chunks((uint32_t)(w[i] * h[i]));
+        const long uthreads = (long)d[i] * unpack_blocks((uint32_t)(w[i] * h[i]));
         lay.ublk0[i] = (int)lay.ublk.size();
         for (long k = 0; k < (uthreads + 255) / 256; ++k) lay.ublk.push_back(i);
     }
