@@ -28,22 +28,28 @@ __device__ __forceinline__ uint32_t spread4(uint32_t b)
     return ((b & 0xfu) * 0x00204081u) & 0x01010101u;
 }
 
-// Bits of the n (1..64) bytes at p: bit k set iff p[k] != 0.
+// Bits of the n (1..64) bytes at p: bit k set iff p[k] != 0.  Reads the
+// 16-byte aligned blocks that overlap the run (at most five uint4 loads; an
+// aligned block holding one byte of the run cannot cross a page), turns the
+// bytes into an 80-bit mask and shifts it by the run's offset in the first
+// block.
 __device__ __forceinline__ uint64_t pack_run_u8(const uint8_t *p, int n)
 {
     const uintptr_t a = (uintptr_t)p;
-    const int r = (int)(a & 3);
-    const uint32_t *d = reinterpret_cast<const uint32_t *>(a - (uintptr_t)r);
-    const int nd = (r + n + 3) >> 2;  // dwords overlapping the run, 1..17
-    uint32_t D[17];
+    const int r = (int)(a & 15);
+    const uint4 *q = reinterpret_cast<const uint4 *>(a - (uintptr_t)r);
+    const int nq = (r + n + 15) >> 4;  // blocks overlapping the run, 1..5
+    uint32_t m[3] = {0u, 0u, 0u};      // bit i = byte i of the blocks nonzero
 #pragma unroll
-    for (int k = 0; k < 17; ++k) D[k] = k < nd ? d[k] : 0u;
-    uint32_t lo = 0, hi = 0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        lo |= nz4(__builtin_amdgcn_alignbyte(D[j + 1], D[j], (uint32_t)r)) << (4 * j);
-        hi |= nz4(__builtin_amdgcn_alignbyte(D[j + 9], D[j + 8], (uint32_t)r)) << (4 * j);
+    for (int k = 0; k < 5; ++k) {
+        if (k < nq) {
+            const uint4 v = q[k];
+            const uint32_t bits = nz4(v.x) | nz4(v.y) << 4 | nz4(v.z) << 8 | nz4(v.w) << 12;
+            m[k >> 1] |= bits << ((k & 1) * 16);
+        }
     }
+    const uint32_t lo = __builtin_amdgcn_alignbit(m[1], m[0], (uint32_t)r);
+    const uint32_t hi = __builtin_amdgcn_alignbit(m[2], m[1], (uint32_t)r);
     uint64_t v = ((uint64_t)hi << 32) | lo;
     if (n < 64) v &= (1ull << n) - 1ull;
     return v;
