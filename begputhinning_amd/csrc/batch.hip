@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void batch_pack2d(const Item2D *__restrict__ i
     const IO2D f = io[i];
     const int t = (blockIdx.x - f.blk0) * 256 + threadIdx.x;
     if (t >= it.h * it.L) return;
-    const int y = t >> it.lgL, x0 = (t & (it.L - 1)) * 64;
+    const int y = t / it.L, x0 = (t - y * it.L) * 64;
     const uint64_t v =
         x0 < it.w ? pack_run_u8(f.in + (size_t)y * f.in_pitch + x0, min(64, it.w - x0)) : 0ull;
     it.buf[0][t] = v;
@@ -434,7 +434,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
             v.buf[1] = cv.take<uint64_t>(p2[i].state_words * sizeof(uint64_t));
             v.chg = nullptr;
             v.st = b.d_states2 ? b.d_states2 + (size_t)i * ST_INTS : nullptr;
-            v.w = p2[i].w; v.h = p2[i].h; v.L = p2[i].L; v.lgL = p2[i].lgL;
+            v.w = p2[i].w; v.h = p2[i].h; v.L = p2[i].L;
             v.nbands = p2[i].nbands;
             v.band_rows = p2[i].band_rows;
             v.tile0 = tile0[i];

@@ -19,7 +19,7 @@ struct Config2D {
 };
 
 struct Plan2D {
-    int w = 0, h = 0, L = 1, lgL = 0;
+    int w = 0, h = 0, L = 1;  // L = 64-bit words per row
     int band_rows = 0, nbands = 0;
     size_t state_words = 0;
 };

@@ -225,9 +225,7 @@ int plan2d(int w, int h, const Config2D &cfg, Plan2D &pl)
     BGT_CHECK(check_config2d(cfg));
     pl.w = w;
     pl.h = h;
-    pl.L = pow2_ceil((w + 63) / 64);
-    pl.lgL = 0;
-    while ((1 << pl.lgL) < pl.L) pl.lgL++;
+    pl.L = (w + 63) / 64;  // words per row; a wave holds floor(64 / L) row segments
     pl.band_rows = band_rows_for(pl.L, cfg);
     if (pl.band_rows < 4 * cfg.iters_per_pass)
         return fail(BGT_ERR_INVALID, "2D band of %d rows is thinner than its halo %d",
@@ -271,7 +269,7 @@ int setup_item2d(const Plan2D &pl, void *ws, hipStream_t s, Item2D **d_item, uns
     v.buf[1] = cv.take<uint64_t>(pl.state_words * sizeof(uint64_t));
     v.chg = cv.take<uint8_t>((size_t)2 * pl.nbands);
     v.st = st;
-    v.w = pl.w; v.h = pl.h; v.L = pl.L; v.lgL = pl.lgL;
+    v.w = pl.w; v.h = pl.h; v.L = pl.L;
     v.nbands = pl.nbands;
     v.band_rows = pl.band_rows;
     v.tile0 = 0;

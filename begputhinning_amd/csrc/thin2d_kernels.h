@@ -2,14 +2,14 @@
 //
 // State layout in HBM: bit-packed rows, bit k of 64-bit word c of row y is
 // pixel x = 64*c + k (the reference packs 32-pixel words the same way,
-// 2D/CUDA_DeviceHelperFunctions.cuh:101).  A row has L words, L = the next
-// power of two >= ceil(width/64); words past the width are zero and stay zero
-// because a subiteration only clears bits.
+// 2D/CUDA_DeviceHelperFunctions.cuh:101).  A row has L = ceil(width/64)
+// words; bits past the width are zero and stay zero because a subiteration
+// only clears bits.
 //
 // Work unit: a "band" of consecutive rows of one image, processed by one
 // workgroup of NW wave64s.  Lane (g, c) = (lane / L, lane % L) of wave v owns
-// word column c of segment q = v*G + g (G = 64 / L segments per wave); a
-// segment is R consecutive rows held in registers.  Per subiteration each
+// word column c of segment q = v*G + g (G = floor(64 / L) segments per wave,
+// the lanes past G*L idle); a segment is R consecutive rows held in registers.  Per subiteration each
 // segment exchanges its top and bottom rows with its neighbours through
 // double-buffered LDS (one barrier per subiteration), W/E pixel neighbours come
 // from the adjacent lane with DPP wave_shr:1 / wave_shl:1 plus v_alignbit, and
@@ -34,7 +34,7 @@ struct Item2D {
     uint64_t *buf[2];   // ping-pong packed state (L words per row)
     uint8_t *chg;       // per band change flags, [2][nbands] (pass parity)
     int *st;            // item state (ST_*)
-    int w, h, L, lgL;   // L lanes per row, lgL = log2(L)
+    int w, h, L;        // L = 64-bit words (lanes) per row
     int nbands;         // bands covering the image
     int band_rows;      // interior rows per band
     int tile0;          // id of band 0 in the batch's tile arrays
@@ -172,8 +172,12 @@ __device__ int band_work(const Item2D &it, int band, int K, int par, uint64_t *e
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int NW = blockDim.x >> 6;
-    const int L = it.L, lgL = it.lgL, G = 64 >> lgL, S = NW * G, SL = NW * 64;
-    const int g = lane >> lgL, c = lane & (L - 1), q = wv * G + g;
+    // G = floor(64 / L) segments per wave; the lanes past G*L are idle: they
+    // hold zeros, load and store nothing and own no interior rows, and their
+    // words reach no real lane (segment-end carries are masked)
+    const int L = it.L, G = 64 / L, S = NW * G, SL = NW * 64;
+    const int g = lane / L, c = lane - g * L, q = wv * G + g;
+    const bool active = g < G;
     const int H = 4 * K;
     const bool whole = it.nbands == 1;  // halo rows all outside the image
     const int y_reg0 = band * it.band_rows - H + q * R;  // image row of register row 0
@@ -181,7 +185,8 @@ __device__ int band_work(const Item2D &it, int band, int K, int par, uint64_t *e
     const uint32_t me = c == L - 1 ? 0u : 0xffffffffu;
     // rows of this segment inside the band interior, as a bit mask
     const int ra = max(0, H - q * R), rb = min(R, H + it.band_rows - q * R);
-    const uint32_t interior = rb > ra ? (uint32_t)(((1ull << rb) - 1ull) & ~((1ull << ra) - 1ull)) : 0u;
+    const uint32_t interior =
+        (active && rb > ra) ? (uint32_t)(((1ull << rb) - 1ull) & ~((1ull << ra) - 1ull)) : 0u;
     const uint64_t *src = par ? it.buf[1] : it.buf[0];
     uint64_t *dst = par ? it.buf[0] : it.buf[1];
 
@@ -190,18 +195,20 @@ __device__ int band_work(const Item2D &it, int band, int K, int par, uint64_t *e
     for (int r = 0; r < R; ++r) {
         const int y = y_reg0 + r;
         uint64_t v = 0;
-        if (y >= 0 && y < it.h) v = src[(size_t)y * L + c];
+        if (active && y >= 0 && y < it.h) v = src[(size_t)y * L + c];
         lo[r] = (uint32_t)v;
         hi[r] = (uint32_t)(v >> 32);
     }
     // edges written after subiteration s live in buffer (s + 1) & 1; buffer 0
     // holds the initial rows.  Layout [buf][top/bot][NW*64]; the entry of lane
-    // (g, c) of wave v is v*64 + lane, segment q-1 is L entries below.
+    // (g, c) of wave v is v*64 + g*L + c = tid, so segment q's column c is at
+    // (q / G)*64 + (q % G)*L + c.  Idle lanes only read their own entries.
     const int me_idx = tid;
-    const int up_idx = q > 0 ? me_idx - L : me_idx;
-    const int dn_idx = q + 1 < S ? me_idx + L : me_idx;
-    const uint32_t up_ok = q > 0 ? 0xffffffffu : 0u;
-    const uint32_t dn_ok = q + 1 < S ? 0xffffffffu : 0u;
+    const bool has_up = active && q > 0, has_dn = active && q + 1 < S;
+    const int up_idx = has_up ? ((q - 1) / G) * 64 + ((q - 1) % G) * L + c : me_idx;
+    const int dn_idx = has_dn ? ((q + 1) / G) * 64 + ((q + 1) % G) * L + c : me_idx;
+    const uint32_t up_ok = has_up ? 0xffffffffu : 0u;
+    const uint32_t dn_ok = has_dn ? 0xffffffffu : 0u;
     edges[(0 * 2 + 0) * SL + me_idx] = ((uint64_t)hi[0] << 32) | lo[0];
     edges[(0 * 2 + 1) * SL + me_idx] = ((uint64_t)hi[R - 1] << 32) | lo[R - 1];
     if (tid == 0) {
