@@ -77,41 +77,66 @@ __device__ __forceinline__ W64 shift_e(W64 v, uint32_t carry_mask)
 }
 
 // ---- the deletion rule ---------------------------------------------------
-// North template (2D/CUDA_SubiterationKernels2D.cuh:15-40) over the clockwise
-// neighbour labels x0..x7 = N, NE, E, SE, S, SW, W, NW.  The South, East and
-// West rules of the reference are this rule with the labels advanced by 4, 2
-// and 6 (checked exhaustively against the transcribed reference rules).
-__device__ __forceinline__ uint32_t north_rule(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3,
-                                               uint32_t x4, uint32_t x5, uint32_t x6, uint32_t x7)
+
+// v_bitop3_b32: any bitwise function of three operands in one instruction.
+// The truth table is written over the operand columns TA, TB, TC: bit
+// (a<<2 | b<<1 | c) of the table is the result for input bits a, b, c.
+constexpr unsigned TA = 0xf0u, TB = 0xccu, TC = 0xaau;
+template <unsigned T>
+__device__ __forceinline__ uint32_t lut3(uint32_t a, uint32_t b, uint32_t c)
 {
-    uint32_t a = ~x1 & (x2 ^ x3 ^ ~x4) & (x3 ^ x5) & (x3 ^ x6) & ~x7;
-    uint32_t b = x2 & x4 & (x6 | ~x7);
-    uint32_t c = x6 & ~x1 & x4;
-    uint32_t d = (x1 ^ x5) & (x2 ^ x5) & (x4 ^ ~x5) & ~x6 & ~x7;
-    uint32_t e = ~x1 & ~x2 & ~x3 & x6 & x7;
-    return ~x0 & (a | b | c | d | e);
+    return __builtin_amdgcn_bitop3_b32(a, b, c, T & 0xffu);
+}
+
+// Pixels of `ctr` that survive the North subiteration, over the clockwise
+// neighbour labels x0..x7 = N, NE, E, SE, S, SW, W, NW.  The North template
+// of 2D/CUDA_SubiterationKernels2D.cuh:15-40 deletes a pixel when
+//   ~x0 & (a | b | c | d | e),  a = ~x1 ~x7 (x2^x3^~x4) (x3^x5) (x3^x6),
+//   b = x2 x4 (x6|~x7),  c = x4 x6 ~x1,  d = ~x6 ~x7 (x1^x5) (x2^x5) (x4^~x5),
+//   e = ~x1 ~x2 ~x3 x6 x7;
+// the result is ctr & ~that, as a network of 14 three-input gates.
+__device__ __forceinline__ uint32_t north_keep(uint32_t ctr, uint32_t x0, uint32_t x1, uint32_t x2,
+                                               uint32_t x3, uint32_t x4, uint32_t x5, uint32_t x6,
+                                               uint32_t x7)
+{
+    const uint32_t g1 = lut3<TA ^ TB ^ ~TC>(x2, x3, x4);          // x2 ^ x3 ^ ~x4
+    const uint32_t g2 = lut3<(TA ^ TB) & (TA ^ TC)>(x3, x5, x6);  // (x3^x5) (x3^x6)
+    const uint32_t g3 = lut3<~TA & ~TB & TC>(x1, x7, g1);         // ~x1 ~x7 g1
+    const uint32_t g4 = lut3<(TA ^ TC) & (TB ^ TC)>(x1, x2, x5);  // (x1^x5) (x2^x5)
+    const uint32_t g5 = lut3<~(TA ^ TB) & ~TC>(x4, x5, x6);       // (x4^~x5) ~x6
+    const uint32_t g6 = lut3<~TA & ~TB & ~TC>(x1, x2, x3);        // ~x1 ~x2 ~x3
+    const uint32_t g7 = lut3<TA & (TB | ~TC)>(x2, x6, x7);        // x2 (x6|~x7)
+    const uint32_t g8 = lut3<(~TA & TB) | TC>(x1, x6, g7);        // b|c = x4 g8
+    const uint32_t d = lut3<TA & TB & ~TC>(g4, g5, x7);
+    const uint32_t e = lut3<TA & TB & TC>(g6, x6, x7);
+    const uint32_t ad = lut3<(TA & TB) | TC>(g2, g3, d);          // a | d
+    const uint32_t bce = lut3<(TA & TB) | TC>(x4, g8, e);         // b | c | e
+    // ctr & ~(~x0 & (ad | bce)) = ctr & (x0 | ~ad) & (x0 | ~bce)
+    const uint32_t k = lut3<TA & (TB | ~TC)>(ctr, x0, ad);
+    return lut3<TA & (TB | ~TC)>(k, x0, bce);
 }
 
 // direction D: 0 = N, 1 = S, 2 = E, 3 = W (the reference's order,
-// 2D/CUDA_ThinningKernelMultiBlock2D.cuh:104-135)
+// 2D/CUDA_ThinningKernelMultiBlock2D.cuh:104-135).  The South, East and West
+// rules of the reference are the North rule with the labels advanced by 4, 2
+// and 6 (checked exhaustively against the transcribed reference rules).
 template <int D>
-__device__ __forceinline__ uint32_t rule(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3,
-                                         uint32_t x4, uint32_t x5, uint32_t x6, uint32_t x7)
+__device__ __forceinline__ uint32_t keep(uint32_t ctr, uint32_t x0, uint32_t x1, uint32_t x2,
+                                         uint32_t x3, uint32_t x4, uint32_t x5, uint32_t x6,
+                                         uint32_t x7)
 {
-    if (D == 0) return north_rule(x0, x1, x2, x3, x4, x5, x6, x7);
-    if (D == 1) return north_rule(x4, x5, x6, x7, x0, x1, x2, x3);
-    if (D == 2) return north_rule(x2, x3, x4, x5, x6, x7, x0, x1);
-    return north_rule(x6, x7, x0, x1, x2, x3, x4, x5);
+    if (D == 0) return north_keep(ctr, x0, x1, x2, x3, x4, x5, x6, x7);
+    if (D == 1) return north_keep(ctr, x4, x5, x6, x7, x0, x1, x2, x3);
+    if (D == 2) return north_keep(ctr, x2, x3, x4, x5, x6, x7, x0, x1);
+    return north_keep(ctr, x6, x7, x0, x1, x2, x3, x4, x5);
 }
 
 // One subiteration of direction D over the R register rows of a segment.
 // up_edge / dn_edge are the rows just above / below the segment (new values
-// of the previous subiteration, from the neighbouring segments).  Bit r of
-// `rows_changed` is set when row r lost a pixel.
+// of the previous subiteration, from the neighbouring segments).
 template <int D, int R>
 __device__ __forceinline__ void subiteration(uint32_t (&lo)[R], uint32_t (&hi)[R], W64 up_edge,
-                                             W64 dn_edge, uint32_t mw, uint32_t me,
-                                             uint32_t &rows_changed)
+                                             W64 dn_edge, uint32_t mw, uint32_t me)
 {
     W64 up = up_edge, upw = shift_w(up, mw), upe = shift_e(up, me);
     W64 cp = {lo[0], hi[0]}, cpw = shift_w(cp, mw), cpe = shift_e(cp, me);
@@ -120,18 +145,27 @@ __device__ __forceinline__ void subiteration(uint32_t (&lo)[R], uint32_t (&hi)[R
         W64 dn = (r + 1 < R) ? W64{lo[r + 1 < R ? r + 1 : 0], hi[r + 1 < R ? r + 1 : 0]} : dn_edge;
         W64 dnw = shift_w(dn, mw), dne = shift_e(dn, me);
         // x0=N=up, x1=NE, x2=E, x3=SE, x4=S=dn, x5=SW, x6=W, x7=NW
-        const uint32_t mlo = rule<D>(up.lo, upe.lo, cpe.lo, dne.lo, dn.lo, dnw.lo, cpw.lo, upw.lo);
-        const uint32_t mhi = rule<D>(up.hi, upe.hi, cpe.hi, dne.hi, dn.hi, dnw.hi, cpw.hi, upw.hi);
-        const uint32_t dlo = mlo & cp.lo, dhi = mhi & cp.hi;
-        lo[r] = cp.lo ^ dlo;
-        hi[r] = cp.hi ^ dhi;
-        rows_changed |= ((dlo | dhi) != 0u ? 1u : 0u) << r;
+        lo[r] = keep<D>(cp.lo, up.lo, upe.lo, cpe.lo, dne.lo, dn.lo, dnw.lo, cpw.lo, upw.lo);
+        hi[r] = keep<D>(cp.hi, up.hi, upe.hi, cpe.hi, dne.hi, dn.hi, dnw.hi, cpw.hi, upw.hi);
         up = cp; upw = cpw; upe = cpe;
         cp = dn; cpw = dnw; cpe = dne;
         // rows are independent; keep the scheduler from interleaving all of
         // them (that multiplies live registers without adding useful ILP)
         __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+// Pixels in the rows of `rows` (bit r = register row r).  Subiterations only
+// clear pixels, so an iteration deleted one of them iff this count dropped.
+template <int R>
+__device__ __forceinline__ uint32_t count_rows(const uint32_t (&lo)[R], const uint32_t (&hi)[R],
+                                               uint32_t rows)
+{
+    uint32_t n = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        n += ((rows >> r) & 1u) * (uint32_t)(__popc(lo[r]) + __popc(hi[r]));
+    return n;
 }
 
 // Advance one item after a pass: converged when some iteration of the pass
@@ -220,18 +254,17 @@ __device__ int band_work(const Item2D &it, int band, int K, int par, uint64_t *e
 
     unsigned bits = 0;
     int s = 0, changed_iters = 0;
+    uint32_t count = count_rows(lo, hi, interior);
     const int max_iters = whole ? 0x7fffffff : K;
 #pragma unroll 1
     for (int itn = 0; itn < max_iters; ++itn) {
-        uint32_t rows_changed = 0;
 #define BGT_SUB(D)                                                                           \
     {                                                                                        \
         const int rb_ = s & 1, wb_ = (s + 1) & 1;                                            \
         const uint64_t ue = edges[(rb_ * 2 + 1) * SL + up_idx];                              \
         const uint64_t de = edges[(rb_ * 2 + 0) * SL + dn_idx];                              \
         subiteration<D, R>(lo, hi, W64{(uint32_t)ue & up_ok, (uint32_t)(ue >> 32) & up_ok},  \
-                           W64{(uint32_t)de & dn_ok, (uint32_t)(de >> 32) & dn_ok}, mw, me,  \
-                           rows_changed);                                                    \
+                           W64{(uint32_t)de & dn_ok, (uint32_t)(de >> 32) & dn_ok}, mw, me); \
         edges[(wb_ * 2 + 0) * SL + me_idx] = ((uint64_t)hi[0] << 32) | lo[0];                \
         edges[(wb_ * 2 + 1) * SL + me_idx] = ((uint64_t)hi[R - 1] << 32) | lo[R - 1];        \
         __syncthreads();                                                                     \
@@ -239,7 +272,9 @@ __device__ int band_work(const Item2D &it, int band, int K, int par, uint64_t *e
     }
         BGT_SUB(0) BGT_SUB(1) BGT_SUB(2) BGT_SUB(3)
 #undef BGT_SUB
-        const bool ch = (rows_changed & interior) != 0u;
+        const uint32_t now = count_rows(lo, hi, interior);
+        const bool ch = now != count;
+        count = now;
         if (!whole) {
             if (ch) bits |= 1u << itn;
             continue;

@@ -55,6 +55,19 @@ def test_random_shapes(w, h):
     check(img, f"random {w}x{h}")
 
 
+def test_every_4x4_pattern():
+    """All 2^16 patterns of a 4x4 block, each in a zero frame of its own (6x6
+    cells tiled into one 1536x1536 frame; cells never interact), so every 3x3
+    neighbourhood meets every direction's rule and the result depends on the
+    subiteration order."""
+    codes = np.arange(1 << 16)
+    cells = np.zeros((1 << 16, 6, 6), np.uint8)
+    for k in range(16):
+        cells[:, 1 + k // 4, 1 + k % 4] = (codes >> k) & 1
+    img = cells.reshape(256, 256, 6, 6).transpose(0, 2, 1, 3).reshape(1536, 1536)
+    check(img, "every 4x4 pattern")
+
+
 def test_full_and_empty():
     check(np.ones((77, 150), np.uint8), "all ones")
     check(np.zeros((77, 150), np.uint8), "all zeros")
