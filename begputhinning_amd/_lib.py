@@ -62,7 +62,7 @@ SIGNATURES = {
     "bgt_synth_polylines": (_i, [_vp, _i, _i, ctypes.c_uint64, _i, ctypes.c_double]),
     "bgt_synth_tree2d": (_i, [_vp, _i, _i, ctypes.c_uint64, _i, ctypes.c_double, ctypes.c_double]),
     "bgt_synth_tree3d": (_i, [_vp, _i, _i, _i, ctypes.c_uint64, _i, _i, ctypes.c_double]),
-    "bgt_set_config2d": (_i, [_i, _i, _i]),
+    "bgt_set_config2d": (_i, [_i, _i, _i, _i]),
     "bgt_set_config3d": (_i, [_i, _i]),
     "bgt_batch_create": (_i, [ctypes.POINTER(ctypes.c_void_p)]),
     "bgt_batch_use_current_config": (_i, [_vp]),
@@ -72,7 +72,8 @@ SIGNATURES = {
     "bgt_batch_iterations": (_i, [_vp, _ip]),
     "bgt_batch_device_bytes": (_i, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     "bgt_batch_destroy": (_i, [_vp]),
-    "bgt_batch_autotune": (_i, [_vp, _vp, _i, _ip, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_double)]),
+    "bgt_batch_autotune": (_i, [_vp, _vp, _i, _ip, _ip, _ip, _ip, _ip,
+                                ctypes.POINTER(ctypes.c_double)]),
     "bgt_batch_set_timing": (_i, [_vp, _i]),
     "bgt_batch_kernel_time": (_i, [_vp, ctypes.POINTER(ctypes.c_double), _ip,
                                    ctypes.POINTER(ctypes.c_double), _ip]),

@@ -52,12 +52,12 @@ class ThinningBatch:
         complete result afterwards); returns the chosen shapes and time."""
         import torch
         st = stream if stream is not None else torch.cuda.current_stream()
-        v = [ctypes.c_int(0) for _ in range(4)]
+        v = [ctypes.c_int(0) for _ in range(5)]
         ms = ctypes.c_double(0.0)
         check(lib().bgt_batch_autotune(self._h, ctypes.c_void_p(st.cuda_stream), reps,
                                        *[ctypes.byref(x) for x in v], ctypes.byref(ms)))
         return {"waves": v[0].value, "rows": v[1].value, "iters_per_pass": v[2].value,
-                "blocks_per_cu_3d": v[3].value, "ms": ms.value}
+                "blocks_per_cu_2d": v[3].value, "blocks_per_cu_3d": v[4].value, "ms": ms.value}
 
     def iterations(self):
         out = np.zeros(self.n, np.int32)

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <functional>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "batch.h"
@@ -193,7 +194,11 @@ struct Carver {
     }
 };
 
-int occupancy_grid(const void *kernel, int block, size_t lds, int ntiles)
+// Workgroups of a work-queue launch: as many as fit on the device (at most
+// `per_cu_cap` per CU when > 0), and no more than there are tiles.  Fewer
+// than fit can be faster: the queue kernel is VALU-bound at two workgroups per
+// CU, and the CU slots left free let a concurrent 3D chain run.
+int occupancy_grid(const void *kernel, int block, size_t lds, int ntiles, int per_cu_cap)
 {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return std::max(1, std::min(ntiles, 1024));
@@ -201,6 +206,7 @@ int occupancy_grid(const void *kernel, int block, size_t lds, int ntiles)
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
+    if (per_cu_cap > 0) per_cu = std::min(per_cu, per_cu_cap);
     if (cus < 1) cus = 256;
     return std::max(1, std::min(ntiles, cus * per_cu));
 }
@@ -495,7 +501,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     if (n2)
         b.grid2 = occupancy_grid(b.c2.rows == 8 ? (const void *)thin2d_queue_pass<8>
                                                 : (const void *)thin2d_queue_pass<16>,
-                                 b.c2.waves * 64, lds2, (int)b.ntiles2);
+                                 b.c2.waves * 64, lds2, (int)b.ntiles2, b.c2.blocks_per_cu);
     if (!b.h_live && hipHostMalloc((void **)&b.h_live, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(BGT_ERR_NOMEM, "pinned allocation failed");
     if (!b.ev_fork) BGT_HIP(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
@@ -669,47 +675,98 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         BGT_HIP(hipEventRecord(b.ev_fork, s));
         BGT_HIP(hipStreamWaitEvent(b.side, b.ev_fork, 0));
     }
-    bool live2 = n2 > 0, live3 = n3 > 0;
     long max_vox = 0;
     for (auto &v : b.in2) max_vox = std::max<long>(max_vox, (long)v.w * v.h);
     for (auto &v : b.in3) max_vox = std::max<long>(max_vox, (long)v.w * v.h * v.d);
-    const long max_rounds = max_vox / kChunk + 2;
-    long rounds = 0;
-    // Chunks are pipelined: chunk k+1 is queued before the counters copied
-    // after chunk k are read, so the GPU does not wait for the host.  A chunk
-    // queued after the last item converged finds empty queues and lists.
-    int slot = 0;
-    bool prev2 = false, prev3 = false;  // a poll of the previous chunk is pending
-    while (live2 || live3) {
+    const long max_chunks = max_vox / kChunk + 3;
+    // Each chain (2D passes on `s`, 3D iterations on `s3`) keeps two chunks in
+    // flight: chunk k+1 is queued before the counters copied after chunk k are
+    // read, so the GPU does not wait for the host.  The chains are polled
+    // without blocking and refilled independently, so a slow chunk of one
+    // never holds back the queueing of the other.  A chunk queued after the
+    // last item converged finds empty queues and lists.
+    struct Chain {
+        bool live = false;
+        int head = 0, inflight = 0;  // oldest pending poll slot, pending polls
+        long chunks = 0;
+    } c2, c3;
+    c2.live = n2 > 0;
+    c3.live = n3 > 0;
+    auto enqueue = [&](Chain &c, bool three_d) -> int {
+        const int slot = (c.head + c.inflight) & 1;
         int *hs = b.h_live + slot * 32;
-        if (live2) {
+        if (!three_d) {
             BGT_CHECK(launch_chunk(b, false, use_graph, s));
             BGT_HIP(hipMemcpyAsync(hs, b.d_live2, sizeof(int), hipMemcpyDeviceToHost, s));
             BGT_HIP(hipEventRecord(b.ev_poll2[slot], s));
-        }
-        if (live3) {
+        } else {
             BGT_CHECK(launch_chunk(b, true, use_graph, s3));
             BGT_HIP(hipMemcpyAsync(hs + 8, b.e3.ctl, SC_INTS * sizeof(int), hipMemcpyDeviceToHost,
                                    s3));
             BGT_HIP(hipEventRecord(b.ev_poll3[slot], s3));
         }
-        const int prev = slot ^ 1;
-        const int *hp = b.h_live + prev * 32;
-        const bool launched2 = live2, launched3 = live3;
-        if (live2 && prev2) {
-            BGT_HIP(hipEventSynchronize(b.ev_poll2[prev]));
-            if (hp[0] <= 0) live2 = false;
-        }
-        if (live3 && prev3) {
-            BGT_HIP(hipEventSynchronize(b.ev_poll3[prev]));
+        ++c.inflight;
+        if (++c.chunks > max_chunks) return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
+        return BGT_OK;
+    };
+    // reads the oldest pending poll of a chain (its event has completed)
+    auto consume = [&](Chain &c, bool three_d) {
+        const int *hp = b.h_live + c.head * 32;
+        if (!three_d) {
+            if (hp[0] <= 0) c.live = false;
+        } else {
             // chunks hold an even number of iterations: list 0 is current
             b.bound3 = hp[8 + SC_COUNT0];
-            if (hp[8 + SC_LIVE] <= 0) live3 = false;
+            if (hp[8 + SC_LIVE] <= 0) c.live = false;
         }
-        prev2 = launched2;
-        prev3 = launched3;
-        slot = prev;
-        if (++rounds > max_rounds) return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
+        c.head ^= 1;
+        --c.inflight;
+    };
+    // *done = the chain's oldest pending poll has landed
+    auto ready = [&](const Chain &c, bool three_d, bool block, bool *done) -> int {
+        hipEvent_t e = three_d ? b.ev_poll3[c.head] : b.ev_poll2[c.head];
+        *done = true;
+        if (block) {
+            BGT_HIP(hipEventSynchronize(e));
+            return BGT_OK;
+        }
+        const hipError_t q = hipEventQuery(e);
+        if (q == hipSuccess) return BGT_OK;
+        if (q != hipErrorNotReady) return fail(BGT_ERR_HIP, "poll: %s", hipGetErrorString(q));
+        (void)hipGetLastError();  // "not ready" is not an error of a later call
+        *done = false;
+        return BGT_OK;
+    };
+    while (c2.live || c3.live) {
+        if (c2.live && c2.inflight < 2) {
+            BGT_CHECK(enqueue(c2, false));
+            continue;
+        }
+        if (c3.live && c3.inflight < 2) {
+            BGT_CHECK(enqueue(c3, true));
+            continue;
+        }
+        // both live chains have two chunks queued: take whichever poll lands
+        // first (blocking when only one chain is left)
+        const bool both = c2.live && c3.live;
+        for (;;) {
+            bool done = false;
+            if (c2.live) {
+                BGT_CHECK(ready(c2, false, !both, &done));
+                if (done) {
+                    consume(c2, false);
+                    break;
+                }
+            }
+            if (c3.live) {
+                BGT_CHECK(ready(c3, true, !both, &done));
+                if (done) {
+                    consume(c3, true);
+                    break;
+                }
+            }
+            std::this_thread::yield();
+        }
     }
     if (n3) BGT_CHECK(sparse3d_end(b.e3, s3));
     if (fork) BGT_HIP(hipEventRecord(b.ev_join, s3));
@@ -789,8 +846,9 @@ int batch_set_config(Batch &b, const Config2D &c2, const Config3D &c3)
 }
 
 // Launch-shape autotuner: runs the batch under candidate 2D shapes (waves,
-// register rows, iterations per pass) with the current 3D shape, then under
-// candidate 3D step grids with the best 2D shape, and keeps the fastest.
+// register rows, iterations per pass) with the current 3D shape, then the
+// best of those under candidate workgroups per CU, then candidate 3D step
+// grids with the best 2D shape, and keeps the fastest.
 // Every shape gives the same results, so only time is compared; shapes that
 // do not fit an item (a band thinner than its halo) are skipped.  Each
 // candidate is run once to lay the batch out and then `reps` times, taking
@@ -804,6 +862,7 @@ int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D 
     reps = std::max(1, reps);
     static const int k2[][3] = {{4, 16, 4}, {4, 16, 3}, {4, 16, 2}, {4, 16, 6}, {4, 8, 2},
                                 {4, 8, 3},  {4, 8, 4},  {2, 16, 3}, {2, 16, 4}, {4, 16, 8}};
+    static const int kb2[] = {1, 2, 3, 0};
     static const int k3[] = {0, 2, 4, 8, 16};
     Config2D c2 = b.c2;
     Config3D c3 = b.c3;
@@ -811,7 +870,7 @@ int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D 
     int n = 0;
     auto trial = [&](const Config2D &t2, const Config3D &t3) {
         if (n && t2.waves == c2.waves && t2.rows == c2.rows && t2.iters_per_pass == c2.iters_per_pass &&
-            t3.blocks_per_cu == c3.blocks_per_cu)
+            t2.blocks_per_cu == c2.blocks_per_cu && t3.blocks_per_cu == c3.blocks_per_cu)
             return;  // the current shape, already timed
         if (batch_set_config(b, t2, t3) != BGT_OK) return;
         if (batch_run(b, s, false) != BGT_OK) return;  // lays out; shape may not fit
@@ -834,10 +893,16 @@ int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D 
     trial(b.c2, start3);  // the current shape first
     if (!b.in2.empty()) {
         for (const auto &k : k2) {
-            Config2D t2;
+            Config2D t2 = b.c2;
             t2.waves = k[0];
             t2.rows = k[1];
             t2.iters_per_pass = k[2];
+            trial(t2, start3);
+        }
+        const Config2D shape2 = c2;
+        for (int bpc : kb2) {
+            Config2D t2 = shape2;
+            t2.blocks_per_cu = bpc;
             trial(t2, start3);
         }
     }

@@ -77,6 +77,7 @@ static Config2D &config2d_storage()
         if (const char *e = std::getenv("BGT_2D_WAVES")) v.waves = std::atoi(e);
         if (const char *e = std::getenv("BGT_2D_ROWS")) v.rows = std::atoi(e);
         if (const char *e = std::getenv("BGT_2D_K")) v.iters_per_pass = std::atoi(e);
+        if (const char *e = std::getenv("BGT_2D_BLOCKS")) v.blocks_per_cu = std::atoi(e);
         return v;
     }();
     return c;
@@ -107,12 +108,13 @@ const char *bgt_last_error(void) { return g_error.c_str(); }
 
 int bgt_version(void) { return 1; }
 
-int bgt_set_config2d(int waves, int rows, int iters_per_pass)
+int bgt_set_config2d(int waves, int rows, int iters_per_pass, int blocks_per_cu)
 {
     Config2D c;
     c.waves = waves;
     c.rows = rows;
     c.iters_per_pass = iters_per_pass;
+    c.blocks_per_cu = blocks_per_cu;
     BGT_CHECK(check_config2d(c));
     config2d_storage() = c;
     return BGT_OK;
@@ -239,7 +241,8 @@ int bgt_batch_kernel_time(bgt_batch b, double *ms2d, int *launches2d, double *ms
 }
 
 int bgt_batch_autotune(bgt_batch b, void *stream, int reps, int *waves, int *rows,
-                       int *iters_per_pass, int *blocks_per_cu, double *best_ms)
+                       int *iters_per_pass, int *blocks_per_cu_2d, int *blocks_per_cu_3d,
+                       double *best_ms)
 {
     if (!b) return fail(BGT_ERR_INVALID, "null batch");
     Config2D c2;
@@ -249,7 +252,8 @@ int bgt_batch_autotune(bgt_batch b, void *stream, int reps, int *waves, int *row
     if (waves) *waves = c2.waves;
     if (rows) *rows = c2.rows;
     if (iters_per_pass) *iters_per_pass = c2.iters_per_pass;
-    if (blocks_per_cu) *blocks_per_cu = c3.blocks_per_cu;
+    if (blocks_per_cu_2d) *blocks_per_cu_2d = c2.blocks_per_cu;
+    if (blocks_per_cu_3d) *blocks_per_cu_3d = c3.blocks_per_cu;
     if (best_ms) *best_ms = ms;
     return BGT_OK;
 }

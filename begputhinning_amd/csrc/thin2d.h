@@ -16,6 +16,7 @@ struct Config2D {
     int waves = 4;           // wave64s per workgroup
     int rows = 16;           // register rows per lane (8, 16 or 32)
     int iters_per_pass = 3;  // K: iterations fused per HBM pass (halo 4K rows)
+    int blocks_per_cu = 2;   // batch queue workgroups per CU (0 = as many as fit)
 };
 
 struct Plan2D {

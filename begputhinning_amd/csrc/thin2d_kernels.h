@@ -215,8 +215,11 @@ __device__ int band_work(const Item2D &it, int band, int K, int par, uint64_t *e
     const int H = 4 * K;
     const bool whole = it.nbands == 1;  // halo rows all outside the image
     const int y_reg0 = band * it.band_rows - H + q * R;  // image row of register row 0
-    const uint32_t mw = c == 0 ? 0u : 0xffffffffu;
-    const uint32_t me = c == L - 1 ? 0u : 0xffffffffu;
+    uint32_t mw = c == 0 ? 0u : 0xffffffffu;
+    uint32_t me = c == L - 1 ? 0u : 0xffffffffu;
+    // opaque to the compiler, so `dpp(v) & mask` stays a VGPR AND that folds
+    // into the DPP instruction instead of becoming a lane-mask select
+    asm volatile("" : "+v"(mw), "+v"(me));
     // rows of this segment inside the band interior, as a bit mask
     const int ra = max(0, H - q * R), rb = min(R, H + it.band_rows - q * R);
     const uint32_t interior =

@@ -44,11 +44,12 @@ int bgt_version(void);
 /* Launch shapes (no reference counterpart: the reference hard-codes them,
  * 2D/LibThinning2D.cu:59-60, 3D/LibThinning3D.cu:50).  2D: waves per
  * workgroup (1-4), register rows per lane (8 or 16), iterations fused per
- * HBM pass (1-16).  3D: workgroups per CU of the subiteration kernel (0-32,
+ * HBM pass (1-16), workgroups per CU of the batch pass kernel (0-32, 0 = the
+ * occupancy limit).  3D: workgroups per CU of the subiteration kernel (0-32,
  * 0 = the occupancy limit) and the most iterations queued between two host
  * convergence polls (1-64).  Applies to later calls of the calling
  * process and to batches created afterwards. */
-int bgt_set_config2d(int waves, int rows, int iters_per_pass);
+int bgt_set_config2d(int waves, int rows, int iters_per_pass, int blocks_per_cu);
 int bgt_set_config3d(int blocks_per_cu, int max_chunk);
 
 /* ------------------------------------------------------------------------- */
@@ -150,13 +151,14 @@ int bgt_batch_kernel_time(bgt_batch b, double *ms2d, int *launches2d, double *ms
                           int *launches3d);
 /* Launch-shape autotuner (no reference counterpart; the reference hard-codes
  * its shapes, 2D/LibThinning2D.cu:59-60, 3D/LibThinning3D.cu:50): runs the
- * batch under candidate 2D shapes (waves, rows, iterations per pass) and 3D
- * step grids, `reps` timed runs each after one layout run, keeps the fastest
- * for the batch and reports it (any output pointer may be NULL).  Results do
- * not depend on the shape.  The batch's outputs hold a complete result
- * afterwards. */
+ * batch under candidate 2D shapes (waves, rows, iterations per pass,
+ * workgroups per CU) and 3D step grids, `reps` timed runs each after one
+ * layout run, keeps the fastest for the batch and reports it (any output
+ * pointer may be NULL).  Results do not depend on the shape.  The batch's
+ * outputs hold a complete result afterwards. */
 int bgt_batch_autotune(bgt_batch b, void *stream, int reps, int *waves, int *rows,
-                       int *iters_per_pass, int *blocks_per_cu, double *best_ms);
+                       int *iters_per_pass, int *blocks_per_cu_2d, int *blocks_per_cu_3d,
+                       double *best_ms);
 int bgt_batch_destroy(bgt_batch b);
 
 /* ------------------------------------------------------------------------- */

@@ -74,6 +74,32 @@ def test_batch_timing_and_single_item_equality():
     b.close()
 
 
+@pytest.mark.parametrize("blocks_per_cu", [1, 3, 0])
+def test_pass_grid_sizes(blocks_per_cu):
+    """The batch pass kernel pulls bands from a queue: any number of
+    workgroups per CU gives the same skeletons and iteration counts."""
+    import begputhinning_amd as B
+    from begputhinning_amd.batch import ThinningBatch
+    items = make_items(4, n2=16, n3=0)
+    ins = [torch.from_numpy(a).cuda() for a in items]
+    outs = [torch.full_like(t, 9) for t in ins]
+    B.set_config2d(4, 16, 3, blocks_per_cu)
+    try:
+        b = ThinningBatch()
+        for t, o in zip(ins, outs):
+            (b.add2d if t.dim() == 2 else b.add3d)(t, o)
+        b.run(use_graph=False)
+        torch.cuda.synchronize()
+        its = b.iterations()
+        b.close()
+    finally:
+        B.set_config2d()
+    for k, (a, o) in enumerate(zip(items, outs)):
+        ref, rit = O.thin2d(a) if a.ndim == 2 else O.thin3d(a)
+        assert np.array_equal(o.cpu().numpy(), ref), (blocks_per_cu, k)
+        assert its[k] == rit, (blocks_per_cu, k)
+
+
 def test_autotune_keeps_results():
     """The autotuner tries launch shapes on the batch itself; every shape gives
     the same skeletons, and the batch keeps the fastest one."""
@@ -88,6 +114,7 @@ def test_autotune_keeps_results():
     print(f"[batch] autotune -> {best}", flush=True)
     assert best["waves"] in (1, 2, 3, 4) and best["rows"] in (8, 16)
     assert 1 <= best["iters_per_pass"] <= 16 and best["ms"] > 0
+    assert 0 <= best["blocks_per_cu_2d"] <= 32 and 0 <= best["blocks_per_cu_3d"] <= 32
     b.run(use_graph=False)
     torch.cuda.synchronize()
     its = b.iterations()
