@@ -131,7 +131,7 @@ __device__ __forceinline__ void enqueue(const Queue &q, int u, int next_pass, in
 }
 
 template <int R>
-__global__ __launch_bounds__(256) void thin2d_queue_pass(const Item2D *__restrict__ items,
+__global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restrict__ items,
                                                           const Tile2D *__restrict__ tiles,
                                                           Queue q, int K, const int *gpass)
 {
@@ -861,7 +861,8 @@ int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D 
     if (b.in2.empty() && b.in3.empty()) return fail(BGT_ERR_INVALID, "empty batch");
     reps = std::max(1, reps);
     static const int k2[][3] = {{4, 16, 4}, {4, 16, 3}, {4, 16, 2}, {4, 16, 6}, {4, 8, 2},
-                                {4, 8, 3},  {4, 8, 4},  {2, 16, 3}, {2, 16, 4}, {4, 16, 8}};
+                                {4, 8, 3},  {4, 8, 4},  {2, 16, 3}, {2, 16, 4}, {4, 16, 8},
+                                {8, 8, 3},  {8, 8, 4},  {8, 8, 6},  {8, 16, 3}, {8, 16, 6}};
     static const int kb2[] = {1, 2, 3, 0};
     static const int k3[] = {0, 2, 4, 8, 16};
     Config2D c2 = b.c2;

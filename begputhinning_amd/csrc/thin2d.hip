@@ -135,7 +135,7 @@ __global__ void init_item2d(Item2D *item, Item2D val, unsigned *ctr)
 // Per-image pass kernel: one workgroup per band, the last workgroup of the
 // launch advances the item.
 template <int R>
-__global__ __launch_bounds__(256) void thin2d_pass(const Item2D *__restrict__ items, int K,
+__global__ __launch_bounds__(512) void thin2d_pass(const Item2D *__restrict__ items, int K,
                                                     unsigned *__restrict__ ctr)
 {
     extern __shared__ uint64_t edges[];
@@ -193,7 +193,7 @@ int band_rows_for(int L, const Config2D &cfg)
 
 int check_config2d(const Config2D &cfg)
 {
-    if (cfg.waves < 1 || cfg.waves > 4 || (cfg.rows != 8 && cfg.rows != 16) ||
+    if (cfg.waves < 1 || cfg.waves > 8 || (cfg.rows != 8 && cfg.rows != 16) ||
         cfg.iters_per_pass < 1 || cfg.iters_per_pass > 16 || cfg.blocks_per_cu < 0 ||
         cfg.blocks_per_cu > 32)
         return fail(BGT_ERR_INVALID, "bad 2D config (waves=%d rows=%d k=%d blocks/CU=%d)", cfg.waves,

@@ -43,7 +43,7 @@ int bgt_version(void);
 
 /* Launch shapes (no reference counterpart: the reference hard-codes them,
  * 2D/LibThinning2D.cu:59-60, 3D/LibThinning3D.cu:50).  2D: waves per
- * workgroup (1-4), register rows per lane (8 or 16), iterations fused per
+ * workgroup (1-8), register rows per lane (8 or 16), iterations fused per
  * HBM pass (1-16), workgroups per CU of the batch pass kernel (0-32, 0 = the
  * occupancy limit).  3D: workgroups per CU of the subiteration kernel (0-32,
  * 0 = the occupancy limit) and the most iterations queued between two host

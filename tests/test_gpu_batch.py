@@ -112,7 +112,7 @@ def test_autotune_keeps_results():
         (b.add2d if t.dim() == 2 else b.add3d)(t, o)
     best = b.autotune(reps=1)
     print(f"[batch] autotune -> {best}", flush=True)
-    assert best["waves"] in (1, 2, 3, 4) and best["rows"] in (8, 16)
+    assert 1 <= best["waves"] <= 8 and best["rows"] in (8, 16)
     assert 1 <= best["iters_per_pass"] <= 16 and best["ms"] > 0
     assert 0 <= best["blocks_per_cu_2d"] <= 32 and 0 <= best["blocks_per_cu_3d"] <= 32
     b.run(use_graph=False)

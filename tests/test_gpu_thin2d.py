@@ -73,8 +73,8 @@ def test_full_and_empty():
     check(np.zeros((77, 150), np.uint8), "all zeros")
 
 
-@pytest.mark.parametrize("cfg", [(1, 8, 1), (2, 8, 3), (4, 8, 4), (1, 16, 1), (3, 16, 2), (4, 16, 2), (4, 16, 4),
-                                 (4, 16, 5)])
+@pytest.mark.parametrize("cfg", [(1, 8, 1), (2, 8, 3), (4, 8, 4), (1, 16, 1), (3, 16, 2), (4, 16, 2),
+                                 (4, 16, 4), (4, 16, 5), (8, 8, 3), (8, 16, 6), (6, 8, 2)])
 def test_launch_shapes(cfg):
     import begputhinning_amd as B
     from begputhinning_amd import synth
