@@ -256,8 +256,9 @@ struct Batch {
     Sparse3D e3;
     bool zeroed3 = false;  // state margins cleared since allocation
     int bound3 = 0;        // upper bound of the current 3D list's count
+    int abound3 = 0;       // upper bound of the current 3D active list's count
     int cur3 = 0;          // current 3D list
-    std::vector<std::pair<int, hipGraphExec_t>> graphs3;  // by launch bound
+    std::vector<std::pair<long, hipGraphExec_t>> graphs3;  // by launch bounds
     hipStream_t graph3_stream = nullptr;
 
     // control words cleared by one init kernel
@@ -415,10 +416,19 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
         for (int k = 0; k < 2; ++k) {
             e.list[k].item = cv.take<int>(sizeof(int) * data3);
             e.list[k].word = cv.take<uint32_t>(sizeof(uint32_t) * data3);
+            e.list[k].blk = cv.take<int>(sizeof(int) * data3);
             e.list[k].count = e.ctl ? e.ctl + SC_COUNT0 + k : nullptr;
         }
+        e.act.item = cv.take<int>(sizeof(int) * data3);
+        e.act.word = cv.take<uint32_t>(sizeof(uint32_t) * data3);
+        e.act.blk = cv.take<int>(sizeof(int) * data3);
+        e.act.count = nullptr;  // in ctl, by iteration parity
         e.prot = cv.take<uint32_t>(sizeof(uint32_t) * data3);
+        e.first = cv.take<uint32_t>(sizeof(uint32_t) * data3);
         e.capacity = (int)data3;
+        e.nblocks = (int)lay3.nblocks;
+        e.bstamp = cv.take<int>(sizeof(int) * lay3.nblocks);
+        e.dstamp = cv.take<int>(sizeof(int) * lay3.nblocks);
         e.states = cv.take<int>(sizeof(int) * (size_t)n3 * ST_INTS);
         e.changed = cv.take<int>(sizeof(int) * n3);
         // 2D control words: [0,1] queue counts, [2] next slot, [3] pass, [4] live
@@ -549,11 +559,12 @@ static int launch_pass2d(Batch &b, hipStream_t s)
     return BGT_OK;
 }
 
-// one 3D iteration from list `cur` into list cur^1; `bound` caps the list count
-static int launch_iter3d(Batch &b, int cur, int bound, hipStream_t s)
+// one 3D iteration from list `cur` into list cur^1; `bound` and `abound` cap
+// the list and active list counts
+static int launch_iter3d(Batch &b, int cur, int bound, int abound, hipStream_t s)
 {
     BGT_CHECK(timed_mark(b, s, 1, true));
-    BGT_CHECK(sparse3d_iteration(b.e3, cur, bound, sparse3d_grid(b.c3.blocks_per_cu), s));
+    BGT_CHECK(sparse3d_iteration(b.e3, cur, bound, abound, sparse3d_grid(b.c3.blocks_per_cu), s));
     BGT_CHECK(timed_mark(b, s, 1, false));
     return BGT_OK;
 }
@@ -589,10 +600,11 @@ static int capture(hipGraphExec_t &graph, const std::function<int(hipStream_t)> 
 
 static int launch_chunk(Batch &b, bool three_d, bool use_graph, hipStream_t s)
 {
-    const int bound3 = bound_blocks3(b.bound3) * 256;
+    const int bound3 = bound_blocks3(b.bound3) * 256, abound3 = bound_blocks3(b.abound3) * 256;
+    const long key3 = ((long)bound3 << 32) | (long)abound3;
     auto body = [&](hipStream_t st) -> int {
         for (int i = 0; i < kChunk; ++i)
-            BGT_CHECK(three_d ? launch_iter3d(b, i & 1, bound3, st) : launch_pass2d(b, st));
+            BGT_CHECK(three_d ? launch_iter3d(b, i & 1, bound3, abound3, st) : launch_pass2d(b, st));
         return BGT_OK;
     };
     if (!use_graph) return body(s);
@@ -612,10 +624,10 @@ static int launch_chunk(Batch &b, bool three_d, bool use_graph, hipStream_t s)
     }
     hipGraphExec_t graph = nullptr;
     for (auto &g : b.graphs3)
-        if (g.first == bound3) graph = g.second;
+        if (g.first == key3) graph = g.second;
     if (!graph) {
         BGT_CHECK(capture(graph, body));
-        b.graphs3.push_back({bound3, graph});
+        b.graphs3.push_back({key3, graph});
     }
     BGT_HIP(hipGraphLaunch(graph, s));
     return BGT_OK;
@@ -661,7 +673,7 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     if (n3) {
         BGT_CHECK(sparse3d_begin(b.e3, !b.zeroed3, s));
         b.zeroed3 = true;
-        b.bound3 = b.e3.capacity;
+        b.bound3 = b.abound3 = b.e3.capacity;
     }
 
     // With both kinds of items, the 3D pipeline runs on a side stream forked
@@ -717,6 +729,7 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         } else {
             // chunks hold an even number of iterations: list 0 is current
             b.bound3 = hp[8 + SC_COUNT0];
+            b.abound3 = hp[8 + SC_ACOUNT0 + (hp[8 + SC_ITER] & 1)];
             if (hp[8 + SC_LIVE] <= 0) c.live = false;
         }
         c.head ^= 1;

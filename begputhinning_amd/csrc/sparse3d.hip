@@ -17,7 +17,10 @@
 // An iteration is six launches, U (with the protection pre-test of all six
 // directions, 3D/CUDA_ThinningKernel3D_Linear.cuh:690-702), N, E, D, S, W,
 // alternating arrays 0 -> 1 -> 0 ..., so it ends in array 0, and a compaction
-// launch that builds the next list and records per-item convergence.
+// launch that builds the next list and records per-item convergence.  The
+// subiterations run over the active list only (the words near the previous
+// iteration's changes, see sparse3d.h); the other words of the list cannot
+// change and hold the same value in both arrays.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,10 +35,13 @@ namespace bgt {
 
 namespace {
 
+const int kCompactPer = 1024;  // list entries per compaction workgroup (4 per thread)
+
 // Appends the words of a workgroup whose flags are set to `out`, keeping
 // lane order, with one atomic per workgroup.  Every thread of the block must
-// call it (flags may be 0).  Each thread offers up to two words (w0, w0 + 1).
-__device__ __forceinline__ void append2(bool f0, bool f1, int item, uint32_t w0, SList out)
+// call it (flags may be 0); two calls in a row need a barrier between them.
+// Each thread offers up to two words (w0, w0 + 1) of blocks (b0, b0 + 1).
+__device__ __forceinline__ void append2(bool f0, bool f1, int item, uint32_t w0, int b0, SList out)
 {
     __shared__ int s_cnt[4];
     __shared__ int s_base;
@@ -56,20 +62,89 @@ __device__ __forceinline__ void append2(bool f0, bool f1, int item, uint32_t w0,
     if (f0) {
         out.item[pos] = item;
         out.word[pos] = w0;
+        out.blk[pos] = b0;
         ++pos;
     }
     if (f1) {
         out.item[pos] = item;
         out.word[pos] = w0 + 1;
+        out.blk[pos] = b0 + 1;
     }
 }
 
-__global__ void sparse3d_init(int *states, int *changed, int *ctl, int nitems)
+// Appends the entries (item, word, block) of a workgroup flagged for list
+// `A` (fa) and for list `B` (fb) -- up to 4 consecutive entries per thread --
+// keeping their order, with one atomic per list per workgroup.  Every thread
+// of the block must call it.
+__device__ __forceinline__ void append4_both(const bool (&fa)[4], const bool (&fb)[4],
+                                             const int (&item)[4], const uint32_t (&word)[4],
+                                             const int (&blk)[4], SList A, SList B)
+{
+    __shared__ int s_cnt[2][16];
+    __shared__ int s_base[2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    int ca = 0, cb = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        ca += fa[j];
+        cb += fb[j];
+    }
+    // exclusive prefix over the wave of the per-thread counts (0..4), bit by bit
+    int pa = 0, pb = 0, ta = 0, tb = 0;
+#pragma unroll
+    for (int bit = 0; bit < 3; ++bit) {
+        const uint64_t ma = __ballot((ca >> bit) & 1), mb = __ballot((cb >> bit) & 1);
+        pa += __popcll(ma & lt) << bit;
+        ta += __popcll(ma) << bit;
+        pb += __popcll(mb & lt) << bit;
+        tb += __popcll(mb) << bit;
+    }
+    if (lane == 0) {
+        s_cnt[0][wv] = ta;
+        s_cnt[1][wv] = tb;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        int sum = 0;
+        for (int k = 0; k < nw; ++k) sum += s_cnt[threadIdx.x][k];
+        s_base[threadIdx.x] = sum ? atomicAdd(threadIdx.x ? B.count : A.count, sum) : 0;
+    }
+    __syncthreads();
+    pa += s_base[0];
+    pb += s_base[1];
+    for (int k = 0; k < wv; ++k) {
+        pa += s_cnt[0][k];
+        pb += s_cnt[1][k];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (fa[j]) {
+            A.item[pa] = item[j];
+            A.word[pa] = word[j];
+            A.blk[pa] = blk[j];
+            ++pa;
+        }
+        if (fb[j]) {
+            B.item[pb] = item[j];
+            B.word[pb] = word[j];
+            B.blk[pb] = blk[j];
+            ++pb;
+        }
+    }
+}
+
+__global__ void sparse3d_init(int *states, int *changed, int *ctl, int nitems, int *bstamp,
+                              int *dstamp, int nblocks)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nitems * ST_INTS) states[t] = 0;
     if (t < nitems) changed[t] = 0;
     if (t < SC_INTS) ctl[t] = t == SC_LIVE ? nitems : 0;
+    if (t < nblocks) {
+        bstamp[t] = 0;
+        dstamp[t] = 0;
+    }
 }
 
 __global__ void sparse3d_zero(uint32_t *a, uint32_t *b, long n)
@@ -82,7 +157,8 @@ __global__ void sparse3d_zero(uint32_t *a, uint32_t *b, long n)
 }
 
 // One thread per 64 voxels (two data words) of an x-line: writes both state
-// arrays and lists the non-empty words (list 0).
+// arrays and lists the non-empty words (list 0, also the first iteration's
+// active list).
 __global__ __launch_bounds__(256) void sparse3d_pack(const SItem3D *__restrict__ items,
                                                      const SIO *__restrict__ io,
                                                      const int *__restrict__ blk, uint32_t *s0,
@@ -94,6 +170,7 @@ __global__ __launch_bounds__(256) void sparse3d_pack(const SItem3D *__restrict__
     const uint32_t nq = (uint32_t)it.nh >> 1;
     const uint32_t t = (uint32_t)(blockIdx.x - f.blk0) * 256u + threadIdx.x;
     uint32_t lo = 0, hi = 0, g = 0;
+    int b0 = 0;
     if (t < (uint32_t)it.h * (uint32_t)it.d * nq) {
         const uint32_t line = t / nq;
         const int x0 = (int)(t - line * nq) * 64;
@@ -104,12 +181,13 @@ __global__ __launch_bounds__(256) void sparse3d_pack(const SItem3D *__restrict__
         const uint32_t y = line % (uint32_t)it.h, z = line / (uint32_t)it.h;
         g = (uint32_t)it.base + ((z + 1) * (uint32_t)(it.h + 2) + y + 1) * (uint32_t)it.P + 1 +
             (uint32_t)x0 / 32;
+        b0 = it.bbase + ((int)(z >> 3) * it.nby + (int)(y >> 3)) * it.nh + x0 / 32;
         s0[g] = lo;
         s0[g + 1] = hi;
         s1[g] = lo;
         s1[g + 1] = hi;
     }
-    append2(lo != 0, hi != 0, i, g, out);
+    append2(lo != 0, hi != 0, i, g, b0, out);
 }
 
 // Pixels of one z-plane of an item in state array 0.
@@ -156,19 +234,25 @@ __global__ __launch_bounds__(256) void sparse3d_unpack(const SItem3D *__restrict
     unpack_block64_u8(f.out + (size_t)z * plane, plane, (uint32_t)it.w, c, src);
 }
 
-// Subiteration D (0..5 = U, N, E, D, S, W) over the list: src -> dst.
+// Subiteration D (0..5 = U, N, E, D, S, W) over the active list (list 0 in
+// the first iteration): src -> dst.  The last one stamps the blocks of the
+// words the iteration changed.
 template <int D>
 __global__ __launch_bounds__(256) void sparse3d_step(const uint32_t *__restrict__ src,
                                                      uint32_t *__restrict__ dst,
-                                                     const int2 *__restrict__ geo, const SList list,
-                                                     uint32_t *__restrict__ prot,
-                                                     int *__restrict__ changed)
+                                                     const int2 *__restrict__ geo, SList act,
+                                                     SList list0, uint32_t *__restrict__ prot,
+                                                     uint32_t *__restrict__ first,
+                                                     int *__restrict__ changed,
+                                                     const int *__restrict__ ctl, int *dstamp)
 {
     // The list is cut into one contiguous range per XCD (workgroups are dealt
     // to the 8 XCDs round-robin; the grid is a multiple of 8 or smaller than
     // 8), so that neighbouring entries, which share most of their 3x3x3
     // neighbourhoods, are gathered through the same L2.
-    const int n = *list.count;
+    const int itn = ctl[SC_ITER];
+    const SList list = itn == 0 ? list0 : act;
+    const int n = itn == 0 ? ctl[SC_COUNT0] : ctl[SC_ACOUNT0 + (itn & 1)];
     const int nx = gridDim.x >= 8 ? 8 : 1;
     const int xcd = blockIdx.x % nx, local = blockIdx.x / nx, per = gridDim.x / nx;
     const int lo = (int)((long)n * xcd / nx), hi = (int)((long)n * (xcd + 1) / nx);
@@ -195,17 +279,52 @@ __global__ __launch_bounds__(256) void sparse3d_step(const uint32_t *__restrict_
             const uint32_t any = cu | rule3<1>(nb) | rule3<2>(nb) | rule3<3>(nb) | rule3<4>(nb) |
                                  rule3<5>(nb);
             prot[k] = v & ~any;
+            first[k] = v;
             del = cu & v;
         } else {
             del = rule3<D>(nb) & v & ~prot[k];
         }
-        dst[g] = v ^ del;
+        const uint32_t nv = v ^ del;
+        dst[g] = nv;
         if (del) changed[ii] = 1;
+        if (D == 5 && nv != first[k]) dstamp[list.blk[k]] = itn + 1;
     }
 }
 
+// Block b is active in the next iteration when one of its 27 neighbour blocks
+// (itself included) was stamped dirty in this one.
+__global__ __launch_bounds__(256) void sparse3d_expand(const SItem3D *__restrict__ items,
+                                                       int nitems, const int *__restrict__ dstamp,
+                                                       int *__restrict__ bstamp, int nblocks,
+                                                       const int *__restrict__ ctl)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    const int stamp = ctl[SC_ITER] + 1;
+    int lo = 0, hi = nitems - 1;  // the item holding block b: the last with bbase <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].bbase <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const int nh = items[lo].nh, nby = items[lo].nby, nbz = items[lo].nbz;
+    const int l = b - items[lo].bbase, bx = l % nh, byz = l / nh, by = byz % nby, bz = byz / nby;
+    bool act = false;
+    for (int dz = -1; dz <= 1; ++dz) {
+        if (bz + dz < 0 || bz + dz >= nbz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            if (by + dy < 0 || by + dy >= nby) continue;
+            const int row = b + (dz * nby + dy) * nh;
+            for (int dx = -1; dx <= 1; ++dx)
+                if (bx + dx >= 0 && bx + dx < nh) act |= dstamp[row + dx] == stamp;
+        }
+    }
+    if (act) bstamp[b] = stamp;
+}
+
 // Keep the entries whose word is non-empty and whose item changed in this
-// iteration (an item whose iteration deleted nothing has converged).  A word
+// iteration (an item whose iteration deleted nothing has converged); those in
+// a block activated in this iteration also form the next active list.  A word
 // emptied by the last subiteration (W) still holds its bits from before that
 // step in array 1; once off the list nothing would rewrite it, and the next
 // iteration's odd steps read array 1, so it is cleared here.
@@ -218,34 +337,62 @@ __global__ __launch_bounds__(256) void sparse3d_step(const uint32_t *__restrict_
 __global__ __launch_bounds__(256) void sparse3d_compact(const uint32_t *__restrict__ s0,
                                                         uint32_t *__restrict__ s1,
                                                         int *__restrict__ changed, const SList in,
-                                                        SList out, int *__restrict__ states,
+                                                        SList out, SList act,
+                                                        const int *__restrict__ bstamp,
+                                                        int *__restrict__ states,
                                                         int *__restrict__ ctl, int nitems)
 {
     __shared__ int s_last;
     const int n = *in.count;
-    // Workgroups past the list take no part (the grid is sized by an upper
-    // bound of n); the others count themselves in to elect the last one.
-    const int active = n > 0 ? (n + 255) / 256 : 1;
+    const int itn = ctl[SC_ITER];
+    act.count = ctl + SC_ACOUNT0 + ((itn + 1) & 1);
+    // Each thread takes 4 consecutive entries.  Workgroups past the list take
+    // no part (the grid is sized by an upper bound of n); the others count
+    // themselves in to elect the last one.
+    const int active = n > 0 ? (n + kCompactPer - 1) / kCompactPer : 1;
     if ((int)blockIdx.x >= active) return;
     {
-        const int k = blockIdx.x * blockDim.x + threadIdx.x;
-        bool keep = false;
-        int ii = 0;
-        uint32_t g = 0;
-        if (k < n) {
-            ii = in.item[k];
-            g = in.word[k];
-            const bool live = changed[ii] != 0;
-            keep = live && s0[g] != 0u;
-            if (live && !keep) s1[g] = 0u;
+        const int k0 = ((int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x) * 4;
+        int ii[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+        uint32_t g[4] = {0u, 0u, 0u, 0u};
+        bool keep[4], ak[4];
+        if (k0 + 4 <= n) {  // lists are 256-byte aligned: k0 is 16-byte aligned
+            const int4 vi = *reinterpret_cast<const int4 *>(in.item + k0);
+            const uint4 vw = *reinterpret_cast<const uint4 *>(in.word + k0);
+            const int4 vb = *reinterpret_cast<const int4 *>(in.blk + k0);
+            ii[0] = vi.x; ii[1] = vi.y; ii[2] = vi.z; ii[3] = vi.w;
+            g[0] = vw.x; g[1] = vw.y; g[2] = vw.z; g[3] = vw.w;
+            b[0] = vb.x; b[1] = vb.y; b[2] = vb.z; b[3] = vb.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (k0 + j < n) {
+                    ii[j] = in.item[k0 + j];
+                    g[j] = in.word[k0 + j];
+                    b[j] = in.blk[k0 + j];
+                }
         }
-        append2(keep, false, ii, g, out);
+        int live[4], bs[4];
+        uint32_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool ok = k0 + j < n;
+            live[j] = ok ? changed[ii[j]] : 0;
+            v[j] = ok ? s0[g[j]] : 0u;
+            bs[j] = ok ? bstamp[b[j]] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            keep[j] = live[j] != 0 && v[j] != 0u;
+            if (live[j] != 0 && v[j] == 0u) s1[g[j]] = 0u;
+            ak[j] = keep[j] && bs[j] == itn + 1;
+        }
+        append4_both(keep, ak, ii, g, b, out, act);
     }
     __syncthreads();
     if (threadIdx.x == 0) s_last = atomicAdd(ctl + SC_BLOCKS, 1) == active - 1;
     __syncthreads();
     if (!s_last) return;
-    const int itn = ctl[SC_ITER];
     for (int i = threadIdx.x; i < nitems; i += blockDim.x) {
         int *st = states + (size_t)i * ST_INTS;
         if (!st[ST_DONE] && !changed[i]) {
@@ -260,6 +407,7 @@ __global__ __launch_bounds__(256) void sparse3d_compact(const uint32_t *__restri
         ctl[SC_ITER] = itn + 1;
         ctl[SC_BLOCKS] = 0;
         *in.count = 0;  // the next iteration compacts into this list
+        ctl[SC_ACOUNT0 + (itn & 1)] = 0;  // and builds its active list here
     }
 }
 
@@ -272,7 +420,7 @@ int sparse3d_layout(const int *w, const int *h, const int *d, int n, Layout3D &l
     lay.geo.assign(n, int2{0, 0});
     lay.blk0.assign(n, 0);
     lay.ublk0.assign(n, 0);
-    long base = 0, data = 0;
+    long base = 0, data = 0, blocks = 0;
     for (int i = 0; i < n; ++i) {
         if (w[i] <= 0 || h[i] <= 0 || d[i] <= 0)
             return fail(BGT_ERR_INVALID, "3D size must be >= 1 (got %d x %d x %d)", w[i], h[i], d[i]);
@@ -292,6 +440,10 @@ int sparse3d_layout(const int *w, const int *h, const int *d, int n, Layout3D &l
         it.plane = (int)plane;
         it.base = base;
         lay.geo[i] = int2{it.P, it.plane};
+        it.nby = (h[i] + 7) / 8;
+        it.nbz = (d[i] + 7) / 8;
+        it.bbase = (int)std::min<long>(blocks, 0x7fffffffL);
+        blocks += (long)it.nh * it.nby * it.nbz;
         base += total;
         data += (long)h[i] * d[i] * it.nh;
         const long threads = (long)h[i] * d[i] * (it.nh / 2);
@@ -302,10 +454,12 @@ int sparse3d_layout(const int *w, const int *h, const int *d, int n, Layout3D &l
         for (long k = 0; k < (uthreads + 255) / 256; ++k) lay.ublk.push_back(i);
     }
     if (data >= 0x7fffffffL) return fail(BGT_ERR_UNSUPPORTED, "3D data exceeds 2^31 words");
+    if (blocks >= 0x7fffffffL) return fail(BGT_ERR_UNSUPPORTED, "3D blocks exceed 2^31");
     if (lay.blk.size() >= 0x7fffffffUL || lay.ublk.size() >= 0x7fffffffUL)
         return fail(BGT_ERR_UNSUPPORTED, "3D grid too large");
     lay.nwords = base;
     lay.data_words = data;
+    lay.nblocks = blocks;
     return BGT_OK;
 }
 
@@ -328,9 +482,9 @@ int sparse3d_grid(int blocks_per_cu)
 
 int sparse3d_begin(const Sparse3D &e, bool zero, hipStream_t s)
 {
-    const int ninit = std::max(e.nitems * ST_INTS, (int)SC_INTS);
+    const int ninit = std::max({e.nitems * ST_INTS, (int)SC_INTS, e.nblocks});
     hipLaunchKernelGGL(sparse3d_init, dim3((ninit + 255) / 256), dim3(256), 0, s, e.states,
-                       e.changed, e.ctl, e.nitems);
+                       e.changed, e.ctl, e.nitems, e.bstamp, e.dstamp, e.nblocks);
     if (zero && e.nwords > 0)
         hipLaunchKernelGGL(sparse3d_zero, dim3((unsigned)((e.nwords + 255) / 256)), dim3(256), 0, s,
                            e.state[0], e.state[1], e.nwords);
@@ -341,22 +495,28 @@ int sparse3d_begin(const Sparse3D &e, bool zero, hipStream_t s)
     return BGT_OK;
 }
 
-int sparse3d_iteration(const Sparse3D &e, int cur, int bound, int grid, hipStream_t s)
+int sparse3d_iteration(const Sparse3D &e, int cur, int bound, int abound, int grid, hipStream_t s)
 {
-    const SList L = e.list[cur], N = e.list[cur ^ 1];
-    const int need = std::max(1, (bound + 255) / 256);
-    int steps = std::min(grid, need);
+    const SList L = e.list[cur], N = e.list[cur ^ 1], A = e.act;
+    const int need = std::max(1, (bound + kCompactPer - 1) / kCompactPer);
+    int steps = std::min(grid, std::max(1, (abound + 255) / 256));
     if (steps >= 8) steps &= ~7;  // whole XCD ranges
     const dim3 g(steps), b(256);
     uint32_t *s0 = e.state[0], *s1 = e.state[1];
-    hipLaunchKernelGGL(sparse3d_step<0>, g, b, 0, s, s0, s1, e.geo, L, e.prot, e.changed);
-    hipLaunchKernelGGL(sparse3d_step<1>, g, b, 0, s, s1, s0, e.geo, L, e.prot, e.changed);
-    hipLaunchKernelGGL(sparse3d_step<2>, g, b, 0, s, s0, s1, e.geo, L, e.prot, e.changed);
-    hipLaunchKernelGGL(sparse3d_step<3>, g, b, 0, s, s1, s0, e.geo, L, e.prot, e.changed);
-    hipLaunchKernelGGL(sparse3d_step<4>, g, b, 0, s, s0, s1, e.geo, L, e.prot, e.changed);
-    hipLaunchKernelGGL(sparse3d_step<5>, g, b, 0, s, s1, s0, e.geo, L, e.prot, e.changed);
-    hipLaunchKernelGGL(sparse3d_compact, dim3(need), b, 0, s, s0, s1, e.changed, L, N, e.states,
-                       e.ctl, e.nitems);
+#define BGT_STEP(D, SRC, DST)                                                                    \
+    hipLaunchKernelGGL(sparse3d_step<D>, g, b, 0, s, SRC, DST, e.geo, A, e.list[0], e.prot,      \
+                       e.first, e.changed, e.ctl, e.dstamp)
+    BGT_STEP(0, s0, s1);
+    BGT_STEP(1, s1, s0);
+    BGT_STEP(2, s0, s1);
+    BGT_STEP(3, s1, s0);
+    BGT_STEP(4, s0, s1);
+    BGT_STEP(5, s1, s0);
+#undef BGT_STEP
+    hipLaunchKernelGGL(sparse3d_expand, dim3((unsigned)std::max(1, (e.nblocks + 255) / 256)), b, 0,
+                       s, e.items, e.nitems, e.dstamp, e.bstamp, e.nblocks, e.ctl);
+    hipLaunchKernelGGL(sparse3d_compact, dim3(need), b, 0, s, s0, s1, e.changed, L, N, A, e.bstamp,
+                       e.states, e.ctl, e.nitems);
     BGT_HIP(hipGetLastError());
     return BGT_OK;
 }

@@ -80,17 +80,18 @@ static int *poll_buffer3()
 }
 
 // Engine for one volume in a workspace: [item|geo|io] [states+ctl] [changed]
-// [state 0] [state 1] [list 0 item|word] [list 1 item|word] [prot].
+// [state 0] [state 1] [list 0 item|word|block] [list 1 ...] [active ...]
+// [prot] [first] [block stamps].
 struct OneVolume {
     Sparse3D e;
     SItem3D item;
 };
 
-static size_t one_volume_bytes(long nwords, long data)
+static size_t one_volume_bytes(long nwords, long data, long nblocks)
 {
     return 3 * 256 + round_up((ST_INTS + SC_INTS) * sizeof(int), 256) + 256 +
            2 * round_up(nwords * sizeof(uint32_t), 256) +
-           4 * round_up(data * sizeof(uint32_t), 256) + round_up(data * sizeof(uint32_t), 256);
+           11 * round_up(data * sizeof(uint32_t), 256) + 2 * round_up(nblocks * sizeof(int), 256);
 }
 
 static int one_volume_setup(int w, int h, int d, const uint8_t *in, uint8_t *out, Workspace &ws,
@@ -99,7 +100,7 @@ static int one_volume_setup(int w, int h, int d, const uint8_t *in, uint8_t *out
     Layout3D lay;
     BGT_CHECK(sparse3d_layout(&w, &h, &d, 1, lay));
     const long nwords = lay.nwords, data = lay.data_words;
-    BGT_CHECK(workspace_reserve(ws, one_volume_bytes(nwords, data)));
+    BGT_CHECK(workspace_reserve(ws, one_volume_bytes(nwords, data, lay.nblocks)));
     char *p = (char *)ws.ptr;
     auto take = [&](size_t bytes) {
         char *r = p;
@@ -128,10 +129,19 @@ static int one_volume_setup(int w, int h, int d, const uint8_t *in, uint8_t *out
     for (int k = 0; k < 2; ++k) {
         e.list[k].item = (int *)take(data * sizeof(int));
         e.list[k].word = (uint32_t *)take(data * sizeof(uint32_t));
+        e.list[k].blk = (int *)take(data * sizeof(int));
         e.list[k].count = e.ctl + SC_COUNT0 + k;
     }
+    e.act.item = (int *)take(data * sizeof(int));
+    e.act.word = (uint32_t *)take(data * sizeof(uint32_t));
+    e.act.blk = (int *)take(data * sizeof(int));
+    e.act.count = nullptr;  // in ctl, by iteration parity
     e.prot = (uint32_t *)take(data * sizeof(uint32_t));
+    e.first = (uint32_t *)take(data * sizeof(uint32_t));
     e.capacity = (int)data;
+    e.nblocks = (int)lay.nblocks;
+    e.bstamp = (int *)take(lay.nblocks * sizeof(int));
+    e.dstamp = (int *)take(lay.nblocks * sizeof(int));
     ov.item = lay.items[0];
     hipLaunchKernelGGL(init_one3d, dim3(1), dim3(64), 0, s, d_item, d_geo, d_io, lay.items[0],
                        SIO{in, out, 0, 0});
@@ -150,10 +160,10 @@ static int one_volume_run(OneVolume &ov, const Config3D &cfg, hipStream_t s, int
     const long max_iter = (long)ov.item.w * ov.item.h * ov.item.d + 2;
     const int grid = sparse3d_grid(cfg.blocks_per_cu);
     long done_iters = 0;
-    int cur = 0, chunk = std::min(2, cfg.max_chunk), bound = e.capacity;
+    int cur = 0, chunk = std::min(2, cfg.max_chunk), bound = e.capacity, abound = e.capacity;
     for (;;) {
         for (int i = 0; i < chunk; ++i) {
-            BGT_CHECK(sparse3d_iteration(e, cur, bound, grid, s));
+            BGT_CHECK(sparse3d_iteration(e, cur, bound, abound, grid, s));
             cur ^= 1;
         }
         done_iters += chunk;
@@ -163,6 +173,7 @@ static int one_volume_run(OneVolume &ov, const Config3D &cfg, hipStream_t s, int
         if (hp[ST_DONE]) break;
         if (done_iters > max_iter) return fail(BGT_ERR_NOT_CONVERGED, "3D thinning did not converge");
         bound = hp[ST_INTS + SC_COUNT0 + cur];
+        abound = hp[ST_INTS + SC_ACOUNT0 + (hp[ST_INTS + SC_ITER] & 1)];
         chunk = std::min(chunk * 2, cfg.max_chunk);
     }
     if (iterations) *iterations = hp[ST_ITERS];
