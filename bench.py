@@ -176,7 +176,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of tuning them on the batch")
     ap.add_argument("--only-batch", action="store_true",
-                    help="time the batched step only (no per-image loop, latency or CPU leg)")
+                    help="time the batched step only, at the default launch shapes (no autotune, "
+                         "per-image loop, latency, serving or CPU leg)")
     ap.add_argument("--serve-qps", type=float, default=1000.0,
                     help="Poisson arrival rate of the serving leg (0 = skip it)")
     ap.add_argument("--serve-requests", type=int, default=2000)
@@ -187,6 +188,7 @@ def main():
     if args.only_batch:
         args.no_cpu_baseline = True
         args.serve_qps = 0
+        args.no_autotune = True
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
               "(items shard across GPUs with no communication)", file=sys.stderr)
@@ -345,8 +347,10 @@ def main():
         log(f"cpu baseline {cpu['value']} Mvoxel/s; bit-exact items {ok}/{len(items)}")
 
     traffic, traffic_shape = measured_traffic()
+    # the library's default 2D shape (4 waves, 8 rows, K = 3) unless set by its environment
     shape = ((tuned["waves"], tuned["rows"], tuned["iters_per_pass"]) if tuned
-             else (4, 16, 3))
+             else tuple(int(os.environ.get(k, v)) for k, v in
+                        (("BGT_2D_WAVES", 4), ("BGT_2D_ROWS", 8), ("BGT_2D_K", 3))))
     if traffic is not None and traffic_shape != shape:
         log(f"traffic figure was measured at launch shape {traffic_shape}, this run uses "
             f"{shape}: not reported")
