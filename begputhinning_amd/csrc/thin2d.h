@@ -14,7 +14,7 @@ struct Tile2D;
 // Launch shape of the fused 2D pass kernel.
 struct Config2D {
     int waves = 4;           // wave64s per workgroup
-    int rows = 8;            // register rows per lane (8 or 16)
+    int rows = 16;           // register rows per lane (8 or 16); 16 fits every width
     int iters_per_pass = 3;  // K: iterations fused per HBM pass (halo 4K rows)
     int blocks_per_cu = 2;   // batch queue workgroups per CU (0 = as many as fit)
 };

@@ -171,7 +171,7 @@ def thin3d_device(vol, out=None, stream=None):
     return out, it.value
 
 
-def set_config2d(waves=4, rows=8, iters_per_pass=3, blocks_per_cu=2):
+def set_config2d(waves=4, rows=16, iters_per_pass=3, blocks_per_cu=2):
     """Launch shape of the 2D pass kernel for later calls: waves per
     workgroup, register rows per lane, iterations per HBM pass and batch
     workgroups per CU (0 = occupancy limit)."""

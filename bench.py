@@ -176,7 +176,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of tuning them on the batch")
     ap.add_argument("--only-batch", action="store_true",
-                    help="time the batched step only, at the default launch shapes (no autotune, "
+                    help="time the batched step only, at the library's launch shapes (no autotune, "
                          "per-image loop, latency, serving or CPU leg)")
     ap.add_argument("--serve-qps", type=float, default=1000.0,
                     help="Poisson arrival rate of the serving leg (0 = skip it)")
@@ -347,10 +347,10 @@ def main():
         log(f"cpu baseline {cpu['value']} Mvoxel/s; bit-exact items {ok}/{len(items)}")
 
     traffic, traffic_shape = measured_traffic()
-    # the library's default 2D shape (4 waves, 8 rows, K = 3) unless set by its environment
+    # the library's default 2D shape (4 waves, 16 rows, K = 3) unless set by its environment
     shape = ((tuned["waves"], tuned["rows"], tuned["iters_per_pass"]) if tuned
              else tuple(int(os.environ.get(k, v)) for k, v in
-                        (("BGT_2D_WAVES", 4), ("BGT_2D_ROWS", 8), ("BGT_2D_K", 3))))
+                        (("BGT_2D_WAVES", 4), ("BGT_2D_ROWS", 16), ("BGT_2D_K", 3))))
     if traffic is not None and traffic_shape != shape:
         log(f"traffic figure was measured at launch shape {traffic_shape}, this run uses "
             f"{shape}: not reported")
