@@ -215,7 +215,8 @@ struct PlaneBits {
     }
 };
 
-// One thread per 64-byte block of a z-plane of the caller's buffer.
+// One thread per 64-byte block of a z-plane of the caller's buffer (whole
+// waves per plane: the stores are exchanged within the wave).
 __global__ __launch_bounds__(256) void sparse3d_unpack(const SItem3D *__restrict__ items,
                                                        const SIO *__restrict__ io,
                                                        const int *__restrict__ ublk,
@@ -225,13 +226,13 @@ __global__ __launch_bounds__(256) void sparse3d_unpack(const SItem3D *__restrict
     const SItem3D it = items[i];
     const SIO f = io[i];
     const uint32_t plane = (uint32_t)it.w * (uint32_t)it.h;
-    const uint32_t cpp = unpack_blocks(plane);
+    const uint32_t cpp = unpack_wave_blocks(plane);  // whole waves per plane
     const uint32_t t = (uint32_t)(blockIdx.x - f.ublk0) * 256u + threadIdx.x;
     const uint32_t z = t / cpp, c = t - z * cpp;
     if (z >= (uint32_t)it.d) return;
     const PlaneBits src{s0, (uint32_t)it.base + ((z + 1) * (uint32_t)(it.h + 2) + 1) * (uint32_t)it.P + 1,
                         (uint32_t)it.P};
-    unpack_block64_u8(f.out + (size_t)z * plane, plane, (uint32_t)it.w, c, src);
+    unpack_block64_wave_u8(f.out + (size_t)z * plane, plane, (uint32_t)it.w, c, src);
 }
 
 // Subiteration D (0..5 = U, N, E, D, S, W) over the active list (list 0 in
@@ -449,7 +450,7 @@ int sparse3d_layout(const int *w, const int *h, const int *d, int n, Layout3D &l
         const long threads = (long)h[i] * d[i] * (it.nh / 2);
         lay.blk0[i] = (int)lay.blk.size();
         for (long k = 0; k < (threads + 255) / 256; ++k) lay.blk.push_back(i);
-        const long uthreads = (long)d[i] * unpack_blocks((uint32_t)(w[i] * h[i]));
+        const long uthreads = (long)d[i] * unpack_wave_blocks((uint32_t)(w[i] * h[i]));
         lay.ublk0[i] = (int)lay.ublk.size();
         for (long k = 0; k < (uthreads + 255) / 256; ++k) lay.ublk.push_back(i);
     }
