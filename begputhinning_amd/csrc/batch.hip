@@ -70,14 +70,21 @@ __global__ __launch_bounds__(256) void batch_pack2d(const Item2D *__restrict__ i
                                                     const IO2D *__restrict__ io,
                                                     const int *__restrict__ blk_item, uint8_t *occ)
 {
+    __shared__ uint32_t s_masks[4][320];
     const int i = blk_item[blockIdx.x];
     const Item2D it = items[i];
     const IO2D f = io[i];
     const int t = (blockIdx.x - f.blk0) * 256 + threadIdx.x;
-    if (t >= it.h * it.L) return;
-    const int y = t / it.L, x0 = (t - y * it.L) * 64;
-    const uint64_t v =
-        x0 < it.w ? pack_run_u8(f.in + (size_t)y * f.in_pitch + x0, min(64, it.w - x0)) : 0ull;
+    const bool valid = t < it.h * it.L;
+    const int y = valid ? t / it.L : 0, x0 = valid ? (t - y * it.L) * 64 : 0;  // x0 < w
+    const int n = min(64, it.w - x0);
+    const uint8_t *p = f.in + (size_t)y * f.in_pitch + x0;
+    uint64_t v;
+    if (f.in_pitch == (size_t)it.w)  // contiguous rows (the same for the whole workgroup)
+        v = pack_runs_wave_u8(p, n, valid, s_masks[threadIdx.x >> 6]);
+    else
+        v = valid ? pack_run_u8(p, n) : 0ull;
+    if (!valid) return;
     it.buf[0][t] = v;
     it.buf[1][t] = v;
     if (v) occ[it.tile0 + y / it.band_rows] = 1;

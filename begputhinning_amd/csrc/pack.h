@@ -55,6 +55,58 @@ __device__ __forceinline__ uint64_t pack_run_u8(const uint8_t *p, int n)
     return v;
 }
 
+// The same for the runs of a whole wave, with coalesced loads.  Every lane of
+// the wave calls it (`valid` lanes form a prefix), and so does every other
+// wave of the workgroup (it holds a barrier).  The valid lanes' runs must
+// follow each other in memory (lane l+1's run starts where lane l's ends, as
+// the runs of a contiguous image or volume do), so they span at most 4096
+// bytes.  The wave loads the 16-byte aligned blocks overlapping that span --
+// at most 257, five uint4 instructions of 64 consecutive blocks, each
+// covering 1 KiB contiguously -- turns each into a 16-bit mask in LDS (`lds`:
+// 320 words per wave), and each lane assembles its run's 80-bit window from
+// the masks, as pack_run_u8 does from its own loads.
+__device__ __forceinline__ uint64_t pack_runs_wave_u8(const uint8_t *p, int n, bool valid,
+                                                      uint32_t *lds)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t vm = __ballot(valid);
+    uintptr_t a0 = 0;
+    uint32_t nb = 0;
+    if (vm) {
+        const int last = 63 - __builtin_clzll(vm);
+        const uintptr_t s = (uintptr_t)p, e = (uintptr_t)p + (uintptr_t)(valid ? n : 0);
+        const uintptr_t s0 = ((uintptr_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)s, 0)) |
+                             ((uintptr_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(s >> 32), 0) << 32);
+        const uintptr_t e1 = ((uintptr_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)e, last)) |
+                             ((uintptr_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(e >> 32), last) << 32);
+        a0 = s0 & ~(uintptr_t)15;
+        nb = (uint32_t)((((e1 + 15) & ~(uintptr_t)15) - a0) >> 4);
+    }
+    const uint4 *q = reinterpret_cast<const uint4 *>(a0);
+#pragma unroll
+    for (uint32_t j = 0; j < 5; ++j) {
+        const uint32_t k = 64u * j + lane;
+        uint32_t m = 0;
+        if (k < nb) {
+            const uint4 v = q[k];
+            m = nz4(v.x) | nz4(v.y) << 4 | nz4(v.z) << 8 | nz4(v.w) << 12;
+        }
+        lds[k] = m;
+    }
+    __syncthreads();
+    uint64_t v = 0;
+    if (valid) {
+        const uint32_t o = (uint32_t)((uintptr_t)p - a0), kb = o >> 4, r = o & 15u;
+        const uint32_t m0 = lds[kb] | lds[kb + 1] << 16, m1 = lds[kb + 2] | lds[kb + 3] << 16;
+        const uint32_t m2 = lds[kb + 4];
+        const uint32_t lo = __builtin_amdgcn_alignbit(m1, m0, r);
+        const uint32_t hi = __builtin_amdgcn_alignbit(m2, m1, r);
+        v = ((uint64_t)hi << 32) | lo;
+        if (n < 64) v &= (1ull << n) - 1ull;
+    }
+    return v;
+}
+
 // Writes the n (1..64) bytes at p as 0/1 from bits v (bit k -> p[k]).
 __device__ __forceinline__ void unpack_run_u8(uint64_t v, uint8_t *p, int n)
 {

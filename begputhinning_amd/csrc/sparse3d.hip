@@ -168,14 +168,17 @@ __global__ __launch_bounds__(256) void sparse3d_pack(const SItem3D *__restrict__
     const SItem3D it = items[i];
     const SIO f = io[i];
     const uint32_t nq = (uint32_t)it.nh >> 1;
+    __shared__ uint32_t s_masks[4][320];
     const uint32_t t = (uint32_t)(blockIdx.x - f.blk0) * 256u + threadIdx.x;
+    const bool valid = t < (uint32_t)it.h * (uint32_t)it.d * nq;
+    const uint32_t line = valid ? t / nq : 0;
+    const int x0 = valid ? (int)(t - line * nq) * 64 : 0;  // < w: nq = ceil(w / 64)
+    // the caller's lines follow each other in memory, and so do the runs
+    const uint64_t v = pack_runs_wave_u8(f.in + (size_t)line * it.w + x0, min(64, it.w - x0), valid,
+                                         s_masks[threadIdx.x >> 6]);
     uint32_t lo = 0, hi = 0, g = 0;
     int b0 = 0;
-    if (t < (uint32_t)it.h * (uint32_t)it.d * nq) {
-        const uint32_t line = t / nq;
-        const int x0 = (int)(t - line * nq) * 64;
-        const uint64_t v =
-            x0 < it.w ? pack_run_u8(f.in + (size_t)line * it.w + x0, min(64, it.w - x0)) : 0ull;
+    if (valid) {
         lo = (uint32_t)v;
         hi = (uint32_t)(v >> 32);
         const uint32_t y = line % (uint32_t)it.h, z = line / (uint32_t)it.h;

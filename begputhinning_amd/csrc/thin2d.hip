@@ -53,14 +53,16 @@ __device__ __forceinline__ T one_value()
 // ---------------------------------------------------------------------------
 
 // uint8 rows (pitch bytes) -> packed words; one thread per 64-bit word
-__global__ void pack2d_u8(const uint8_t *__restrict__ in, size_t pitch, int w, int h, int L,
-                          uint64_t *__restrict__ out)
+// Per-image pack: one thread per 64-pixel run with its own aligned loads (the
+// wave-coalesced pack_runs_wave_u8 pays off on whole batches, but its LDS
+// round trip costs more than it saves on a single latency-bound frame).
+__global__ __launch_bounds__(256) void pack2d_u8(const uint8_t *__restrict__ in, size_t pitch,
+                                                 int w, int h, int L, uint64_t *__restrict__ out)
 {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)L * h) return;
-    const int y = (int)(idx / L), c = (int)(idx % L);
-    const int x0 = c * 64;
-    out[idx] = x0 < w ? pack_run_u8(in + (size_t)y * pitch + x0, min(64, w - x0)) : 0ull;
+    const int y = (int)(idx / L), x0 = (int)(idx - (long)y * L) * 64;  // x0 < w
+    out[idx] = pack_run_u8(in + (size_t)y * pitch + x0, min(64, w - x0));
 }
 
 template <typename T>
