@@ -150,12 +150,21 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
     int *nlist = q.list[cur ^ 1];
     int *ncnt = &q.cnt[cur ^ 1];
     const int n = q.cnt[cur];
-    for (;;) {
-        if (tid == 0) info.t = atomicAdd(q.wctr, 1);
-        __syncthreads();
-        const int t = info.t;
-        __syncthreads();
-        if (t >= n) break;
+    // Workgroup b takes band b first, and only the bands past the grid go
+    // through the work counter: same-address atomics, one per grab, cost about
+    // as much as a short pass when every workgroup makes one.  Workgroups past
+    // the queue leave at once.
+    if ((int)blockIdx.x >= n) return;
+    for (int k = 0;; ++k) {
+        int t = blockIdx.x;
+        if (k > 0) {
+            if (n <= (int)gridDim.x) break;
+            if (tid == 0) info.t = (int)gridDim.x + atomicAdd(q.wctr, 1);
+            __syncthreads();
+            t = info.t;
+            __syncthreads();
+            if (t >= n) break;
+        }
         const int tile = list[t];
         const Tile2D tl = tiles[tile];
         const Item2D it = items[tl.item];
