@@ -273,6 +273,7 @@ struct Batch {
     bool zeroed3 = false;  // state margins cleared since allocation
     int bound3 = 0;        // upper bound of the current 3D list's count
     int abound3 = 0;       // upper bound of the current 3D active list's count
+    int iter3 = 0;         // 3D iterations completed at the last poll
     int cur3 = 0;          // current 3D list
     std::vector<std::pair<long, hipGraphExec_t>> graphs3;  // by launch bounds
     hipStream_t graph3_stream = nullptr;
@@ -690,6 +691,7 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         BGT_CHECK(sparse3d_begin(b.e3, !b.zeroed3, s));
         b.zeroed3 = true;
         b.bound3 = b.abound3 = b.e3.capacity;
+        b.iter3 = 0;
     }
 
     // With both kinds of items, the 3D pipeline runs on a side stream forked
@@ -715,6 +717,7 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     // last item converged finds empty queues and lists.
     struct Chain {
         bool live = false;
+        bool final = false;          // its last launch is queued (the 3D tail)
         int head = 0, inflight = 0;  // oldest pending poll slot, pending polls
         long chunks = 0;
     } c2, c3;
@@ -728,7 +731,15 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
             BGT_HIP(hipMemcpyAsync(hs, b.d_live2, sizeof(int), hipMemcpyDeviceToHost, s));
             BGT_HIP(hipEventRecord(b.ev_poll2[slot], s));
         } else {
-            BGT_CHECK(launch_chunk(b, true, use_graph, s3));
+            if (b.c3.tail_words > 0 && b.iter3 >= 1 && b.abound3 <= b.c3.tail_words) {
+                // few active words left: the remaining iterations in one launch
+                BGT_CHECK(timed_mark(b, s3, 1, true));
+                BGT_CHECK(sparse3d_tail(b.e3, s3));
+                BGT_CHECK(timed_mark(b, s3, 1, false));
+                c.final = true;
+            } else {
+                BGT_CHECK(launch_chunk(b, true, use_graph, s3));
+            }
             BGT_HIP(hipMemcpyAsync(hs + 8, b.e3.ctl, SC_INTS * sizeof(int), hipMemcpyDeviceToHost,
                                    s3));
             BGT_HIP(hipEventRecord(b.ev_poll3[slot], s3));
@@ -746,6 +757,7 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
             // chunks hold an even number of iterations: list 0 is current
             b.bound3 = hp[8 + SC_COUNT0];
             b.abound3 = hp[8 + SC_ACOUNT0 + (hp[8 + SC_ITER] & 1)];
+            b.iter3 = hp[8 + SC_ITER];
             if (hp[8 + SC_LIVE] <= 0) c.live = false;
         }
         c.head ^= 1;
@@ -771,10 +783,12 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
             BGT_CHECK(enqueue(c2, false));
             continue;
         }
-        if (c3.live && c3.inflight < 2) {
+        if (c3.live && c3.inflight < 2 && !c3.final) {
             BGT_CHECK(enqueue(c3, true));
             continue;
         }
+        if (c3.live && c3.final && c3.inflight == 0)
+            return fail(BGT_ERR_NOT_CONVERGED, "3D tail did not converge");
         // both live chains have two chunks queued: take whichever poll lands
         // first (blocking when only one chain is left)
         const bool both = c2.live && c3.live;

@@ -91,6 +91,7 @@ static Config3D &config3d_storage()
         Config3D v;
         if (const char *e = std::getenv("BGT_3D_BLOCKS")) v.blocks_per_cu = std::atoi(e);
         if (const char *e = std::getenv("BGT_3D_CHUNK")) v.max_chunk = std::atoi(e);
+        if (const char *e = std::getenv("BGT_3D_TAIL")) v.tail_words = std::atoi(e);
         return v;
     }();
     return c;
@@ -120,11 +121,12 @@ int bgt_set_config2d(int waves, int rows, int iters_per_pass, int blocks_per_cu)
     return BGT_OK;
 }
 
-int bgt_set_config3d(int blocks_per_cu, int max_chunk)
+int bgt_set_config3d(int blocks_per_cu, int max_chunk, int tail_words)
 {
     Config3D c;
     c.blocks_per_cu = blocks_per_cu;
     c.max_chunk = max_chunk;
+    c.tail_words = tail_words;
     BGT_CHECK(check_config3d(c));
     config3d_storage() = c;
     return BGT_OK;

@@ -39,6 +39,7 @@ struct SItem3D {
     int P, plane;  // padded line and plane strides in words
     int nby, nbz;  // blocks along y and z (nh along x)
     int bbase;     // first block of the item
+    int dbase;     // sum of the data words of the items before it
 };
 
 // Caller buffers of one item: uint8 voxels, x fastest, then y, then z.
@@ -107,6 +108,16 @@ int sparse3d_grid(int blocks_per_cu);
 // later), packs every item into both state arrays and builds list 0 and the
 // first active list (all of list 0).
 int sparse3d_begin(const Sparse3D &e, bool zero, hipStream_t s);
+
+// The remaining iterations of every open item in one launch: one workgroup of
+// 1024 threads per item runs the six subiterations over the item's active
+// words with workgroup barriers between them, marks the blocks its changes
+// touch, and gathers the next active words from the non-empty words of the
+// blocks around them, until an iteration changes nothing.  For the tail of a
+// run, when few words are active: it replaces the per-iteration launches (and
+// their dispatch gaps) by one.  Uses the list, prot and first arrays as
+// per-item scratch (item i's part starts at its dbase).
+int sparse3d_tail(const Sparse3D &e, hipStream_t s);
 
 // One iteration: six subiteration launches over the active list, then the
 // compaction of list `cur` into list cur^1 (which also builds the next active

@@ -238,6 +238,51 @@ __global__ __launch_bounds__(256) void sparse3d_unpack(const SItem3D *__restrict
     unpack_block64_wave_u8(f.out + (size_t)z * plane, plane, (uint32_t)it.w, c, src);
 }
 
+// The 3x3x3 word neighbourhood of interior word g, each word shifted so that
+// bit k holds the neighbour of voxel k (raster (dz, dy, dx) order).
+__device__ __forceinline__ void gather27(const uint32_t *__restrict__ src, uint32_t g, int2 gp,
+                                         uint32_t (&nb)[27])
+{
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+            const uint32_t *r = src + (g + (uint32_t)((dz - 1) * gp.y + (dy - 1) * gp.x));
+            const uint32_t a = r[-1], b = r[0], c = r[1];
+            const int q = dz * 9 + dy * 3;
+            nb[q + 0] = __builtin_amdgcn_alignbit(b, a, 31);  // x-1
+            nb[q + 1] = b;
+            nb[q + 2] = __builtin_amdgcn_alignbit(c, b, 1);   // x+1
+        }
+}
+
+// Subiteration D on word g: src -> dst.  U (D = 0) also returns the
+// protection mask and the word's value at the start of the iteration; the
+// other steps take the protection mask.  Returns the deleted voxels.
+template <int D>
+__device__ __forceinline__ uint32_t sub_word(const uint32_t *__restrict__ src, uint32_t *dst,
+                                             uint32_t g, int2 gp, uint32_t &prot, uint32_t &first,
+                                             uint32_t &nv)
+{
+    uint32_t nb[27];
+    gather27(src, g, gp, nb);
+    const uint32_t v = nb[13];
+    uint32_t del;
+    if (D == 0) {
+        const uint32_t cu = rule3<0>(nb);
+        const uint32_t any = cu | rule3<1>(nb) | rule3<2>(nb) | rule3<3>(nb) | rule3<4>(nb) |
+                             rule3<5>(nb);
+        prot = v & ~any;
+        first = v;
+        del = cu & v;
+    } else {
+        del = rule3<D>(nb) & v & ~prot;
+    }
+    nv = v ^ del;
+    dst[g] = nv;
+    return del;
+}
+
 // Subiteration D (0..5 = U, N, E, D, S, W) over the active list (list 0 in
 // the first iteration): src -> dst.  The last one stamps the blocks of the
 // words the iteration changed.
@@ -263,33 +308,12 @@ __global__ __launch_bounds__(256) void sparse3d_step(const uint32_t *__restrict_
     for (int k = lo + local * 256 + threadIdx.x; k < hi; k += per * 256) {
         const int ii = list.item[k];
         const uint32_t g = list.word[k];
-        const int2 gp = geo[ii];
-        uint32_t nb[27];
-#pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-            for (int dy = 0; dy < 3; ++dy) {
-                const uint32_t *r = src + (g + (uint32_t)((dz - 1) * gp.y + (dy - 1) * gp.x));
-                const uint32_t a = r[-1], b = r[0], c = r[1];
-                const int q = dz * 9 + dy * 3;
-                nb[q + 0] = __builtin_amdgcn_alignbit(b, a, 31);  // x-1
-                nb[q + 1] = b;
-                nb[q + 2] = __builtin_amdgcn_alignbit(c, b, 1);   // x+1
-            }
-        const uint32_t v = nb[13];
-        uint32_t del;
+        uint32_t pk = D == 0 ? 0u : prot[k], fk = 0u, nv;
+        const uint32_t del = sub_word<D>(src, dst, g, geo[ii], pk, fk, nv);
         if (D == 0) {
-            const uint32_t cu = rule3<0>(nb);
-            const uint32_t any = cu | rule3<1>(nb) | rule3<2>(nb) | rule3<3>(nb) | rule3<4>(nb) |
-                                 rule3<5>(nb);
-            prot[k] = v & ~any;
-            first[k] = v;
-            del = cu & v;
-        } else {
-            del = rule3<D>(nb) & v & ~prot[k];
+            prot[k] = pk;
+            first[k] = fk;
         }
-        const uint32_t nv = v ^ del;
-        dst[g] = nv;
         if (del) changed[ii] = 1;
         if (D == 5 && nv != first[k]) dstamp[list.blk[k]] = itn + 1;
     }
@@ -415,7 +439,151 @@ __global__ __launch_bounds__(256) void sparse3d_compact(const uint32_t *__restri
     }
 }
 
+// Block b of item `it` as (bx, by, bz).
+__device__ __forceinline__ void block_xyz(const SItem3D &it, int b, int &bx, int &by, int &bz)
+{
+    const int l = b - it.bbase, byz = l / it.nh;
+    bx = l - byz * it.nh;
+    bz = byz / it.nby;
+    by = byz - bz * it.nby;
+}
+
+// The remaining iterations of item blockIdx.x (see sparse3d_tail).  All
+// branches on shared values are taken after a barrier, so every thread
+// reaches every barrier; the loop ends when an iteration changes nothing,
+// which each iteration that does change something brings closer (it deletes
+// a voxel).
+__global__ __launch_bounds__(1024) void sparse3d_tail_kernel(
+    const SItem3D *__restrict__ items, const int2 *__restrict__ geo, const SList act,
+    const SList l0, const SList l1, uint32_t *s0, uint32_t *s1, uint32_t *prot, uint32_t *first,
+    int *states, int *ctl, int *bstamp, int *dstamp)
+{
+    __shared__ int s_na, s_nd, s_nab, s_nn, s_changed;
+    const int i = blockIdx.x, tid = threadIdx.x, NT = blockDim.x;
+    int *st = states + (size_t)i * ST_INTS;
+    if (st[ST_DONE]) return;  // no other workgroup writes item i's state
+    const SItem3D it = items[i];
+    const int2 gp = geo[i];
+    // per-item scratch: active words (two generations), dirty and active blocks
+    uint32_t *aw[2] = {l0.word + it.dbase, l1.word + it.dbase};
+    int *ab[2] = {l0.blk + it.dbase, l1.blk + it.dbase};
+    int *dirty = l0.item + it.dbase, *actb = l1.item + it.dbase;
+    uint32_t *pr = prot + it.dbase, *fi = first + it.dbase;
+    const int itn0 = ctl[SC_ITER];
+    if (tid == 0) {
+        s_na = 0;
+        s_nd = 0;
+        s_nab = 0;
+        s_nn = 0;
+        s_changed = 0;
+    }
+    __syncthreads();
+    // the item's entries of the current active list (the host launches this
+    // after iteration 0, whose active list is list 0, the scratch here)
+    {
+        const int ng = ctl[SC_ACOUNT0 + (itn0 & 1)];
+        for (int k = tid; k < ng; k += NT)
+            if (act.item[k] == i) {
+                const int p = atomicAdd(&s_na, 1);
+                aw[0][p] = act.word[k];
+                ab[0][p] = act.blk[k];
+            }
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int t = itn0;; ++t) {
+        const int na = s_na, stamp = t + 1;
+        const uint32_t *w = aw[cur];
+#define BGT_TAIL_SUB(D, SRC, DST)                                                          \
+    for (int k = tid; k < na; k += NT) {                                                   \
+        uint32_t pk = D == 0 ? 0u : pr[k], fk = 0u, nv;                                     \
+        const uint32_t del = sub_word<D>(SRC, DST, w[k], gp, pk, fk, nv);                   \
+        if (D == 0) {                                                                      \
+            pr[k] = pk;                                                                    \
+            fi[k] = fk;                                                                    \
+        }                                                                                  \
+        if (del) s_changed = 1;                                                            \
+        if (D == 5 && nv != fi[k]) {                                                       \
+            const int b = ab[cur][k];                                                      \
+            if (atomicMax(&dstamp[b], stamp) < stamp) dirty[atomicAdd(&s_nd, 1)] = b;      \
+        }                                                                                  \
+    }                                                                                      \
+    __syncthreads();
+        BGT_TAIL_SUB(0, s0, s1)
+        BGT_TAIL_SUB(1, s1, s0)
+        BGT_TAIL_SUB(2, s0, s1)
+        BGT_TAIL_SUB(3, s1, s0)
+        BGT_TAIL_SUB(4, s0, s1)
+        BGT_TAIL_SUB(5, s1, s0)
+#undef BGT_TAIL_SUB
+        if (!s_changed) {
+            if (tid == 0) {
+                st[ST_ITERS] = t + 1;
+                st[ST_PARITY] = 0;
+                st[ST_DONE] = 1;
+                atomicSub(ctl + SC_LIVE, 1);
+            }
+            return;
+        }
+        // words W emptied still hold their bits from before W in array 1
+        for (int k = tid; k < na; k += NT)
+            if (s0[w[k]] == 0u) s1[w[k]] = 0u;
+        // activate the 27 blocks around every dirty block
+        const int nd = s_nd;
+        for (int x = tid; x < nd * 27; x += NT) {
+            const int b = dirty[x / 27], j = x % 27;
+            const int dz = j / 9 - 1, dy = (j / 3) % 3 - 1, dx = j % 3 - 1;
+            int bx, by, bz;
+            block_xyz(it, b, bx, by, bz);
+            if (bx + dx < 0 || bx + dx >= it.nh || by + dy < 0 || by + dy >= it.nby ||
+                bz + dz < 0 || bz + dz >= it.nbz)
+                continue;
+            const int nb = b + (dz * it.nby + dy) * it.nh + dx;
+            if (atomicMax(&bstamp[nb], stamp) < stamp) actb[atomicAdd(&s_nab, 1)] = nb;
+        }
+        __syncthreads();
+        // the next active words: the non-empty words of the active blocks
+        const int nab = s_nab;
+        for (int x = tid; x < nab * 64; x += NT) {
+            const int b = actb[x >> 6], q = x & 63;
+            int bx, by, bz;
+            block_xyz(it, b, bx, by, bz);
+            const int y = by * 8 + (q & 7), z = bz * 8 + (q >> 3);
+            if (y >= it.h || z >= it.d) continue;
+            const uint32_t g = (uint32_t)it.base +
+                               ((uint32_t)(z + 1) * (uint32_t)(it.h + 2) + (uint32_t)(y + 1)) *
+                                   (uint32_t)it.P +
+                               1u + (uint32_t)bx;
+            if (s0[g] != 0u) {
+                const int p = atomicAdd(&s_nn, 1);
+                aw[cur ^ 1][p] = g;
+                ab[cur ^ 1][p] = b;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            s_na = s_nn;
+            s_nn = 0;
+            s_nd = 0;
+            s_nab = 0;
+            s_changed = 0;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+}
+
 }  // namespace
+
+int sparse3d_tail(const Sparse3D &e, hipStream_t s)
+{
+    if (e.nitems > 0)
+        hipLaunchKernelGGL(sparse3d_tail_kernel, dim3(e.nitems), dim3(1024), 0, s, e.items, e.geo,
+                           e.act, e.list[0], e.list[1], e.state[0], e.state[1], e.prot, e.first,
+                           e.states, e.ctl, e.bstamp, e.dstamp);
+    BGT_HIP(hipGetLastError());
+    return BGT_OK;
+}
 
 int sparse3d_layout(const int *w, const int *h, const int *d, int n, Layout3D &lay)
 {
@@ -447,6 +615,7 @@ int sparse3d_layout(const int *w, const int *h, const int *d, int n, Layout3D &l
         it.nby = (h[i] + 7) / 8;
         it.nbz = (d[i] + 7) / 8;
         it.bbase = (int)std::min<long>(blocks, 0x7fffffffL);
+        it.dbase = (int)std::min<long>(data, 0x7fffffffL);
         blocks += (long)it.nh * it.nby * it.nbz;
         base += total;
         data += (long)h[i] * d[i] * it.nh;

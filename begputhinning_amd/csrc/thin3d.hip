@@ -64,9 +64,10 @@ __global__ void init_one3d(SItem3D *item, int2 *geo, SIO *io, SItem3D v, SIO f)
 
 int check_config3d(const Config3D &c)
 {
-    if (c.blocks_per_cu < 0 || c.blocks_per_cu > 32 || c.max_chunk < 1 || c.max_chunk > 64)
-        return fail(BGT_ERR_INVALID, "bad 3D config (blocks_per_cu=%d max_chunk=%d)",
-                    c.blocks_per_cu, c.max_chunk);
+    if (c.blocks_per_cu < 0 || c.blocks_per_cu > 32 || c.max_chunk < 1 || c.max_chunk > 64 ||
+        c.tail_words < 0)
+        return fail(BGT_ERR_INVALID, "bad 3D config (blocks_per_cu=%d max_chunk=%d tail_words=%d)",
+                    c.blocks_per_cu, c.max_chunk, c.tail_words);
     return BGT_OK;
 }
 
@@ -174,6 +175,15 @@ static int one_volume_run(OneVolume &ov, const Config3D &cfg, hipStream_t s, int
         if (done_iters > max_iter) return fail(BGT_ERR_NOT_CONVERGED, "3D thinning did not converge");
         bound = hp[ST_INTS + SC_COUNT0 + cur];
         abound = hp[ST_INTS + SC_ACOUNT0 + (hp[ST_INTS + SC_ITER] & 1)];
+        if (cfg.tail_words > 0 && abound <= cfg.tail_words) {
+            // few active words left: the remaining iterations in one launch
+            BGT_CHECK(sparse3d_tail(e, s));
+            BGT_HIP(hipMemcpyAsync(hp, e.states, (ST_INTS + SC_INTS) * sizeof(int),
+                                   hipMemcpyDeviceToHost, s));
+            BGT_HIP(hipStreamSynchronize(s));
+            if (!hp[ST_DONE]) return fail(BGT_ERR_NOT_CONVERGED, "3D tail did not converge");
+            break;
+        }
         chunk = std::min(chunk * 2, cfg.max_chunk);
     }
     if (iterations) *iterations = hp[ST_ITERS];

@@ -34,6 +34,36 @@ def make_items(seed, n2=24, n3=5):
     return items
 
 
+@pytest.mark.parametrize("tail_words", [0, 2000, 1 << 30])
+def test_3d_tail_in_batch(tail_words):
+    """Volumes of very different depth in one batch, finishing in the
+    one-launch tail from different iterations on: each equals the oracle."""
+    import begputhinning_amd as B
+    from begputhinning_amd.batch import ThinningBatch
+    from begputhinning_amd import synth
+    vols = [synth.tree3d(70, 90, 80, 99), synth.tree3d(40, 30, 120, 5, target_fraction=0.1),
+            np.ones((9, 10, 11), np.uint8), np.zeros((4, 5, 6), np.uint8),
+            synth.tree3d(130, 60, 50, 7, target_fraction=0.02)]
+    ins = [torch.from_numpy(a).cuda() for a in vols]
+    outs = [torch.full_like(t, 9) for t in ins]
+    B.set_config3d(0, 16, tail_words)
+    try:
+        b = ThinningBatch()
+        for t, o in zip(ins, outs):
+            b.add3d(t, o)
+        for rep in range(2):
+            b.run(use_graph=False)
+            torch.cuda.synchronize()
+            its = b.iterations()
+            for k, a in enumerate(vols):
+                ref, rit = O.thin3d(a)
+                assert np.array_equal(outs[k].cpu().numpy(), ref), (tail_words, rep, k)
+                assert its[k] == rit, (tail_words, rep, k, its[k], rit)
+        b.close()
+    finally:
+        B.set_config3d()
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_mixed_batch_matches_oracle(graph):
     from begputhinning_amd.batch import ThinningBatch

@@ -63,6 +63,23 @@ def test_workload3_256():
     check(synth.workload3(), "tree 256^3")
 
 
+@pytest.mark.parametrize("tail_words", [0, 1, 500, 1 << 30])
+def test_tail_launch(tail_words):
+    """The remaining iterations in one launch (one workgroup per volume) from
+    whenever the active words drop to `tail_words`: never (0), late, or from
+    the first poll on; the result and iteration count stay the oracle's."""
+    import begputhinning_amd as B
+    from begputhinning_amd import synth
+    B.set_config3d(0, 16, tail_words)
+    try:
+        check(synth.tree3d(70, 90, 80, 99), f"tail {tail_words}")
+        check(synth.workload3(), f"tail {tail_words} 256^3")
+        rng = np.random.default_rng(tail_words % 1000)
+        check((rng.random((20, 33, 47)) < 0.5).astype(np.uint8), f"tail {tail_words} random")
+    finally:
+        B.set_config3d()
+
+
 @pytest.mark.parametrize("cfg", [(1, 2), (2, 4), (8, 64)])
 def test_launch_shapes(cfg):
     import begputhinning_amd as B
