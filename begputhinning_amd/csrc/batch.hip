@@ -169,11 +169,15 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         const Tile2D tl = tiles[tile];
         const Item2D it = items[tl.item];
         if (it.st[ST_DONE]) continue;
-        const int run = band_work<R>(it, tl.band, K, cur, edges, &info);
+        // a single-band item also runs at most K iterations per pass, so that no
+        // pass waits on one item iterating to convergence while CUs idle
+        band_work<R>(it, tl.band, K, cur, edges, &info, K);
         if (tid == 0) {
             q.occ[tile] = info.occ != 0;
             if (info.mask) atomicOr((unsigned *)&it.st[ST_MASK], info.mask);
-            if (it.nbands == 1) it.st[ST_RUN] = run + 1;
+            // all K iterations deleted something: not converged, go on next pass
+            if (it.nbands == 1 && info.mask == (1u << K) - 1u)
+                enqueue(q, tile, P + 1, nlist, ncnt, true);
         }
         if (info.mask && it.nbands > 1 && tid < 3) {
             const int b = tl.band - 1 + tid;

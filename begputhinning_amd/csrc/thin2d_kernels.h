@@ -198,11 +198,12 @@ struct TileInfo {
 // Run one pass over band `band` of item `it`: K iterations (4K subiterations)
 // from buffer `par` into buffer par^1, or -- when the band is the item's only
 // band, so that its halo rows are all outside the image -- iterations until one
-// deletes nothing.  Results land in `info`; returns the number of iterations
-// that deleted something in the unbounded case (else 0).
+// deletes nothing, at most whole_max of them.  Results land in `info` (bit j
+// of the mask: iteration j deleted an interior pixel); returns the number of
+// iterations that deleted something in the single-band case (else 0).
 template <int R>
 __device__ int band_work(const Item2D &it, int band, int K, int par, uint64_t *edges,
-                         TileInfo *info)
+                         TileInfo *info, int whole_max = 0x7fffffff)
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int NW = blockDim.x >> 6;
@@ -258,7 +259,7 @@ __device__ int band_work(const Item2D &it, int band, int K, int par, uint64_t *e
     unsigned bits = 0;
     int s = 0, changed_iters = 0;
     uint32_t count = count_rows(lo, hi, interior);
-    const int max_iters = whole ? 0x7fffffff : K;
+    const int max_iters = whole ? whole_max : K;
 #pragma unroll 1
     for (int itn = 0; itn < max_iters; ++itn) {
 #define BGT_SUB(D)                                                                           \
@@ -278,10 +279,8 @@ __device__ int band_work(const Item2D &it, int band, int K, int par, uint64_t *e
         const uint32_t now = count_rows(lo, hi, interior);
         const bool ch = now != count;
         count = now;
-        if (!whole) {
-            if (ch) bits |= 1u << itn;
-            continue;
-        }
+        if (ch && itn < 32) bits |= 1u << itn;
+        if (!whole) continue;
         // tile-local convergence: stop after the first iteration that deletes nothing
         if (ch) atomicOr(&info->flag, 1u);
         __syncthreads();
