@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -51,15 +52,30 @@ struct IO2D {
     int flat;           // rows contiguous: the whole image is one segment
 };
 
-// Work queue of one kernel class (all 2D items, or the 3D items of one line
-// width class).  Tile ids index `tiles`, `stamp` and `occ`.
+// Work queue of the 2D bands.  Tile ids index `tiles`, `stamp` and `occ`.
+// Pass P reads list P % 3 and appends to list (P + 1) % 3; list (P + 2) % 3
+// is idle during pass P (read by pass P - 1, appended to by pass P + 1), so
+// pass P clears its counters without racing any reader or writer.
 struct Queue {
-    int *list[2];   // active tile ids of pass parity 0 / 1
-    int *cnt;       // [2] entries in list[0] / list[1]
-    int *wctr;      // next list slot to hand out
+    int *list[3];   // active tile ids, by pass % 3
+    int *cnt;       // [3] entries in each list
+    int *wctr;      // [3] next list slot to hand out, by pass % 3
     int *stamp;     // per tile: last pass it was queued for
     uint8_t *occ;   // per tile: interior non-empty
     int ntiles;
+};
+
+// Per-item convergence of the 2D batch.  The tiles of pass P OR their
+// iteration masks into hist[P % 4][item]; pass P + 1 skips the tiles of an
+// item whose pass-P mask misses an iteration (that iteration deleted nothing
+// in the whole image, so it is a fixed point), and pass P + 2 closes the item
+// from that mask.  Slot (P + 1) % 4 is cleared during pass P: its last reader
+// is pass P - 1 and its next writer pass P + 1.
+struct Close2D {
+    unsigned *hist;  // [4][nitems]
+    int *states;     // ST_INTS words per item
+    int nitems;
+    int *report;     // host-mapped [16]: bands queued for pass P, at P % 16 (or null)
 };
 
 namespace {
@@ -103,22 +119,18 @@ __global__ __launch_bounds__(256) void batch_unpack2d(const Item2D *__restrict__
     unpack_image_u8(buf, (uint32_t)it.L, it.w, it.h, f.out, f.out_pitch, f.uchunks, f.flat, t);
 }
 
-// zero states, tile flags and queue control words
-__global__ void batch_init(int *states, long nstate, uint8_t *occ, int *stamp, long ntiles,
-                           int *ctl, int nctl)
+// zero states, iteration masks, tile flags and queue control words
+__global__ void batch_init(int *states, long nstate, unsigned *hist, long nhist, uint8_t *occ,
+                           int *stamp, long ntiles, int *ctl, int nctl)
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < nstate) states[g] = 0;
+    if (g < nhist) hist[g] = 0u;
     if (g < ntiles) {
         occ[g] = 0;
         stamp[g] = 0;
     }
     if (g < nctl) ctl[g] = 0;
-}
-
-__global__ void batch_set_live(int *live2, int n2)
-{
-    if (threadIdx.x == 0) *live2 = n2;
 }
 
 // queue of pass 0 = every occupied tile
@@ -137,19 +149,58 @@ __device__ __forceinline__ void enqueue(const Queue &q, int u, int next_pass, in
     if (atomicMax(&q.stamp[u], next_pass) < next_pass) nlist[atomicAdd(ncnt, 1)] = u;
 }
 
+// close item i from its mask of pass P (the first pass whose mask misses an
+// iteration is the one in which it converged)
+__device__ __forceinline__ void close_item(const Close2D &c, int i, int K, int P)
+{
+    int *st = c.states + (size_t)i * ST_INTS;
+    if (st[ST_DONE]) return;
+    const int m = __popc(c.hist[(size_t)(P & 3) * c.nitems + i]);
+    if (m < K) {
+        st[ST_ITERS] = P * K + m + 1;  // counting the iteration that deleted nothing
+        st[ST_PARITY] = (P + 1) & 1;
+        st[ST_DONE] = 1;
+    }
+}
+
+// One pass P = (*pbase if pbase else 0) + p of the 2D batch over the queued
+// bands.  Workgroups 0 .. gridDim.x - 2 take bands; the last workgroup does
+// the pass's bookkeeping (closes the items of pass P - 2, clears the idle
+// queue counters and history slot) and takes no band.
 template <int R>
 __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restrict__ items,
                                                           const Tile2D *__restrict__ tiles,
-                                                          Queue q, int K, const int *gpass)
+                                                          Queue q, Close2D c, int K,
+                                                          const int *pbase, int p)
 {
     extern __shared__ uint64_t edges[];
     __shared__ TileInfo info;
     const int tid = threadIdx.x;
-    const int P = *gpass, cur = P & 1;
+    const int P = (pbase ? *pbase : 0) + p;
+    const int nwg = (int)gridDim.x - 1;
+    if ((int)blockIdx.x == nwg) {
+        if (P >= 2)
+            for (int i = tid; i < c.nitems; i += blockDim.x) close_item(c, i, K, P - 2);
+        unsigned *h = c.hist + (size_t)((P + 1) & 3) * c.nitems;
+        for (int i = tid; i < c.nitems; i += blockDim.x) h[i] = 0u;
+        if (tid == 0) {
+            q.cnt[(P + 2) % 3] = 0;
+            q.wctr[(P + 2) % 3] = 0;
+            // the host polls this instead of copying the count: a pass with no
+            // queued band means every item converged
+            if (c.report)
+                __hip_atomic_store(&c.report[P & 15], q.cnt[P % 3], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        return;
+    }
+    const int cur = P % 3;
     const int *list = q.list[cur];
-    int *nlist = q.list[cur ^ 1];
-    int *ncnt = &q.cnt[cur ^ 1];
+    int *nlist = q.list[(P + 1) % 3];
+    int *ncnt = &q.cnt[(P + 1) % 3];
     const int n = q.cnt[cur];
+    const unsigned *hprev = c.hist + (size_t)((P + 3) & 3) * c.nitems;  // pass P - 1
+    unsigned *hcur = c.hist + (size_t)(P & 3) * c.nitems;
     // Workgroup b takes band b first, and only the bands past the grid go
     // through the work counter: same-address atomics, one per grab, cost about
     // as much as a short pass when every workgroup makes one.  Workgroups past
@@ -158,8 +209,8 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
     for (int k = 0;; ++k) {
         int t = blockIdx.x;
         if (k > 0) {
-            if (n <= (int)gridDim.x) break;
-            if (tid == 0) info.t = (int)gridDim.x + atomicAdd(q.wctr, 1);
+            if (n <= nwg) break;
+            if (tid == 0) info.t = nwg + atomicAdd(&q.wctr[cur], 1);
             __syncthreads();
             t = info.t;
             __syncthreads();
@@ -167,14 +218,15 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         }
         const int tile = list[t];
         const Tile2D tl = tiles[tile];
+        // converged in pass P - 1: an iteration of that pass deleted nothing
+        if (P >= 1 && __popc(hprev[tl.item]) < K) continue;
         const Item2D it = items[tl.item];
-        if (it.st[ST_DONE]) continue;
         // a single-band item also runs at most K iterations per pass, so that no
         // pass waits on one item iterating to convergence while CUs idle
-        band_work<R>(it, tl.band, K, cur, edges, &info, K);
+        band_work<R>(it, tl.band, K, P & 1, edges, &info, K);
         if (tid == 0) {
             q.occ[tile] = info.occ != 0;
-            if (info.mask) atomicOr((unsigned *)&it.st[ST_MASK], info.mask);
+            if (info.mask) atomicOr(&hcur[tl.item], info.mask);
             // all K iterations deleted something: not converged, go on next pass
             if (it.nbands == 1 && info.mask == (1u << K) - 1u)
                 enqueue(q, tile, P + 1, nlist, ncnt, true);
@@ -187,19 +239,17 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
     }
 }
 
-// advance the items of one pipeline and reset its queues for the next pass
-__global__ void batch_finalize(int *states, int nitems, int K, int *live, int *gpass,
-                               const Queue *queues, int nq)
+// after the last pass Pend - 1: close the items of passes Pend - 2 and Pend - 1
+__global__ void batch_close2d(Close2D c, int K, int pend)
 {
-    const int P = *gpass;
-    for (int i = threadIdx.x; i < nitems; i += blockDim.x)
-        finalize_state(states + (size_t)i * ST_INTS, K, P, live);
-    for (int k = threadIdx.x; k < nq; k += blockDim.x) {
-        queues[k].cnt[P & 1] = 0;
-        *queues[k].wctr = 0;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) *gpass = P + 1;
+    for (int P = max(0, pend - 2); P < pend; ++P)
+        for (int i = threadIdx.x; i < c.nitems; i += blockDim.x) close_item(c, i, K, P);
+}
+
+// graph replays: the next chunk's passes start `n` later
+__global__ void batch_advance(int *pbase, int n)
+{
+    if (threadIdx.x == 0) *pbase += n;
 }
 
 struct Carver {
@@ -264,11 +314,11 @@ struct Batch {
     int nublk2 = 0;
     Tile2D *d_tiles2 = nullptr;
     Queue q2{};
-    Queue *d_q2 = nullptr;
-    int grid2 = 0;
+    Close2D close2{};
+    int grid2 = 0;          // band workgroups of a pass (one more does the bookkeeping)
     int *d_states2 = nullptr;
-    int *d_gpass2 = nullptr;
-    int *d_live2 = nullptr;
+    int *d_gpass2 = nullptr;  // pass number base of graph replays
+    int pass2 = 0;            // 2D passes queued in this run
     hipGraphExec_t graph2 = nullptr;
     hipStream_t graph2_stream = nullptr;
 
@@ -289,7 +339,8 @@ struct Batch {
     int *d_stamp = nullptr;
     long ntiles2 = 0;
 
-    int *h_live = nullptr;  // pinned, two slots of 32: [0] 2D live, [8..] 3D control words
+    int *h_live = nullptr;  // pinned, two slots of 32: [8..] 3D control words
+    int *h_rep = nullptr;   // pinned, device-mapped: bands queued per pass, by pass % 16
     char *h_stage = nullptr;   // pinned staging of the tables uploaded by prepare
     size_t h_stage_bytes = 0;
     int *h_states = nullptr;   // pinned copy of the item states after a run
@@ -300,9 +351,14 @@ struct Batch {
 
     // optional per-launch kernel timing (direct launches only)
     bool timing = false;
+    std::mutex mark_mu;  // the two chains' host threads share the timing events
     std::vector<hipEvent_t> events;
     size_t ev_used = 0;
-    std::vector<std::pair<size_t, int>> ev_marks;  // (first event index, kind 0=2D 1=3D)
+    struct Mark {
+        size_t begin, end;  // event indices
+        int kind;           // 0 = 2D pass, 1 = 3D iteration
+    };
+    std::vector<Mark> ev_marks;
     double ms2d = 0, ms3d = 0;
     int n2d = 0, n3d = 0;
 
@@ -317,6 +373,7 @@ struct Batch {
             if (ev_poll3[k]) (void)hipEventDestroy(ev_poll3[k]);
         }
         if (h_live) (void)hipHostFree(h_live);
+        if (h_rep) (void)hipHostFree(h_rep);
         if (h_stage) (void)hipHostFree(h_stage);
         if (h_states) (void)hipHostFree(h_states);
     }
@@ -411,9 +468,9 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
         b.d_tiles2 = cv.take<Tile2D>(sizeof(Tile2D) * t2.size());
         b.d_occ = cv.take<uint8_t>((size_t)b.ntiles2);
         b.d_stamp = cv.take<int>(sizeof(int) * b.ntiles2);
-        int *lists2 = cv.take<int>(sizeof(int) * 2 * b.ntiles2);
-        b.d_q2 = cv.take<Queue>(sizeof(Queue));
+        int *lists2 = cv.take<int>(sizeof(int) * 3 * b.ntiles2);
         b.d_states2 = cv.take<int>(sizeof(int) * (size_t)n2 * ST_INTS);
+        unsigned *hist2 = cv.take<unsigned>(sizeof(unsigned) * 4 * (size_t)n2);
         // 3D
         Sparse3D &e = b.e3;
         e = Sparse3D{};
@@ -452,17 +509,18 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
         e.dstamp = cv.take<int>(sizeof(int) * lay3.nblocks);
         e.states = cv.take<int>(sizeof(int) * (size_t)n3 * ST_INTS);
         e.changed = cv.take<int>(sizeof(int) * n3);
-        // 2D control words: [0,1] queue counts, [2] next slot, [3] pass, [4] live
+        // 2D control words: [0..2] queue counts, [3..5] next slots, [6] pass base
         int *ctl = b.d_ctl;
-        b.q2.list[0] = lists2;
-        b.q2.list[1] = lists2 ? lists2 + b.ntiles2 : nullptr;
+        for (int k = 0; k < 3; ++k) b.q2.list[k] = lists2 ? lists2 + k * b.ntiles2 : nullptr;
         b.q2.cnt = ctl;
-        b.q2.wctr = ctl ? ctl + 2 : nullptr;
+        b.q2.wctr = ctl ? ctl + 3 : nullptr;
         b.q2.stamp = b.d_stamp;
         b.q2.occ = b.d_occ;
         b.q2.ntiles = (int)b.ntiles2;
-        b.d_gpass2 = ctl ? ctl + 3 : nullptr;
-        b.d_live2 = ctl ? ctl + 4 : nullptr;
+        b.d_gpass2 = ctl ? ctl + 6 : nullptr;
+        b.close2.hist = hist2;
+        b.close2.states = b.d_states2;
+        b.close2.nitems = n2;
 
         std::vector<Item2D> h2(n2);
         for (int i = 0; i < n2; ++i) {
@@ -508,7 +566,6 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
             stage(b.d_blk2, blk2.data(), sizeof(int) * blk2.size());
             stage(b.d_ublk2, ublk2.data(), sizeof(int) * ublk2.size());
             stage(b.d_tiles2, t2.data(), sizeof(Tile2D) * t2.size());
-            stage(b.d_q2, &b.q2, sizeof(Queue));
         }
         if (n3) {
             stage(d_items3, lay3.items.data(), sizeof(SItem3D) * n3);
@@ -535,6 +592,16 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
                                  b.c2.waves * 64, lds2, (int)b.ntiles2, b.c2.blocks_per_cu);
     if (!b.h_live && hipHostMalloc((void **)&b.h_live, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(BGT_ERR_NOMEM, "pinned allocation failed");
+    if (!b.h_rep) {
+        if (hipHostMalloc((void **)&b.h_rep, 16 * sizeof(int),
+                          hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+            b.h_rep = nullptr;
+            return fail(BGT_ERR_NOMEM, "pinned allocation failed");
+        }
+    }
+    void *rep_dev = nullptr;
+    BGT_HIP(hipHostGetDevicePointer(&rep_dev, b.h_rep, 0));
+    b.close2.report = (int *)rep_dev;
     if (!b.ev_fork) BGT_HIP(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
     if (!b.ev_join) BGT_HIP(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
     for (int k = 0; k < 2; ++k) {
@@ -545,9 +612,12 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     return BGT_OK;
 }
 
-static int timed_mark(Batch &b, hipStream_t s, int kind, bool begin)
+// Records the event opening (begin) or closing a timed launch on stream s;
+// `open` carries the opening event's index from the one call to the other.
+static int timed_mark(Batch &b, hipStream_t s, int kind, bool begin, size_t *open)
 {
     if (!b.timing) return BGT_OK;
+    std::lock_guard<std::mutex> lock(b.mark_mu);
     if (b.ev_used == b.events.size()) {
         for (int i = 0; i < 64; ++i) {
             hipEvent_t e;
@@ -555,28 +625,30 @@ static int timed_mark(Batch &b, hipStream_t s, int kind, bool begin)
             b.events.push_back(e);
         }
     }
-    if (begin) b.ev_marks.push_back({b.ev_used, kind});
-    BGT_HIP(hipEventRecord(b.events[b.ev_used++], s));
+    const size_t k = b.ev_used++;
+    BGT_HIP(hipEventRecord(b.events[k], s));
+    if (begin)
+        *open = k;
+    else
+        b.ev_marks.push_back({*open, k, kind});
     return BGT_OK;
 }
 
-// one 2D pass: queue kernel + finalize
-static int launch_pass2d(Batch &b, hipStream_t s)
+// one 2D pass, number (pbase ? *pbase : 0) + p: one launch, bookkeeping included
+static int launch_pass2d(Batch &b, hipStream_t s, const int *pbase, int p)
 {
     const size_t lds = (size_t)2 * 2 * b.c2.waves * 64 * sizeof(uint64_t);
-    const dim3 g(b.grid2), blk(b.c2.waves * 64);
-    BGT_CHECK(timed_mark(b, s, 0, true));
+    const dim3 g(b.grid2 + 1), blk(b.c2.waves * 64);
+    size_t mk = 0;
+    BGT_CHECK(timed_mark(b, s, 0, true, &mk));
     if (b.c2.rows == 8)
         hipLaunchKernelGGL(thin2d_queue_pass<8>, g, blk, lds, s, b.d_items2, b.d_tiles2, b.q2,
-                           b.c2.iters_per_pass, b.d_gpass2);
+                           b.close2, b.c2.iters_per_pass, pbase, p);
     else
         hipLaunchKernelGGL(thin2d_queue_pass<16>, g, blk, lds, s, b.d_items2, b.d_tiles2, b.q2,
-                           b.c2.iters_per_pass, b.d_gpass2);
+                           b.close2, b.c2.iters_per_pass, pbase, p);
     BGT_HIP(hipGetLastError());
-    BGT_CHECK(timed_mark(b, s, 0, false));
-    hipLaunchKernelGGL(batch_finalize, dim3(1), dim3(256), 0, s, b.d_states2, (int)b.in2.size(),
-                       b.c2.iters_per_pass, b.d_live2, b.d_gpass2, b.d_q2, 1);
-    BGT_HIP(hipGetLastError());
+    BGT_CHECK(timed_mark(b, s, 0, false, &mk));
     return BGT_OK;
 }
 
@@ -584,9 +656,10 @@ static int launch_pass2d(Batch &b, hipStream_t s)
 // the list and active list counts
 static int launch_iter3d(Batch &b, int cur, int bound, int abound, hipStream_t s)
 {
-    BGT_CHECK(timed_mark(b, s, 1, true));
+    size_t mk = 0;
+    BGT_CHECK(timed_mark(b, s, 1, true, &mk));
     BGT_CHECK(sparse3d_iteration(b.e3, cur, bound, abound, sparse3d_grid(b.c3.blocks_per_cu), s));
-    BGT_CHECK(timed_mark(b, s, 1, false));
+    BGT_CHECK(timed_mark(b, s, 1, false, &mk));
     return BGT_OK;
 }
 
@@ -623,11 +696,23 @@ static int launch_chunk(Batch &b, bool three_d, bool use_graph, hipStream_t s)
 {
     const int bound3 = bound_blocks3(b.bound3) * 256, abound3 = bound_blocks3(b.abound3) * 256;
     const long key3 = ((long)bound3 << 32) | (long)abound3;
+    const int p0 = b.pass2;
     auto body = [&](hipStream_t st) -> int {
-        for (int i = 0; i < kChunk; ++i)
-            BGT_CHECK(three_d ? launch_iter3d(b, i & 1, bound3, abound3, st) : launch_pass2d(b, st));
+        for (int i = 0; i < kChunk; ++i) {
+            if (three_d)
+                BGT_CHECK(launch_iter3d(b, i & 1, bound3, abound3, st));
+            else if (use_graph)  // pass numbers from the device, advanced per replay
+                BGT_CHECK(launch_pass2d(b, st, b.d_gpass2, i));
+            else
+                BGT_CHECK(launch_pass2d(b, st, nullptr, p0 + i));
+        }
+        if (!three_d && use_graph) {
+            hipLaunchKernelGGL(batch_advance, dim3(1), dim3(64), 0, st, b.d_gpass2, kChunk);
+            BGT_HIP(hipGetLastError());
+        }
         return BGT_OK;
     };
+    if (!three_d) b.pass2 += kChunk;
     if (!use_graph) return body(s);
     if (!three_d) {
         if (!b.graph2 || b.graph2_stream != s) {
@@ -660,8 +745,8 @@ static int collect_timing(Batch &b)
     b.n2d = b.n3d = 0;
     for (auto &m : b.ev_marks) {
         float ms = 0;
-        BGT_HIP(hipEventElapsedTime(&ms, b.events[m.first], b.events[m.first + 1]));
-        if (m.second == 0) { b.ms2d += ms; b.n2d++; }
+        BGT_HIP(hipEventElapsedTime(&ms, b.events[m.begin], b.events[m.end]));
+        if (m.kind == 0) { b.ms2d += ms; b.n2d++; }
         else { b.ms3d += ms; b.n3d++; }
     }
     b.ev_marks.clear();
@@ -678,12 +763,25 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     b.ev_used = 0;
 
     // set-up on the caller's stream
-    const long init_n = std::max<long>({(long)n2 * ST_INTS, (long)n3 * ST_INTS, b.ntiles2,
-                                        (long)b.nctl, (long)n3, 1L});
+    const long init_n = std::max<long>({(long)n2 * ST_INTS, (long)n3 * ST_INTS, 4L * n2,
+                                        b.ntiles2, (long)b.nctl, (long)n3, 1L});
     hipLaunchKernelGGL(batch_init, dim3((unsigned)((init_n + 255) / 256)), dim3(256), 0, s,
-                       b.d_states2, (long)n2 * ST_INTS, b.d_occ, b.d_stamp, b.ntiles2, b.d_ctl,
-                       b.nctl);
-    hipLaunchKernelGGL(batch_set_live, dim3(1), dim3(64), 0, s, b.d_live2, n2);
+                       b.d_states2, (long)n2 * ST_INTS, b.close2.hist, 4L * n2, b.d_occ,
+                       b.d_stamp, b.ntiles2, b.d_ctl, b.nctl);
+    b.pass2 = 0;
+    BGT_HIP(hipGetLastError());
+
+    // With both kinds of items, the 3D pipeline (its pack included) runs on a
+    // side stream forked from and joined into `s`; a batch of one kind uses
+    // `s` alone (no extra stream, so a caller with one stream per worker keeps
+    // one hardware queue per worker).
+    const bool fork = n2 > 0 && n3 > 0;
+    if (fork && !b.side) BGT_HIP(hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking));
+    hipStream_t s3 = fork ? b.side : s;
+    if (fork) {
+        BGT_HIP(hipEventRecord(b.ev_fork, s));
+        BGT_HIP(hipStreamWaitEvent(b.side, b.ev_fork, 0));
+    }
     if (b.nblk2)
         hipLaunchKernelGGL(batch_pack2d, dim3(b.nblk2), dim3(256), 0, s, b.d_items2, b.d_io2,
                            b.d_blk2, b.d_occ);
@@ -692,37 +790,24 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
                            0, s, b.q2);
     BGT_HIP(hipGetLastError());
     if (n3) {
-        BGT_CHECK(sparse3d_begin(b.e3, !b.zeroed3, s));
+        BGT_CHECK(sparse3d_begin(b.e3, !b.zeroed3, s3));
         b.zeroed3 = true;
         b.bound3 = b.abound3 = b.e3.capacity;
         b.iter3 = 0;
-    }
-
-    // With both kinds of items, the 3D pipeline runs on a side stream forked
-    // from and joined into `s`; a batch of one kind uses `s` alone (no extra
-    // stream, so a caller with one stream per worker keeps one hardware queue
-    // per worker).
-    const bool fork = n2 > 0 && n3 > 0;
-    if (fork && !b.side) BGT_HIP(hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking));
-    hipStream_t s3 = fork ? b.side : s;
-    if (fork) {
-        BGT_HIP(hipEventRecord(b.ev_fork, s));
-        BGT_HIP(hipStreamWaitEvent(b.side, b.ev_fork, 0));
     }
     long max_vox = 0;
     for (auto &v : b.in2) max_vox = std::max<long>(max_vox, (long)v.w * v.h);
     for (auto &v : b.in3) max_vox = std::max<long>(max_vox, (long)v.w * v.h * v.d);
     const long max_chunks = max_vox / kChunk + 3;
     // Each chain (2D passes on `s`, 3D iterations on `s3`) keeps two chunks in
-    // flight: chunk k+1 is queued before the counters copied after chunk k are
-    // read, so the GPU does not wait for the host.  The chains are polled
-    // without blocking and refilled independently, so a slow chunk of one
-    // never holds back the queueing of the other.  A chunk queued after the
-    // last item converged finds empty queues and lists.
+    // flight: chunk k+1 is queued before the poll after chunk k is read, so
+    // the GPU does not wait for the host.  A chunk queued after the last item
+    // converged finds empty queues and lists.
     struct Chain {
         bool live = false;
         bool final = false;          // its last launch is queued (the 3D tail)
         int head = 0, inflight = 0;  // oldest pending poll slot, pending polls
+        int last[2] = {0, 0};        // per poll slot: the chunk's last 2D pass
         long chunks = 0;
     } c2, c3;
     c2.live = n2 > 0;
@@ -732,14 +817,17 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         int *hs = b.h_live + slot * 32;
         if (!three_d) {
             BGT_CHECK(launch_chunk(b, false, use_graph, s));
-            BGT_HIP(hipMemcpyAsync(hs, b.d_live2, sizeof(int), hipMemcpyDeviceToHost, s));
+            c.last[slot] = b.pass2 - 1;
             BGT_HIP(hipEventRecord(b.ev_poll2[slot], s));
         } else {
-            if (b.c3.tail_words > 0 && b.iter3 >= 1 && b.abound3 <= b.c3.tail_words) {
-                // few active words left: the remaining iterations in one launch
-                BGT_CHECK(timed_mark(b, s3, 1, true));
+            if (b.c3.tail_words > 0 && b.iter3 >= 1 &&
+                (long)b.abound3 <= (long)b.c3.tail_words * n3) {
+                // few active words per volume left: the remaining iterations in
+                // one launch, one workgroup per volume
+                size_t mk = 0;
+                BGT_CHECK(timed_mark(b, s3, 1, true, &mk));
                 BGT_CHECK(sparse3d_tail(b.e3, s3));
-                BGT_CHECK(timed_mark(b, s3, 1, false));
+                BGT_CHECK(timed_mark(b, s3, 1, false, &mk));
                 c.final = true;
             } else {
                 BGT_CHECK(launch_chunk(b, true, use_graph, s3));
@@ -756,7 +844,11 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     auto consume = [&](Chain &c, bool three_d) {
         const int *hp = b.h_live + c.head * 32;
         if (!three_d) {
-            if (hp[0] <= 0) c.live = false;
+            // the chunk's last pass found no queued band: every item converged
+            // (a later pass may have overwritten the slot: once a pass finds no
+            // band, so does every later one, so its value gives the same answer)
+            const int q = __atomic_load_n(&b.h_rep[c.last[c.head] & 15], __ATOMIC_ACQUIRE);
+            if (q <= 0) c.live = false;
         } else {
             // chunks hold an even number of iterations: list 0 is current
             b.bound3 = hp[8 + SC_COUNT0];
@@ -767,61 +859,71 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         c.head ^= 1;
         --c.inflight;
     };
-    // *done = the chain's oldest pending poll has landed
-    auto ready = [&](const Chain &c, bool three_d, bool block, bool *done) -> int {
-        hipEvent_t e = three_d ? b.ev_poll3[c.head] : b.ev_poll2[c.head];
-        *done = true;
-        if (block) {
-            BGT_HIP(hipEventSynchronize(e));
-            return BGT_OK;
-        }
-        const hipError_t q = hipEventQuery(e);
-        if (q == hipSuccess) return BGT_OK;
-        if (q != hipErrorNotReady) return fail(BGT_ERR_HIP, "poll: %s", hipGetErrorString(q));
-        (void)hipGetLastError();  // "not ready" is not an error of a later call
-        *done = false;
+    // blocks until the chain's oldest pending poll has landed
+    auto wait_poll = [&](const Chain &c, bool three_d) -> int {
+        BGT_HIP(hipEventSynchronize(three_d ? b.ev_poll3[c.head] : b.ev_poll2[c.head]));
         return BGT_OK;
     };
-    while (c2.live || c3.live) {
-        if (c2.live && c2.inflight < 2) {
-            BGT_CHECK(enqueue(c2, false));
-            continue;
-        }
-        if (c3.live && c3.inflight < 2 && !c3.final) {
-            BGT_CHECK(enqueue(c3, true));
-            continue;
-        }
-        if (c3.live && c3.final && c3.inflight == 0)
-            return fail(BGT_ERR_NOT_CONVERGED, "3D tail did not converge");
-        // both live chains have two chunks queued: take whichever poll lands
-        // first (blocking when only one chain is left)
-        const bool both = c2.live && c3.live;
-        for (;;) {
-            bool done = false;
-            if (c2.live) {
-                BGT_CHECK(ready(c2, false, !both, &done));
-                if (done) {
-                    consume(c2, false);
-                    break;
-                }
+    // With both kinds of items each chain has a host thread of its own:
+    // queueing a 3D chunk is ~32 launches, ~150 us of host time, long enough to
+    // drain the two short tail chunks queued on the 2D chain if one thread fed
+    // both.
+    auto drive = [&](Chain &c, bool three_d) -> int {
+        while (c.live) {
+            if (c.inflight < 2 && !c.final) {
+                BGT_CHECK(enqueue(c, three_d));
+                continue;
             }
-            if (c3.live) {
-                BGT_CHECK(ready(c3, true, !both, &done));
-                if (done) {
-                    consume(c3, true);
-                    break;
-                }
-            }
-            std::this_thread::yield();
+            if (c.final && c.inflight == 0)
+                return fail(BGT_ERR_NOT_CONVERGED, "3D tail did not converge");
+            BGT_CHECK(wait_poll(c, three_d));
+            consume(c, three_d);
         }
+        return BGT_OK;
+    };
+    // each chain's closing launches follow its last chunk on its own stream,
+    // so the 2D unpack overlaps the end of the 3D chain
+    auto run2 = [&]() -> int {
+        BGT_CHECK(drive(c2, false));
+        if (!n2) return BGT_OK;
+        hipLaunchKernelGGL(batch_close2d, dim3(1), dim3(256), 0, s, b.close2,
+                           b.c2.iters_per_pass, b.pass2);
+        if (b.nublk2)
+            hipLaunchKernelGGL(batch_unpack2d, dim3(b.nublk2), dim3(256), 0, s, b.d_items2,
+                               b.d_io2, b.d_ublk2);
+        BGT_HIP(hipGetLastError());
+        return BGT_OK;
+    };
+    auto run3 = [&]() -> int {
+        BGT_CHECK(drive(c3, true));
+        if (!n3) return BGT_OK;
+        BGT_CHECK(sparse3d_end(b.e3, s3));
+        if (fork) BGT_HIP(hipEventRecord(b.ev_join, s3));
+        return BGT_OK;
+    };
+    if (fork) {
+        int dev = 0;
+        BGT_HIP(hipGetDevice(&dev));
+        int rc3 = BGT_OK;
+        std::string err3;
+        std::thread t3([&] {
+            const hipError_t e = hipSetDevice(dev);
+            rc3 = e != hipSuccess ? fail(BGT_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e))
+                                  : run3();
+            if (rc3 != BGT_OK) err3 = last_error_message();
+        });
+        const int rc2 = run2();
+        t3.join();
+        BGT_CHECK(rc2);
+        if (rc3 != BGT_OK) {
+            set_error(rc3, err3);
+            return rc3;
+        }
+        BGT_HIP(hipStreamWaitEvent(s, b.ev_join, 0));
+    } else {
+        BGT_CHECK(run2());  // at most one of the two has items
+        BGT_CHECK(run3());
     }
-    if (n3) BGT_CHECK(sparse3d_end(b.e3, s3));
-    if (fork) BGT_HIP(hipEventRecord(b.ev_join, s3));
-    if (b.nublk2)
-        hipLaunchKernelGGL(batch_unpack2d, dim3(b.nublk2), dim3(256), 0, s, b.d_items2, b.d_io2,
-                           b.d_ublk2);
-    BGT_HIP(hipGetLastError());
-    if (fork) BGT_HIP(hipStreamWaitEvent(s, b.ev_join, 0));
     const size_t nst = (size_t)(n2 + n3) * ST_INTS;
     if (nst > b.h_states_ints) {
         if (b.h_states) (void)hipHostFree(b.h_states);

@@ -27,6 +27,8 @@ void set_error(int code, const std::string &msg)
     g_error = msg;
 }
 
+std::string last_error_message() { return g_error; }
+
 int fail(int code, const char *fmt, ...)
 {
     char buf[512];

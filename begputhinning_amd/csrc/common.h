@@ -13,6 +13,7 @@ namespace bgt {
 
 // ---- errors ----------------------------------------------------------------
 void set_error(int code, const std::string &msg);
+std::string last_error_message();  // the calling thread's last message
 int fail(int code, const char *fmt, ...);
 
 #define BGT_HIP(expr)                                                                \

@@ -12,9 +12,10 @@ namespace bgt {
 struct Config3D {
     int blocks_per_cu = 0;  // step-kernel workgroups per CU; 0 = occupancy limit
     int max_chunk = 16;     // most iterations queued between two host polls
-    // run the remaining iterations in one launch (sparse3d_tail) once at
-    // most this many words are active (0 = never)
-    int tail_words = 8192;
+    // run the remaining iterations in one launch (sparse3d_tail, one
+    // workgroup per volume) once at most this many words per volume are
+    // active on average (0 = never)
+    int tail_words = 2048;
 };
 
 int check_config3d(const Config3D &c);
