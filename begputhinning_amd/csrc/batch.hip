@@ -76,6 +76,7 @@ struct Close2D {
     int *states;     // ST_INTS words per item
     int nitems;
     int *report;     // host-mapped [16]: bands queued for pass P, at P % 16 (or null)
+    int ncu;         // compute units of the device
 };
 
 namespace {
@@ -163,6 +164,9 @@ __device__ __forceinline__ void close_item(const Close2D &c, int i, int K, int P
     }
 }
 
+// tail passes: the most workgroups a band is split over
+constexpr int kParts = 16;
+
 // One pass P = (*pbase if pbase else 0) + p of the 2D batch over the queued
 // bands.  Workgroups 0 .. gridDim.x - 2 take bands; the last workgroup does
 // the pass's bookkeeping (closes the items of pass P - 2, clears the idle
@@ -201,6 +205,51 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
     const int n = q.cnt[cur];
     const unsigned *hprev = c.hist + (size_t)((P + 3) & 3) * c.nitems;  // pass P - 1
     unsigned *hcur = c.hist + (size_t)(P & 3) * c.nitems;
+    // Fewer bands than half the CUs (the tail of a batch): each band is split
+    // over kp workgroups, at most one per CU, whose windows hold 4 or 8
+    // register rows per segment.  A pass's latency is bound by the rows one
+    // wave works through per subiteration, so it drops while the CUs would
+    // otherwise idle; with fewer idle CUs the parts' halo rows cost more than
+    // they save.  Each part carries full halos, so the band's result is the
+    // same.
+    const int kp = n > 0 ? min(min(kParts, c.ncu / n), nwg / n) : 0;
+    if (kp >= 2) {
+        const int e = blockIdx.x / kp, part = blockIdx.x % kp;
+        if (e >= n) return;
+        const int tile = list[e];
+        const Tile2D tl = tiles[tile];
+        if (P >= 1 && __popc(hprev[tl.item]) < K) return;  // converged in pass P - 1
+        const Item2D it = items[tl.item];
+        const int row0 = tl.band * it.band_rows;
+        const int seg = (int)(blockDim.x >> 6) * (64 / it.L);  // segments per window
+        const int in4 = seg * 4 - 8 * K, in8 = seg * 8 - 8 * K;  // interior rows of a part
+        const int np4 = in4 >= 1 ? (it.band_rows + in4 - 1) / in4 : kParts + 1;
+        const int np8 = in8 >= 1 ? (it.band_rows + in8 - 1) / in8 : kParts + 1;
+        if (np4 <= kp) {
+            if (part >= np4 || row0 + part * in4 >= it.h) return;
+            band_work<4>(it, row0 + part * in4, min(in4, it.band_rows - part * in4), false, K,
+                         P & 1, edges, &info);
+        } else if (R > 8 && np8 <= kp) {
+            if (part >= np8 || row0 + part * in8 >= it.h) return;
+            band_work<8>(it, row0 + part * in8, min(in8, it.band_rows - part * in8), false, K,
+                         P & 1, edges, &info);
+        } else {  // windows too short for parts: one workgroup per band
+            if (part != 0) return;
+            band_work<R>(it, row0, it.band_rows, false, K, P & 1, edges, &info);
+        }
+        if (tid == 0) {
+            // parts only set the flag: a band emptied in a tail pass stays
+            // marked, which can only cost a pass over an empty band
+            if (info.occ) q.occ[tile] = 1;
+            if (info.mask) atomicOr(&hcur[tl.item], info.mask);
+        }
+        // a part that changed queues its band and the band's neighbours
+        if (info.mask && tid < 3) {
+            const int b = tl.band - 1 + tid;
+            if (b >= 0 && b < it.nbands) enqueue(q, it.tile0 + b, P + 1, nlist, ncnt, tid == 1);
+        }
+        return;
+    }
     // Workgroup b takes band b first, and only the bands past the grid go
     // through the work counter: same-address atomics, one per grab, cost about
     // as much as a short pass when every workgroup makes one.  Workgroups past
@@ -223,7 +272,8 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         const Item2D it = items[tl.item];
         // a single-band item also runs at most K iterations per pass, so that no
         // pass waits on one item iterating to convergence while CUs idle
-        band_work<R>(it, tl.band, K, P & 1, edges, &info, K);
+        band_work<R>(it, tl.band * it.band_rows, it.band_rows, it.nbands == 1, K, P & 1, edges,
+                     &info, K);
         if (tid == 0) {
             q.occ[tile] = info.occ != 0;
             if (info.mask) atomicOr(&hcur[tl.item], info.mask);
@@ -589,7 +639,9 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     if (n2)
         b.grid2 = occupancy_grid(b.c2.rows == 8 ? (const void *)thin2d_queue_pass<8>
                                                 : (const void *)thin2d_queue_pass<16>,
-                                 b.c2.waves * 64, lds2, (int)b.ntiles2, b.c2.blocks_per_cu);
+                                 b.c2.waves * 64, lds2,
+                                 (int)std::min<long>(b.ntiles2 * kParts, 1L << 30),
+                                 b.c2.blocks_per_cu);
     if (!b.h_live && hipHostMalloc((void **)&b.h_live, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(BGT_ERR_NOMEM, "pinned allocation failed");
     if (!b.h_rep) {
@@ -602,6 +654,12 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     void *rep_dev = nullptr;
     BGT_HIP(hipHostGetDevicePointer(&rep_dev, b.h_rep, 0));
     b.close2.report = (int *)rep_dev;
+    {
+        int dev = 0, cus = 0;
+        BGT_HIP(hipGetDevice(&dev));
+        BGT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        b.close2.ncu = std::max(1, cus);
+    }
     if (!b.ev_fork) BGT_HIP(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
     if (!b.ev_join) BGT_HIP(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
     for (int k = 0; k < 2; ++k) {

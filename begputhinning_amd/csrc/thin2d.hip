@@ -163,7 +163,8 @@ __global__ __launch_bounds__(512) void thin2d_pass(const Item2D *__restrict__ it
         }
     }
     if (work) {
-        const int run = band_work<R>(it, band, K, par, edges, &info);
+        const int run = band_work<R>(it, band * it.band_rows, it.band_rows, it.nbands == 1, K,
+                                     par, edges, &info);
         if (tid == 0) {
             it.chg[par * it.nbands + band] = info.mask != 0;
             if (info.mask) atomicOr((unsigned *)&it.st[ST_MASK], info.mask);

@@ -195,15 +195,17 @@ struct TileInfo {
     int t;           // work-queue slot
 };
 
-// Run one pass over band `band` of item `it`: K iterations (4K subiterations)
-// from buffer `par` into buffer par^1, or -- when the band is the item's only
-// band, so that its halo rows are all outside the image -- iterations until one
-// deletes nothing, at most whole_max of them.  Results land in `info` (bit j
-// of the mask: iteration j deleted an interior pixel); returns the number of
-// iterations that deleted something in the single-band case (else 0).
+// Run one pass over the interior rows [row0, row0 + nrows) of item `it` (a
+// band, or a part of one): K iterations (4K subiterations) from buffer `par`
+// into buffer par^1, or -- when `whole`: the rows are the whole image, so that
+// the halo rows are all outside it -- iterations until one deletes nothing, at
+// most whole_max of them.  The window of the launch (waves x G segments x R
+// rows) must hold nrows + 8K rows.  Results land in `info` (bit j of the mask:
+// iteration j deleted an interior pixel); returns the number of iterations
+// that deleted something in the whole case (else 0).
 template <int R>
-__device__ int band_work(const Item2D &it, int band, int K, int par, uint64_t *edges,
-                         TileInfo *info, int whole_max = 0x7fffffff)
+__device__ int band_work(const Item2D &it, int row0, int nrows, bool whole, int K, int par,
+                         uint64_t *edges, TileInfo *info, int whole_max = 0x7fffffff)
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int NW = blockDim.x >> 6;
@@ -214,15 +216,14 @@ __device__ int band_work(const Item2D &it, int band, int K, int par, uint64_t *e
     const int g = lane / L, c = lane - g * L, q = wv * G + g;
     const bool active = g < G;
     const int H = 4 * K;
-    const bool whole = it.nbands == 1;  // halo rows all outside the image
-    const int y_reg0 = band * it.band_rows - H + q * R;  // image row of register row 0
+    const int y_reg0 = row0 - H + q * R;  // image row of register row 0
     uint32_t mw = c == 0 ? 0u : 0xffffffffu;
     uint32_t me = c == L - 1 ? 0u : 0xffffffffu;
     // opaque to the compiler, so `dpp(v) & mask` stays a VGPR AND that folds
     // into the DPP instruction instead of becoming a lane-mask select
     asm volatile("" : "+v"(mw), "+v"(me));
     // rows of this segment inside the band interior, as a bit mask
-    const int ra = max(0, H - q * R), rb = min(R, H + it.band_rows - q * R);
+    const int ra = max(0, H - q * R), rb = min(R, H + nrows - q * R);
     const uint32_t interior =
         (active && rb > ra) ? (uint32_t)(((1ull << rb) - 1ull) & ~((1ull << ra) - 1ull)) : 0u;
     const uint64_t *src = par ? it.buf[1] : it.buf[0];

@@ -104,6 +104,52 @@ def test_batch_timing_and_single_item_equality():
     b.close()
 
 
+def split_items():
+    """Images whose tails leave a few bands queued (a thick disc, thick lines),
+    over every segment count per wave from 1 (width > 2048) to 64 (width <= 64),
+    single-band and multi-band."""
+    from begputhinning_amd import synth
+    items = []
+    for k, (w, h) in enumerate([(60, 700), (200, 90), (700, 650), (1100, 420), (1400, 500),
+                                (2500, 300), (1500, 40)]):
+        yy, xx = np.mgrid[0:h, 0:w]
+        r = min(w, h) * 0.4
+        disc = ((xx - w * 0.55) ** 2 + (yy - h * 0.45) ** 2 < r * r).astype(np.uint8)
+        lines = synth.polylines(w, h, 50 + k, n_lines=6)
+        items.append(np.maximum(disc, lines))
+    return items
+
+
+@pytest.mark.parametrize("cfg", [(4, 16, 3), (4, 8, 2), (8, 16, 6), (1, 16, 1), (2, 8, 1),
+                                 (4, 16, 1), (8, 8, 4), (3, 16, 2)])
+def test_split_band_tail(cfg):
+    """Once few bands are queued, a band is split over several workgroups with
+    short windows (or kept whole where the window is too short for parts):
+    every launch shape gives the oracle's skeletons and iteration counts."""
+    import begputhinning_amd as B
+    from begputhinning_amd.batch import ThinningBatch
+    items = split_items()
+    print(f"[split] cfg={cfg}", flush=True)
+    ins = [torch.from_numpy(a).cuda() for a in items]
+    outs = [torch.full_like(t, 9) for t in ins]
+    B.set_config2d(*cfg)
+    try:
+        b = ThinningBatch()
+        for t, o in zip(ins, outs):
+            b.add2d(t, o)
+        for rep in range(2):
+            b.run(use_graph=rep == 1)
+            torch.cuda.synchronize()
+            its = b.iterations()
+            for k, (a, o) in enumerate(zip(items, outs)):
+                ref, rit = O.thin2d(a)
+                assert np.array_equal(o.cpu().numpy(), ref), (cfg, rep, k, a.shape)
+                assert its[k] == rit, (cfg, rep, k, its[k], rit)
+        b.close()
+    finally:
+        B.set_config2d()
+
+
 @pytest.mark.parametrize("blocks_per_cu", [1, 3, 0])
 def test_pass_grid_sizes(blocks_per_cu):
     """The batch pass kernel pulls bands from a queue: any number of
