@@ -75,7 +75,9 @@ struct Close2D {
     unsigned *hist;  // [4][nitems]
     int *states;     // ST_INTS words per item
     int nitems;
-    int *report;     // host-mapped [16]: bands queued for pass P, at P % 16 (or null)
+    // host-mapped [16] (or null): at P % 16, (P + 1) << 32 | bands queued for
+    // pass P, stored when pass P starts
+    unsigned long long *report;
     int ncu;         // compute units of the device
 };
 
@@ -190,11 +192,12 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         if (tid == 0) {
             q.cnt[(P + 2) % 3] = 0;
             q.wctr[(P + 2) % 3] = 0;
-            // the host polls this instead of copying the count: a pass with no
-            // queued band means every item converged
+            // the host follows the passes through these: a pass with no queued
+            // band means every item converged
             if (c.report)
-                __hip_atomic_store(&c.report[P & 15], q.cnt[P % 3], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(&c.report[P & 15],
+                                   ((unsigned long long)(P + 1) << 32) | (unsigned)q.cnt[P % 3],
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         return;
     }
@@ -390,12 +393,12 @@ struct Batch {
     long ntiles2 = 0;
 
     int *h_live = nullptr;  // pinned, two slots of 32: [8..] 3D control words
-    int *h_rep = nullptr;   // pinned, device-mapped: bands queued per pass, by pass % 16
+    unsigned long long *h_rep = nullptr;  // pinned, device-mapped pass reports (Close2D)
     char *h_stage = nullptr;   // pinned staging of the tables uploaded by prepare
     size_t h_stage_bytes = 0;
     int *h_states = nullptr;   // pinned copy of the item states after a run
     size_t h_states_ints = 0;
-    hipEvent_t ev_poll2[2] = {nullptr, nullptr}, ev_poll3[2] = {nullptr, nullptr};
+    hipEvent_t ev_poll3[2] = {nullptr, nullptr};
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
@@ -419,7 +422,6 @@ struct Batch {
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         for (int k = 0; k < 2; ++k) {
-            if (ev_poll2[k]) (void)hipEventDestroy(ev_poll2[k]);
             if (ev_poll3[k]) (void)hipEventDestroy(ev_poll3[k]);
         }
         if (h_live) (void)hipHostFree(h_live);
@@ -645,7 +647,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     if (!b.h_live && hipHostMalloc((void **)&b.h_live, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(BGT_ERR_NOMEM, "pinned allocation failed");
     if (!b.h_rep) {
-        if (hipHostMalloc((void **)&b.h_rep, 16 * sizeof(int),
+        if (hipHostMalloc((void **)&b.h_rep, 16 * sizeof(unsigned long long),
                           hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
             b.h_rep = nullptr;
             return fail(BGT_ERR_NOMEM, "pinned allocation failed");
@@ -653,7 +655,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     }
     void *rep_dev = nullptr;
     BGT_HIP(hipHostGetDevicePointer(&rep_dev, b.h_rep, 0));
-    b.close2.report = (int *)rep_dev;
+    b.close2.report = (unsigned long long *)rep_dev;
     {
         int dev = 0, cus = 0;
         BGT_HIP(hipGetDevice(&dev));
@@ -663,7 +665,6 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     if (!b.ev_fork) BGT_HIP(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
     if (!b.ev_join) BGT_HIP(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
     for (int k = 0; k < 2; ++k) {
-        if (!b.ev_poll2[k]) BGT_HIP(hipEventCreateWithFlags(&b.ev_poll2[k], hipEventDisableTiming));
         if (!b.ev_poll3[k]) BGT_HIP(hipEventCreateWithFlags(&b.ev_poll3[k], hipEventDisableTiming));
     }
     b.ready = true;
@@ -724,6 +725,8 @@ static int launch_iter3d(Batch &b, int cur, int bound, int abound, hipStream_t s
 // A chunk is an even number of passes / iterations so that it starts and ends
 // on the same queue parity and can be captured once and replayed.
 static const int kChunk = 4;
+// 2D passes queued ahead of the oldest pass that has not reported yet
+static const int kAhead2 = 4;
 
 // 3D launch sizes follow the list-count bound rounded up to a power of two
 // of workgroups, so that a few captured graphs cover a whole run.
@@ -856,94 +859,108 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     long max_vox = 0;
     for (auto &v : b.in2) max_vox = std::max<long>(max_vox, (long)v.w * v.h);
     for (auto &v : b.in3) max_vox = std::max<long>(max_vox, (long)v.w * v.h * v.d);
-    const long max_chunks = max_vox / kChunk + 3;
-    // Each chain (2D passes on `s`, 3D iterations on `s3`) keeps two chunks in
-    // flight: chunk k+1 is queued before the poll after chunk k is read, so
-    // the GPU does not wait for the host.  A chunk queued after the last item
-    // converged finds empty queues and lists.
+    const long max_chunks = max_vox + 3;  // an iteration that deletes nothing ends an item
+    // The 2D chain follows the passes' reports: it keeps at most kAhead2
+    // passes queued past the oldest unreported one, and stops at the first
+    // pass that finds no queued band (every later pass finds none either and
+    // leaves at once).
+    auto drive2 = [&]() -> int {
+        for (int k = 0; k < 16; ++k) __atomic_store_n(&b.h_rep[k], 0ull, __ATOMIC_RELAXED);
+        long chunks = 0;
+        for (int next = 0, idle = 0;;) {
+            if (b.pass2 - next < kAhead2) {
+                if (++chunks > max_chunks)
+                    return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
+                BGT_CHECK(launch_chunk(b, false, use_graph, s));
+                continue;
+            }
+            const unsigned long long v = __atomic_load_n(&b.h_rep[next & 15], __ATOMIC_ACQUIRE);
+            if ((long)(v >> 32) == (long)next + 1) {
+                if ((unsigned)v == 0u) return BGT_OK;
+                ++next;
+                idle = 0;
+                continue;
+            }
+            // not reported yet: every 64 looks, make sure the stream still runs
+            if (++idle % 64 == 0) {
+                const hipError_t q = hipStreamQuery(s);
+                if (q == hipSuccess) {  // all queued passes ran: the report must be there
+                    const unsigned long long w =
+                        __atomic_load_n(&b.h_rep[next & 15], __ATOMIC_ACQUIRE);
+                    if ((long)(w >> 32) != (long)next + 1)
+                        return fail(BGT_ERR_HIP, "2D pass %d did not report", next);
+                    continue;
+                }
+                if (q != hipErrorNotReady) return fail(BGT_ERR_HIP, "2D chain: %s", hipGetErrorString(q));
+                (void)hipGetLastError();  // "not ready" is not an error of a later call
+            }
+            std::this_thread::yield();
+        }
+    };
+    // The 3D chain keeps two chunks in flight: chunk k+1 is queued before the
+    // poll after chunk k is read, so the GPU does not wait for the host.  A
+    // chunk queued after the last volume converged finds empty lists.
     struct Chain {
         bool live = false;
         bool final = false;          // its last launch is queued (the 3D tail)
         int head = 0, inflight = 0;  // oldest pending poll slot, pending polls
-        int last[2] = {0, 0};        // per poll slot: the chunk's last 2D pass
         long chunks = 0;
-    } c2, c3;
-    c2.live = n2 > 0;
+    } c3;
     c3.live = n3 > 0;
-    auto enqueue = [&](Chain &c, bool three_d) -> int {
+    auto enqueue3 = [&](Chain &c) -> int {
         const int slot = (c.head + c.inflight) & 1;
         int *hs = b.h_live + slot * 32;
-        if (!three_d) {
-            BGT_CHECK(launch_chunk(b, false, use_graph, s));
-            c.last[slot] = b.pass2 - 1;
-            BGT_HIP(hipEventRecord(b.ev_poll2[slot], s));
+        if (b.c3.tail_words > 0 && b.iter3 >= 1 &&
+            (long)b.abound3 <= (long)b.c3.tail_words * n3) {
+            // few active words per volume left: the remaining iterations in one
+            // launch, one workgroup per volume
+            size_t mk = 0;
+            BGT_CHECK(timed_mark(b, s3, 1, true, &mk));
+            BGT_CHECK(sparse3d_tail(b.e3, s3));
+            BGT_CHECK(timed_mark(b, s3, 1, false, &mk));
+            c.final = true;
         } else {
-            if (b.c3.tail_words > 0 && b.iter3 >= 1 &&
-                (long)b.abound3 <= (long)b.c3.tail_words * n3) {
-                // few active words per volume left: the remaining iterations in
-                // one launch, one workgroup per volume
-                size_t mk = 0;
-                BGT_CHECK(timed_mark(b, s3, 1, true, &mk));
-                BGT_CHECK(sparse3d_tail(b.e3, s3));
-                BGT_CHECK(timed_mark(b, s3, 1, false, &mk));
-                c.final = true;
-            } else {
-                BGT_CHECK(launch_chunk(b, true, use_graph, s3));
-            }
-            BGT_HIP(hipMemcpyAsync(hs + 8, b.e3.ctl, SC_INTS * sizeof(int), hipMemcpyDeviceToHost,
-                                   s3));
-            BGT_HIP(hipEventRecord(b.ev_poll3[slot], s3));
+            BGT_CHECK(launch_chunk(b, true, use_graph, s3));
         }
+        BGT_HIP(hipMemcpyAsync(hs + 8, b.e3.ctl, SC_INTS * sizeof(int), hipMemcpyDeviceToHost, s3));
+        BGT_HIP(hipEventRecord(b.ev_poll3[slot], s3));
         ++c.inflight;
         if (++c.chunks > max_chunks) return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
         return BGT_OK;
     };
-    // reads the oldest pending poll of a chain (its event has completed)
-    auto consume = [&](Chain &c, bool three_d) {
+    // reads the oldest pending poll (its event has completed)
+    auto consume3 = [&](Chain &c) {
         const int *hp = b.h_live + c.head * 32;
-        if (!three_d) {
-            // the chunk's last pass found no queued band: every item converged
-            // (a later pass may have overwritten the slot: once a pass finds no
-            // band, so does every later one, so its value gives the same answer)
-            const int q = __atomic_load_n(&b.h_rep[c.last[c.head] & 15], __ATOMIC_ACQUIRE);
-            if (q <= 0) c.live = false;
-        } else {
-            // chunks hold an even number of iterations: list 0 is current
-            b.bound3 = hp[8 + SC_COUNT0];
-            b.abound3 = hp[8 + SC_ACOUNT0 + (hp[8 + SC_ITER] & 1)];
-            b.iter3 = hp[8 + SC_ITER];
-            if (hp[8 + SC_LIVE] <= 0) c.live = false;
-        }
+        // chunks hold an even number of iterations: list 0 is current
+        b.bound3 = hp[8 + SC_COUNT0];
+        b.abound3 = hp[8 + SC_ACOUNT0 + (hp[8 + SC_ITER] & 1)];
+        b.iter3 = hp[8 + SC_ITER];
+        if (hp[8 + SC_LIVE] <= 0) c.live = false;
         c.head ^= 1;
         --c.inflight;
     };
-    // blocks until the chain's oldest pending poll has landed
-    auto wait_poll = [&](const Chain &c, bool three_d) -> int {
-        BGT_HIP(hipEventSynchronize(three_d ? b.ev_poll3[c.head] : b.ev_poll2[c.head]));
-        return BGT_OK;
-    };
     // With both kinds of items each chain has a host thread of its own:
     // queueing a 3D chunk is ~32 launches, ~150 us of host time, long enough to
-    // drain the two short tail chunks queued on the 2D chain if one thread fed
+    // drain the short tail passes queued on the 2D chain if one thread fed
     // both.
-    auto drive = [&](Chain &c, bool three_d) -> int {
+    auto drive3 = [&](Chain &c) -> int {
         while (c.live) {
             if (c.inflight < 2 && !c.final) {
-                BGT_CHECK(enqueue(c, three_d));
+                BGT_CHECK(enqueue3(c));
                 continue;
             }
             if (c.final && c.inflight == 0)
                 return fail(BGT_ERR_NOT_CONVERGED, "3D tail did not converge");
-            BGT_CHECK(wait_poll(c, three_d));
-            consume(c, three_d);
+            BGT_HIP(hipEventSynchronize(b.ev_poll3[c.head]));
+            consume3(c);
         }
         return BGT_OK;
     };
     // each chain's closing launches follow its last chunk on its own stream,
     // so the 2D unpack overlaps the end of the 3D chain
     auto run2 = [&]() -> int {
-        BGT_CHECK(drive(c2, false));
         if (!n2) return BGT_OK;
+        BGT_CHECK(drive2());
         hipLaunchKernelGGL(batch_close2d, dim3(1), dim3(256), 0, s, b.close2,
                            b.c2.iters_per_pass, b.pass2);
         if (b.nublk2)
@@ -953,8 +970,8 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         return BGT_OK;
     };
     auto run3 = [&]() -> int {
-        BGT_CHECK(drive(c3, true));
         if (!n3) return BGT_OK;
+        BGT_CHECK(drive3(c3));
         BGT_CHECK(sparse3d_end(b.e3, s3));
         if (fork) BGT_HIP(hipEventRecord(b.ev_join, s3));
         return BGT_OK;
