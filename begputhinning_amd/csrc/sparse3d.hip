@@ -36,39 +36,49 @@ namespace bgt {
 namespace {
 
 const int kCompactPer = 1024;  // list entries per compaction workgroup (4 per thread)
+const int kPackRuns = 8;       // 64-voxel runs per pack thread
 
-// Appends the words of a workgroup whose flags are set to `out`, keeping
-// lane order, with one atomic per workgroup.  Every thread of the block must
-// call it (flags may be 0); two calls in a row need a barrier between them.
-// Each thread offers up to two words (w0, w0 + 1) of blocks (b0, b0 + 1).
-__device__ __forceinline__ void append2(bool f0, bool f1, int item, uint32_t w0, int b0, SList out)
+// Appends the flagged entries (item, word[j], blk[j]) of a workgroup to
+// `out`, up to N per thread, keeping their order, with one atomic per
+// workgroup.  Every thread of the block must call it (flags may be 0); two
+// calls in a row need a barrier between them.
+template <int N>
+__device__ __forceinline__ void append_n(const bool (&f)[N], int item, const uint32_t (&word)[N],
+                                         const int (&blk)[N], SList out)
 {
-    __shared__ int s_cnt[4];
+    __shared__ int s_cnt[16];
     __shared__ int s_base;
-    const uint64_t m0 = __ballot(f0), m1 = __ballot(f1);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
     const uint64_t lt = (1ull << lane) - 1ull;
-    const int pre = __popcll(m0 & lt) + __popcll(m1 & lt);
-    if (lane == 0) s_cnt[wv] = __popcll(m0) + __popcll(m1);
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) c += f[j];
+    // exclusive prefix over the wave of the per-thread counts (0..N), bit by bit
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int bit = 0; (1 << bit) <= N; ++bit) {
+        const uint64_t m = __ballot((c >> bit) & 1);
+        pre += __popcll(m & lt) << bit;
+        tot += __popcll(m) << bit;
+    }
+    if (lane == 0) s_cnt[wv] = tot;
     __syncthreads();
     if (threadIdx.x == 0) {
         int sum = 0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) sum += s_cnt[k];
+        for (int k = 0; k < nw; ++k) sum += s_cnt[k];
         s_base = sum ? atomicAdd(out.count, sum) : 0;
     }
     __syncthreads();
     int pos = s_base + pre;
     for (int k = 0; k < wv; ++k) pos += s_cnt[k];
-    if (f0) {
-        out.item[pos] = item;
-        out.word[pos] = w0;
-        out.blk[pos] = b0;
-        ++pos;
-    }
-    if (f1) {
-        out.item[pos] = item;
-        out.word[pos] = w0 + 1;
-        out.blk[pos] = b0 + 1;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        if (f[j]) {
+            out.item[pos] = item;
+            out.word[pos] = word[j];
+            out.blk[pos] = blk[j];
+            ++pos;
+        }
     }
 }
 
@@ -156,9 +166,9 @@ __global__ void sparse3d_zero(uint32_t *a, uint32_t *b, long n)
     }
 }
 
-// One thread per 64 voxels (two data words) of an x-line: writes both state
-// arrays and lists the non-empty words (list 0, also the first iteration's
-// active list).
+// Each thread packs kPackRuns runs of 64 voxels (two data words) of x-lines:
+// writes both state arrays and lists the non-empty words (list 0, also the
+// first iteration's active list).
 __global__ __launch_bounds__(256) void sparse3d_pack(const SItem3D *__restrict__ items,
                                                      const SIO *__restrict__ io,
                                                      const int *__restrict__ blk, uint32_t *s0,
@@ -168,29 +178,45 @@ __global__ __launch_bounds__(256) void sparse3d_pack(const SItem3D *__restrict__
     const SItem3D it = items[i];
     const SIO f = io[i];
     const uint32_t nq = (uint32_t)it.nh >> 1;
+    const uint32_t nruns = (uint32_t)it.h * (uint32_t)it.d * nq;
     __shared__ uint32_t s_masks[4][320];
-    const uint32_t t = (uint32_t)(blockIdx.x - f.blk0) * 256u + threadIdx.x;
-    const bool valid = t < (uint32_t)it.h * (uint32_t)it.d * nq;
-    const uint32_t line = valid ? t / nq : 0;
-    const int x0 = valid ? (int)(t - line * nq) * 64 : 0;  // < w: nq = ceil(w / 64)
-    // the caller's lines follow each other in memory, and so do the runs
-    const uint64_t v = pack_runs_wave_u8(f.in + (size_t)line * it.w + x0, min(64, it.w - x0), valid,
-                                         s_masks[threadIdx.x >> 6]);
-    uint32_t lo = 0, hi = 0, g = 0;
-    int b0 = 0;
-    if (valid) {
-        lo = (uint32_t)v;
-        hi = (uint32_t)(v >> 32);
-        const uint32_t y = line % (uint32_t)it.h, z = line / (uint32_t)it.h;
-        g = (uint32_t)it.base + ((z + 1) * (uint32_t)(it.h + 2) + y + 1) * (uint32_t)it.P + 1 +
-            (uint32_t)x0 / 32;
-        b0 = it.bbase + ((int)(z >> 3) * it.nby + (int)(y >> 3)) * it.nh + x0 / 32;
-        s0[g] = lo;
-        s0[g + 1] = hi;
-        s1[g] = lo;
-        s1[g + 1] = hi;
+    bool fl[2 * kPackRuns];
+    uint32_t wd[2 * kPackRuns];
+    int bk[2 * kPackRuns];
+#pragma unroll
+    for (int k = 0; k < kPackRuns; ++k) {
+        // runs k*256 .. k*256 + 255 of the workgroup: whole waves of consecutive runs
+        const uint32_t t = ((uint32_t)(blockIdx.x - f.blk0) * kPackRuns + k) * 256u + threadIdx.x;
+        const bool valid = t < nruns;
+        const uint32_t line = valid ? t / nq : 0;
+        const int x0 = valid ? (int)(t - line * nq) * 64 : 0;  // < w: nq = ceil(w / 64)
+        // the caller's lines follow each other in memory, and so do the runs
+        const uint64_t v = pack_runs_wave_u8(f.in + (size_t)line * it.w + x0, min(64, it.w - x0),
+                                             valid, s_masks[threadIdx.x >> 6]);
+        uint32_t lo = 0, hi = 0, g = 0;
+        int b0 = 0;
+        if (valid) {
+            lo = (uint32_t)v;
+            hi = (uint32_t)(v >> 32);
+            const uint32_t y = line % (uint32_t)it.h, z = line / (uint32_t)it.h;
+            g = (uint32_t)it.base + ((z + 1) * (uint32_t)(it.h + 2) + y + 1) * (uint32_t)it.P + 1 +
+                (uint32_t)x0 / 32;
+            b0 = it.bbase + ((int)(z >> 3) * it.nby + (int)(y >> 3)) * it.nh + x0 / 32;
+            s0[g] = lo;
+            s0[g + 1] = hi;
+            s1[g] = lo;
+            s1[g + 1] = hi;
+        }
+        fl[2 * k] = lo != 0;
+        fl[2 * k + 1] = hi != 0;
+        wd[2 * k] = g;
+        wd[2 * k + 1] = g + 1;
+        bk[2 * k] = b0;
+        bk[2 * k + 1] = b0 + 1;
     }
-    append2(lo != 0, hi != 0, i, g, b0, out);
+    // one list atomic per workgroup: same-address atomics are what bounds a
+    // pack of many small workgroups
+    append_n<2 * kPackRuns>(fl, i, wd, bk, out);
 }
 
 // Pixels of one z-plane of an item in state array 0.
@@ -621,7 +647,8 @@ int sparse3d_layout(const int *w, const int *h, const int *d, int n, Layout3D &l
         data += (long)h[i] * d[i] * it.nh;
         const long threads = (long)h[i] * d[i] * (it.nh / 2);
         lay.blk0[i] = (int)lay.blk.size();
-        for (long k = 0; k < (threads + 255) / 256; ++k) lay.blk.push_back(i);
+        for (long k = 0; k < (threads + 256 * kPackRuns - 1) / (256 * kPackRuns); ++k)
+            lay.blk.push_back(i);
         const long uthreads = (long)d[i] * unpack_wave_blocks((uint32_t)(w[i] * h[i]));
         lay.ublk0[i] = (int)lay.ublk.size();
         for (long k = 0; k < (uthreads + 255) / 256; ++k) lay.ublk.push_back(i);
