@@ -1,21 +1,24 @@
 // batch.hip -- many images and volumes of uneven size thinned together.
 //
 // All items of a batch live in one device allocation.  2D items and 3D items
-// form two independent pipelines on two streams; the host polls their
-// live-item counters once per chunk of passes, and a chunk can be replayed as
-// a captured hipGraph.
+// form two independent pipelines on two streams, each fed by its own host
+// thread; chunks of launches can be replayed as captured hipGraphs.
 //
 // 2D: each pass runs from a work queue of active bands.  A band is in the
 // queue of pass P+1 iff it changed in pass P, or it is non-empty and a
 // neighbouring band changed in pass P; empty bands are never queued.  Bands
 // never queued for a pass hold identical data in both ping-pong buffers
 // (packing writes both), so skipping them is exact.  The queue kernel runs a
-// fixed, co-resident grid whose workgroups pull bands from an atomic counter;
-// a one-workgroup finalize kernel then advances every item (converged when one
-// iteration of the pass deleted nothing), resets the counters and bumps the
-// pass number.
+// fixed, co-resident grid: workgroup b takes queued band b and the bands past
+// the grid come from an atomic counter (or, with few bands queued, each band
+// is split over several workgroups).  Its last workgroup does the pass's
+// bookkeeping: it closes the items of pass P-2 (converged when one iteration
+// of a pass deleted nothing), clears the idle queue counters and reports the
+// pass's queued-band count to the host through host-mapped memory.  No launch
+// runs between passes.
 //
-// 3D: the non-empty-word list engine (sparse3d.hip) over all volumes at once.
+// 3D: the non-empty-word list engine (sparse3d.hip) over all volumes at once;
+// the host polls its control words once per chunk of iterations.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
