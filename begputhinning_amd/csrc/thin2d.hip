@@ -128,17 +128,23 @@ __global__ void init_item2d(Item2D *item, Item2D val, unsigned *ctr)
     if (t == 0) *item = val;
     if (t < ST_INTS) val.st[t] = 0;
     if (t == 0 && ctr) *ctr = 0;
+    for (int k = t; k < 2 * val.nbands; k += blockDim.x) val.chg[k] = 0;  // flags only get set
 }
 
 // ---------------------------------------------------------------------------
 // the fused pass kernel
 // ---------------------------------------------------------------------------
 
-// Per-image pass kernel: one workgroup per band, the last workgroup of the
-// launch advances the item.
+// Per-image pass kernel: kp workgroups per band (part = blockIdx % kp), the
+// last workgroup of the launch advances the item.  With kp > 1 a band is
+// split into parts that run in windows of 4 or 8 register rows per segment
+// (the finest whose parts number at most kp), each with full 4K-row halos, so
+// the band's result is the same while a pass's latency, bound by the rows one
+// wave works through per subiteration, drops; where no such window fits,
+// part 0 runs the whole band.
 template <int R>
 __global__ __launch_bounds__(512) void thin2d_pass(const Item2D *__restrict__ items, int K,
-                                                    unsigned *__restrict__ ctr)
+                                                    unsigned *__restrict__ ctr, int kp)
 {
     extern __shared__ uint64_t edges[];
     __shared__ TileInfo info;
@@ -146,7 +152,7 @@ __global__ __launch_bounds__(512) void thin2d_pass(const Item2D *__restrict__ it
 
     const int tid = threadIdx.x;
     const Item2D it = items[0];
-    const int band = blockIdx.x;
+    const int band = blockIdx.x / kp, part = blockIdx.x % kp;
     bool work = !it.st[ST_DONE];
     int p = 0, par = 0;
     if (work) {
@@ -154,19 +160,43 @@ __global__ __launch_bounds__(512) void thin2d_pass(const Item2D *__restrict__ it
         par = p & 1;
         if (p > 0) {
             const uint8_t *c = it.chg + (par ^ 1) * it.nbands;
-            bool act = c[band] || (band > 0 && c[band - 1]) ||
-                       (band + 1 < it.nbands && c[band + 1]);
-            if (!act) {
-                if (tid == 0) it.chg[par * it.nbands + band] = 0;
-                work = false;
-            }
+            work = c[band] || (band > 0 && c[band - 1]) || (band + 1 < it.nbands && c[band + 1]);
         }
     }
+    int row0 = band * it.band_rows, nrows = it.band_rows, win = R;
+    if (work && kp > 1) {
+        const int seg = (int)(blockDim.x >> 6) * (64 / it.L);  // segments per window
+        const int in4 = seg * 4 - 8 * K, in8 = seg * 8 - 8 * K;  // interior rows of a part
+        const int np4 = in4 >= 1 ? (it.band_rows + in4 - 1) / in4 : kp + 1;
+        const int np8 = in8 >= 1 ? (it.band_rows + in8 - 1) / in8 : kp + 1;
+        int inner = 0, np = 1;
+        if (np4 <= kp) {
+            win = 4;
+            inner = in4;
+            np = np4;
+        } else if (R > 8 && np8 <= kp) {
+            win = 8;
+            inner = in8;
+            np = np8;
+        }
+        if (win != R) {
+            row0 += part * inner;
+            nrows = min(inner, it.band_rows - part * inner);
+        }
+        work = part < np && row0 < it.h;
+    }
     if (work) {
-        const int run = band_work<R>(it, band * it.band_rows, it.band_rows, it.nbands == 1, K,
-                                     par, edges, &info);
+        int run = 0;
+        if (win == 4)
+            band_work<4>(it, row0, nrows, false, K, par, edges, &info);
+        else if (win == 8 && R > 8)
+            band_work<8>(it, row0, nrows, false, K, par, edges, &info);
+        else
+            run = band_work<R>(it, row0, nrows, it.nbands == 1, K, par, edges, &info);
         if (tid == 0) {
-            it.chg[par * it.nbands + band] = info.mask != 0;
+            // flags are cleared before the pass (init, then each pass's last
+            // workgroup), so the parts of a band only set them
+            if (info.mask) it.chg[par * it.nbands + band] = 1;
             if (info.mask) atomicOr((unsigned *)&it.st[ST_MASK], info.mask);
             if (it.nbands == 1) it.st[ST_RUN] = run + 1;
         }
@@ -176,7 +206,12 @@ __global__ __launch_bounds__(512) void thin2d_pass(const Item2D *__restrict__ it
         s_last = atomicAdd(ctr, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (s_last && tid == 0) {
+    if (!s_last) return;
+    // every workgroup of this pass has read the previous pass's flags: clear
+    // them for the next pass to set
+    if (!it.st[ST_DONE])
+        for (int k = tid; k < it.nbands; k += blockDim.x) it.chg[(par ^ 1) * it.nbands + k] = 0;
+    if (tid == 0) {
         __threadfence();
         finalize_state(it.st, K, it.st[ST_PASS], nullptr);
         __threadfence();
@@ -204,21 +239,39 @@ int check_config2d(const Config2D &cfg)
     return BGT_OK;
 }
 
-static int launch_image_pass2d(const Config2D &cfg, int nblocks, const Item2D *item,
+static int launch_image_pass2d(const Config2D &cfg, int nbands, int kp, const Item2D *item,
                                unsigned *ctr, hipStream_t s)
 {
     const size_t lds = (size_t)2 * 2 * cfg.waves * 64 * sizeof(uint64_t);
-    dim3 blk(cfg.waves * 64);
+    const dim3 grid(nbands * kp), blk(cfg.waves * 64);
+    const int K = cfg.iters_per_pass;
     switch (cfg.rows) {
     case 8:
-        hipLaunchKernelGGL(thin2d_pass<8>, dim3(nblocks), blk, lds, s, item, cfg.iters_per_pass, ctr);
+        hipLaunchKernelGGL(thin2d_pass<8>, grid, blk, lds, s, item, K, ctr, kp);
         break;
     default:
-        hipLaunchKernelGGL(thin2d_pass<16>, dim3(nblocks), blk, lds, s, item, cfg.iters_per_pass, ctr);
+        hipLaunchKernelGGL(thin2d_pass<16>, grid, blk, lds, s, item, K, ctr, kp);
         break;
     }
     BGT_HIP(hipGetLastError());
     return BGT_OK;
+}
+
+// Workgroups per band of the per-image pass: a multi-band image spreads its
+// bands over the CUs (at most 16 parts a band); a single-band image runs to
+// convergence in one workgroup.
+static int image_parts(int nbands)
+{
+    static const int cus = [] {
+        int dev = 0, c = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            c > 0)
+            return c;
+        return 256;
+    }();
+    if (nbands <= 1) return 1;
+    return std::max(1, std::min(16, cus / nbands));
 }
 
 int plan2d(int w, int h, const Config2D &cfg, Plan2D &pl)
@@ -303,9 +356,10 @@ int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned
     const long max_pass = (long)pl.w * pl.h / cfg.iters_per_pass + 2;  // each changing iteration deletes >= 1 px
     long launched = 0;
     int chunk = 2;
+    const int kp = image_parts(pl.nbands);
     for (;;) {
         for (int i = 0; i < chunk; ++i)
-            BGT_CHECK(launch_image_pass2d(cfg, pl.nbands, d_item, d_ctr, s));
+            BGT_CHECK(launch_image_pass2d(cfg, pl.nbands, kp, d_item, d_ctr, s));
         launched += chunk;
         BGT_HIP(hipMemcpyAsync(hp, d_state, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
         BGT_HIP(hipStreamSynchronize(s));
