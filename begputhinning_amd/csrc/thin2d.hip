@@ -355,7 +355,9 @@ int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned
     if (!hp) return fail(BGT_ERR_NOMEM, "pinned poll buffer allocation failed");
     const long max_pass = (long)pl.w * pl.h / cfg.iters_per_pass + 2;  // each changing iteration deletes >= 1 px
     long launched = 0;
-    int chunk = 2;
+    // a single-band image converges in its first launch; a multi-band one
+    // takes a few passes of K iterations, so the first poll waits for four
+    int chunk = pl.nbands == 1 ? 1 : 4;
     const int kp = image_parts(pl.nbands);
     for (;;) {
         for (int i = 0; i < chunk; ++i)
