@@ -104,6 +104,39 @@ def test_batch_timing_and_single_item_equality():
     b.close()
 
 
+@pytest.mark.parametrize("kind", ["empty", "tiny", "one_band", "full"])
+def test_degenerate_batches(kind):
+    """Batches whose first pass finds no band (all empty), single pixels, one
+    single-band image, and solid images: every item equals the oracle,
+    iteration counts included, over two runs."""
+    from begputhinning_amd.batch import ThinningBatch
+    if kind == "empty":
+        items = [np.zeros((5, 7), np.uint8), np.zeros((300, 2100), np.uint8),
+                 np.zeros((4, 4, 4), np.uint8)]
+    elif kind == "tiny":
+        items = [np.ones((1, 1), np.uint8), np.zeros((1, 1), np.uint8), np.ones((1, 1, 1), np.uint8)]
+    elif kind == "one_band":
+        rng = np.random.default_rng(11)
+        items = [(rng.random((40, 300)) < 0.7).astype(np.uint8)]
+    else:
+        items = [np.ones((500, 130), np.uint8), np.ones((64, 2000), np.uint8),
+                 np.ones((20, 30, 40), np.uint8)]
+    ins = [torch.from_numpy(a).cuda() for a in items]
+    outs = [torch.full_like(t, 9) for t in ins]
+    b = ThinningBatch()
+    for t, o in zip(ins, outs):
+        (b.add2d if t.dim() == 2 else b.add3d)(t, o)
+    for rep in range(2):
+        b.run(use_graph=rep == 1)
+        torch.cuda.synchronize()
+        its = b.iterations()
+        for k, (a, o) in enumerate(zip(items, outs)):
+            ref, rit = O.thin2d(a) if a.ndim == 2 else O.thin3d(a)
+            assert np.array_equal(o.cpu().numpy(), ref), (kind, rep, k)
+            assert its[k] == rit, (kind, rep, k, its[k], rit)
+    b.close()
+
+
 def split_items():
     """Images whose tails leave a few bands queued (a thick disc, thick lines),
     over every segment count per wave from 1 (width > 2048) to 64 (width <= 64),
