@@ -54,6 +54,9 @@ SIGNATURES = {
     "bgt_thin3d_device": (_i, [_vp, _i, _i, _i, _vp, _vp, _ip]),
     "bgt_extract_segments2d": (_i, [_i, _vp, _i, _i, ctypes.POINTER(Segments)]),
     "bgt_extract_segments3d": (_i, [_i, _vp, _i, _i, _i, ctypes.POINTER(Segments)]),
+    "bgt_extract_segments2d_device": (_i, [_vp, _i, _i, _i, _i, _vp, ctypes.POINTER(Segments)]),
+    "bgt_extract_segments3d_device": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp,
+                                           ctypes.POINTER(Segments)]),
     "bgt_segments_free": (None, [ctypes.POINTER(Segments)]),
     "bgt_centerline2d": (_i, [_i, _vp, _i, _i, _i, _i, _fpp, _fpp, _upp, _up]),
     "bgt_centerline3d": (_i, [_i, _vp, _i, _i, _i, _i, _i, _fpp, _fpp, _fpp, _upp, _up]),

@@ -11,6 +11,7 @@
 
 #include "batch.h"
 #include "common.h"
+#include "seggraph.h"
 #include "segments.h"
 #include "server.h"
 #include "thin2d.h"
@@ -68,8 +69,9 @@ int workspace_reserve(Workspace &ws, size_t bytes)
 
 Workspace &thread_workspace(int slot)
 {
-    thread_local Workspace ws[4];
-    return ws[slot & 3];
+    // 0, 1: 2D state and I/O; 2, 3: 3D state and I/O; 5, 6: segment pre-pass
+    thread_local Workspace ws[8];
+    return ws[slot & 7];
 }
 
 static Config2D &config2d_storage()
@@ -167,6 +169,36 @@ int bgt_extract_segments3d(int dtype, const void *vol, int width, int height, in
                            bgt_segments *out)
 {
     return extract_segments(dtype, vol, width, height, depth, true, out);
+}
+
+static int extract_device(const void *d_skel, int w, int h, int d, int pw, int ph, int pd,
+                          bool three_d, void *stream, bgt_segments *out)
+{
+    if (!d_skel || !out) return fail(BGT_ERR_INVALID, "null pointer");
+    std::memset(out, 0, sizeof(*out));
+    if (w <= 0 || h <= 0 || d <= 0) return fail(BGT_ERR_INVALID, "bad size");
+    if (pw < w || ph < h || pd < d) return fail(BGT_ERR_INVALID, "padded frame smaller than skeleton");
+    if (pw > 32767 || ph > 32767 || pd > 32767)
+        return fail(BGT_ERR_UNSUPPORTED, "coordinates are int16 (sides <= 32767)");
+    SegGraph g;
+    BGT_CHECK(segment_graph_device((const uint8_t *)d_skel, w, h, d, pw, ph, pd, three_d,
+                                   (hipStream_t)stream, g));
+    return extract_segments_graph(g, pw, ph, three_d, out);
+}
+
+int bgt_extract_segments2d_device(const void *d_skel, int width, int height, int padded_width,
+                                  int padded_height, void *stream, bgt_segments *out)
+{
+    return extract_device(d_skel, width, height, 1, padded_width, padded_height, 1, false, stream,
+                          out);
+}
+
+int bgt_extract_segments3d_device(const void *d_skel, int width, int height, int depth,
+                                  int padded_width, int padded_height, int padded_depth,
+                                  void *stream, bgt_segments *out)
+{
+    return extract_device(d_skel, width, height, depth, padded_width, padded_height, padded_depth,
+                          true, stream, out);
 }
 
 void bgt_segments_free(bgt_segments *s)
@@ -343,13 +375,14 @@ int bgt_centerline2d(int dtype, const void *img, int width, int height, int max_
     const int q = (height + 63) / 128;
     const int ph = (q > 1 ? q : 1) * 128 + 64;
     if (pw > 32767 || ph > 32767) return fail(BGT_ERR_UNSUPPORTED, "padded size exceeds int16");
-    std::vector<uint8_t> skel((size_t)width * height);
-    BGT_CHECK(thin2d_host_mask(dtype, width, height, img, skel.data(), default_config2d()));
-    std::vector<uint8_t> frame((size_t)pw * ph, 0);
-    for (int y = 0; y < height; ++y)
-        std::memcpy(frame.data() + (size_t)y * pw, skel.data() + (size_t)y * width, width);
+    // thin on the device, then classify and compact the skeleton points there:
+    // the host traces only those, not the padded frame
+    const uint8_t *d_skel = nullptr;
+    BGT_CHECK(thin2d_host_keep(dtype, width, height, img, &d_skel, default_config2d()));
+    SegGraph g;
+    BGT_CHECK(segment_graph_device(d_skel, width, height, 1, pw, ph, 1, false, nullptr, g));
     bgt_segments s;
-    BGT_CHECK(extract_segments_mask(frame.data(), pw, ph, 1, false, &s));
+    BGT_CHECK(extract_segments_graph(g, pw, ph, false, &s));
     int r = centerline_from_segments(s, false, max_segment_length, smoothing_span, out_x, out_y,
                                      nullptr, out_idx, nsegs);
     bgt_segments_free(&s);
@@ -369,17 +402,16 @@ int bgt_centerline3d(int dtype, const void *vol, int width, int height, int dept
     if (pw > 32767 || ph > 32767 || pd > 32767)
         return fail(BGT_ERR_UNSUPPORTED, "padded size exceeds int16");
     const size_t n = (size_t)width * height * depth;
-    std::vector<uint8_t> mask(n), skel(n);
+    std::vector<uint8_t> mask(n);
     BGT_CHECK(threshold_int_positive(dtype, vol, n, mask.data()));
-    BGT_CHECK(thin3d_host(BGT_U8, width, height, depth, mask.data(), skel.data(),
-                          default_config3d()));
-    std::vector<uint8_t> frame((size_t)pw * ph * pd, 0);
-    for (int z = 0; z < depth; ++z)
-        for (int y = 0; y < height; ++y)
-            std::memcpy(frame.data() + ((size_t)z * ph + y) * pw,
-                        skel.data() + ((size_t)z * height + y) * width, width);
+    // thin on the device, then classify and compact the skeleton points there:
+    // the host traces only those, not the padded volume
+    const uint8_t *d_skel = nullptr;
+    BGT_CHECK(thin3d_mask_keep(width, height, depth, mask.data(), &d_skel, default_config3d()));
+    SegGraph g;
+    BGT_CHECK(segment_graph_device(d_skel, width, height, depth, pw, ph, pd, true, nullptr, g));
     bgt_segments s;
-    BGT_CHECK(extract_segments_mask(frame.data(), pw, ph, pd, true, &s));
+    BGT_CHECK(extract_segments_graph(g, pw, ph, true, &s));
     int r = centerline_from_segments(s, true, max_segment_length, smoothing_span, out_x, out_y,
                                      out_z, out_idx, nsegs);
     bgt_segments_free(&s);

@@ -1,0 +1,28 @@
+// seggraph.h -- device pre-pass of the segment tracer (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace bgt {
+
+// Skeleton points of a padded frame in raster order: raster index, the
+// tracer's code (-1 isolated, 2 path, -n end or junction with n neighbours),
+// and for each of the nd neighbours in the reference's order the rank of that
+// neighbour if it is a point itself, else -1.
+struct SegGraph {
+    int npts = 0;
+    int nd = 8;
+    std::vector<int64_t> idx;
+    std::vector<int8_t> code;
+    std::vector<int> nbr;  // npts x nd
+};
+
+// d_skel: w x h x d uint8 skeleton in device memory (nonzero = set), the top
+// left corner of a zero-padded pw x ph x pd frame (d = pd = 1 in 2D).
+int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int ph, int pd,
+                         bool three_d, hipStream_t s, SegGraph &g);
+
+}  // namespace bgt

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "common.h"
+#include "seggraph.h"
 #include "segments.h"
 
 namespace bgt {
@@ -114,6 +115,56 @@ void trace(const uint8_t *mask, int w, int h, int d, bool three_d, Flat &out)
     }
 }
 
+// The second pass of trace() over the device pre-pass's compact graph
+// (seggraph.hip): the same visiting order, neighbour order and code
+// bookkeeping, with a point's rank in place of its raster index and a
+// neighbour rank of -1 (not a point) in place of a zero code.
+void trace_graph(const SegGraph &g, int pw, int ph, bool three_d, Flat &out)
+{
+    const int nd = g.nd;
+    const long sl = (long)pw * ph;
+    std::vector<int8_t> code(g.code), orig(g.code);
+    auto emit = [&](int r) {
+        const long q = (long)g.idx[r];
+        const int zz = (int)(q / sl);
+        const long rr = q - (long)zz * sl;
+        out.x.push_back((int16_t)(rr % pw));
+        out.y.push_back((int16_t)(rr / pw));
+        if (three_d) out.z.push_back((int16_t)zz);
+    };
+    auto close = [&](bool end) {
+        out.is_end.push_back(end ? 1 : 0);
+        out.start.push_back((uint32_t)out.x.size());
+    };
+    auto live = [&](int r) { return r >= 0 && code[r] != 0; };
+    for (int idx = 0; idx < g.npts; ++idx) {
+        const int8_t nodes = code[idx];
+        if (nodes >= 0) continue;
+        code[idx] = 0;
+        const int *nb = &g.nbr[(size_t)idx * nd];
+        int ni = -1;
+        for (int8_t si = 0; si > nodes; --si) {
+            bool end = std::abs((int)orig[idx]) < 2;
+            emit(idx);
+            ++ni;
+            while (ni < nd && !live(nb[ni])) ++ni;
+            if (ni >= nd) { close(end); continue; }
+            int t = nb[ni];
+            while (code[t] == 2) {
+                code[t] = 0;
+                emit(t);
+                const int *nt = &g.nbr[(size_t)t * nd];
+                for (int k = 0; k < nd; ++k)
+                    if (live(nt[k])) { t = nt[k]; break; }
+            }
+            emit(t);
+            if (std::abs((int)orig[t]) < 2) end = true;
+            code[t] = (int8_t)(code[t] + 1);
+            close(end);
+        }
+    }
+}
+
 template <typename T>
 void to_mask_nonzero(const T *in, size_t n, uint8_t *m)
 {
@@ -167,6 +218,14 @@ int extract_segments_mask(const uint8_t *mask, int w, int h, int d, bool three_d
     std::memset(out, 0, sizeof(*out));
     Flat f;
     if (w > 0 && h > 0 && d > 0) trace(mask, w, h, d, three_d, f);
+    return export_segments(f, three_d, out);
+}
+
+int extract_segments_graph(const SegGraph &g, int pw, int ph, bool three_d, bgt_segments *out)
+{
+    std::memset(out, 0, sizeof(*out));
+    Flat f;
+    trace_graph(g, pw, ph, three_d, f);
     return export_segments(f, three_d, out);
 }
 

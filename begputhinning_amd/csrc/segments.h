@@ -11,6 +11,9 @@ int extract_segments_mask(const uint8_t *mask, int w, int h, int d, bool three_d
                           bgt_segments *out);
 int extract_segments(int dtype, const void *data, int w, int h, int d, bool three_d,
                      bgt_segments *out);
+struct SegGraph;
+// segments from the device pre-pass's graph of a pw x ph (x pd) frame
+int extract_segments_graph(const SegGraph &g, int pw, int ph, bool three_d, bgt_segments *out);
 int centerline_from_segments(const bgt_segments &s, bool three_d, int max_segment_length,
                              int smoothing_span, float **ox, float **oy, float **oz,
                              uint32_t **oidx, uint32_t *nsegs);

@@ -44,4 +44,8 @@ const Config2D &default_config2d();
 
 namespace bgt {
 int thin2d_host_mask(int dtype, int w, int h, const void *in, uint8_t *out, const Config2D &cfg);
+// as thin2d_host_mask, but the w x h uint8 skeleton stays on the device, in
+// this thread's I/O workspace (valid until the thread's next thinning call)
+int thin2d_host_keep(int dtype, int w, int h, const void *in, const uint8_t **d_skel,
+                     const Config2D &cfg);
 }

@@ -400,7 +400,8 @@ int thin2d_device_u8(const uint8_t *d_in, size_t in_pitch, int w, int h, uint8_t
 
 // OutT is T (drop-in Thinning2D) or uint8_t (0/1 mask for the tracer)
 template <typename T, typename OutT = T>
-static int thin2d_host_t(int w, int h, const T *in, OutT *out, const Config2D &cfg)
+static int thin2d_host_t(int w, int h, const T *in, OutT *out, const Config2D &cfg,
+                         const OutT **dev_out = nullptr)
 {
     Plan2D pl;
     BGT_CHECK(plan2d(w, h, cfg, pl));
@@ -429,6 +430,11 @@ static int thin2d_host_t(int w, int h, const T *in, OutT *out, const Config2D &c
     else
         hipLaunchKernelGGL(unpack2d_t<OutT>, g, b, 0, s, d_item, (OutT *)io.ptr);
     BGT_HIP(hipGetLastError());
+    if (dev_out) {  // the result stays in this thread's I/O workspace
+        *dev_out = (const OutT *)io.ptr;
+        BGT_HIP(hipStreamSynchronize(s));
+        return BGT_OK;
+    }
     BGT_HIP(hipMemcpyAsync(out, io.ptr, (size_t)w * h * sizeof(OutT), hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
     return BGT_OK;
@@ -447,6 +453,24 @@ int thin2d_host(int dtype, int w, int h, const void *in, void *out, const Config
     case BGT_F32: return thin2d_host_t<float>(w, h, (const float *)in, (float *)out, cfg);
     case BGT_F64: return thin2d_host_t<double>(w, h, (const double *)in, (double *)out, cfg);
     case BGT_BOOL: return thin2d_host_t<bool>(w, h, (const bool *)in, (bool *)out, cfg);
+    default: return fail(BGT_ERR_INVALID, "unknown pixel type %d", dtype);
+    }
+}
+
+int thin2d_host_keep(int dtype, int w, int h, const void *in, const uint8_t **d_skel,
+                     const Config2D &cfg)
+{
+    if (!in || !d_skel) return fail(BGT_ERR_INVALID, "null pointer");
+    switch (dtype) {
+    case BGT_U8: return thin2d_host_t<uint8_t, uint8_t>(w, h, (const uint8_t *)in, nullptr, cfg, d_skel);
+    case BGT_I8: return thin2d_host_t<int8_t, uint8_t>(w, h, (const int8_t *)in, nullptr, cfg, d_skel);
+    case BGT_U16: return thin2d_host_t<uint16_t, uint8_t>(w, h, (const uint16_t *)in, nullptr, cfg, d_skel);
+    case BGT_I16: return thin2d_host_t<int16_t, uint8_t>(w, h, (const int16_t *)in, nullptr, cfg, d_skel);
+    case BGT_U32: return thin2d_host_t<uint32_t, uint8_t>(w, h, (const uint32_t *)in, nullptr, cfg, d_skel);
+    case BGT_I32: return thin2d_host_t<int32_t, uint8_t>(w, h, (const int32_t *)in, nullptr, cfg, d_skel);
+    case BGT_F32: return thin2d_host_t<float, uint8_t>(w, h, (const float *)in, nullptr, cfg, d_skel);
+    case BGT_F64: return thin2d_host_t<double, uint8_t>(w, h, (const double *)in, nullptr, cfg, d_skel);
+    case BGT_BOOL: return thin2d_host_t<bool, uint8_t>(w, h, (const bool *)in, nullptr, cfg, d_skel);
     default: return fail(BGT_ERR_INVALID, "unknown pixel type %d", dtype);
     }
 }

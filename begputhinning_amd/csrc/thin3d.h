@@ -22,6 +22,10 @@ int check_config3d(const Config3D &c);
 int thin3d_device_u8(const uint8_t *d_in, int w, int h, int d, uint8_t *d_out, hipStream_t s,
                      int *iterations, const Config3D &cfg);
 int thin3d_host(int dtype, int w, int h, int d, const void *in, void *out, const Config3D &cfg);
+// thins a host uint8 mask; the w x h x d skeleton stays on the device, in this
+// thread's I/O workspace (valid until the thread's next thinning call)
+int thin3d_mask_keep(int w, int h, int d, const uint8_t *mask, const uint8_t **d_skel,
+                     const Config3D &cfg);
 int threshold_int_positive(int dtype, const void *in, size_t n, uint8_t *mask);
 
 const Config3D &default_config3d();

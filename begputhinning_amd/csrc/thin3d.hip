@@ -205,7 +205,8 @@ int thin3d_device_u8(const uint8_t *d_in, int w, int h, int d, uint8_t *d_out, h
 }
 
 template <typename T>
-static int thin3d_host_t(int w, int h, int d, const T *in, T *out, const Config3D &cfg)
+static int thin3d_host_t(int w, int h, int d, const T *in, T *out, const Config3D &cfg,
+                         const uint8_t **dev_mask = nullptr)
 {
     BGT_CHECK(check_config3d(cfg));
     if (w <= 0 || h <= 0 || d <= 0)
@@ -225,11 +226,23 @@ static int thin3d_host_t(int w, int h, int d, const T *in, T *out, const Config3
     OneVolume ov;
     BGT_CHECK(one_volume_setup(w, h, d, mask, mask, thread_workspace(2), s, ov));
     BGT_CHECK(one_volume_run(ov, cfg, s, nullptr));
+    if (dev_mask) {  // the skeleton stays in this thread's I/O workspace
+        *dev_mask = mask;
+        BGT_HIP(hipStreamSynchronize(s));
+        return BGT_OK;
+    }
     if (!direct) hipLaunchKernelGGL(mask3d_to<T>, g, b, 0, s, mask, (T *)io.ptr, n);
     BGT_HIP(hipGetLastError());
     BGT_HIP(hipMemcpyAsync(out, io.ptr, bytes, hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
     return BGT_OK;
+}
+
+int thin3d_mask_keep(int w, int h, int d, const uint8_t *mask, const uint8_t **d_skel,
+                     const Config3D &cfg)
+{
+    if (!mask || !d_skel) return fail(BGT_ERR_INVALID, "null pointer");
+    return thin3d_host_t<uint8_t>(w, h, d, mask, nullptr, cfg, d_skel);
 }
 
 int thin3d_host(int dtype, int w, int h, int d, const void *in, void *out, const Config3D &cfg)

@@ -184,3 +184,28 @@ def set_config3d(blocks_per_cu=0, max_chunk=16, tail_words=2048):
     average active words per volume at which the remaining iterations run in
     one launch, one workgroup per volume (0 = never)."""
     check(lib().bgt_set_config3d(blocks_per_cu, max_chunk, tail_words))
+
+
+def extract_segments_device(skel, padded=None, stream=None):
+    """ExtractSegmentsV over a contiguous uint8 torch tensor on the GPU (h x w
+    or d x h x w), the top-left corner of a zero-padded frame of shape
+    `padded` (default: the tensor's own): the skeleton points are classified
+    and compacted on the GPU, and the host follows only them.  Returns the
+    same flat arrays as extract_segments_flat on the padded frame."""
+    import torch
+    if skel.dtype != torch.uint8 or not skel.is_cuda or not skel.is_contiguous():
+        raise ValueError("expects a contiguous uint8 CUDA tensor")
+    shape = tuple(skel.shape)
+    pad = tuple(padded) if padded is not None else shape
+    s = Segments()
+    if skel.dim() == 2:
+        check(lib().bgt_extract_segments2d_device(ctypes.c_void_p(skel.data_ptr()), shape[1], shape[0],
+                                                  pad[1], pad[0], _stream_handle(stream),
+                                                  ctypes.byref(s)))
+        return _take_segments(s, False)
+    if skel.dim() == 3:
+        check(lib().bgt_extract_segments3d_device(ctypes.c_void_p(skel.data_ptr()), shape[2], shape[1],
+                                                  shape[0], pad[2], pad[1], pad[0],
+                                                  _stream_handle(stream), ctypes.byref(s)))
+        return _take_segments(s, True)
+    raise ValueError("expects a 2D or 3D tensor")

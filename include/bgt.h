@@ -99,6 +99,19 @@ int bgt_extract_segments2d(int dtype, const void *img, int width, int height, bg
  * (3D/ExtractSegments.h:21-22, 3D/ExtractSegments.cpp:135-251). */
 int bgt_extract_segments3d(int dtype, const void *vol, int width, int height, int depth,
                            bgt_segments *out);
+/* ExtractSegmentsV over a skeleton in device memory (no reference
+ * counterpart; the reference traces host arrays only, 2D/ExtractSegments.cpp:16-127,
+ * 3D/ExtractSegments.cpp:135-251).  d_skel holds width x height (x depth)
+ * uint8 values, nonzero = set, rows contiguous; it is the top-left corner of
+ * a zero-padded padded_width x padded_height (x padded_depth) frame, and the
+ * result equals the host tracer's on that frame.  The skeleton points are
+ * classified and compacted on the GPU; the host follows only them.  `stream`
+ * is a hipStream_t (NULL = default stream), synchronised before returning. */
+int bgt_extract_segments2d_device(const void *d_skel, int width, int height, int padded_width,
+                                  int padded_height, void *stream, bgt_segments *out);
+int bgt_extract_segments3d_device(const void *d_skel, int width, int height, int depth,
+                                  int padded_width, int padded_height, int padded_depth,
+                                  void *stream, bgt_segments *out);
 void bgt_segments_free(bgt_segments *s);
 
 /* ------------------------------------------------------------------------- */
