@@ -401,13 +401,11 @@ int bgt_centerline3d(int dtype, const void *vol, int width, int height, int dept
     const int pw = (width + 31) / 32 * 32, ph = (height + 31) / 32 * 32, pd = (depth + 31) / 32 * 32;
     if (pw > 32767 || ph > 32767 || pd > 32767)
         return fail(BGT_ERR_UNSUPPORTED, "padded size exceeds int16");
-    const size_t n = (size_t)width * height * depth;
-    std::vector<uint8_t> mask(n);
-    BGT_CHECK(threshold_int_positive(dtype, vol, n, mask.data()));
-    // thin on the device, then classify and compact the skeleton points there:
-    // the host traces only those, not the padded volume
+    // threshold and thin on the device, then classify and compact the skeleton
+    // points there: the host traces only those, not the padded volume
     const uint8_t *d_skel = nullptr;
-    BGT_CHECK(thin3d_mask_keep(width, height, depth, mask.data(), &d_skel, default_config3d()));
+    BGT_CHECK(thin3d_keep_int_positive(dtype, width, height, depth, vol, &d_skel,
+                                       default_config3d()));
     SegGraph g;
     BGT_CHECK(segment_graph_device(d_skel, width, height, depth, pw, ph, pd, true, nullptr, g));
     bgt_segments s;

@@ -22,11 +22,11 @@ int check_config3d(const Config3D &c);
 int thin3d_device_u8(const uint8_t *d_in, int w, int h, int d, uint8_t *d_out, hipStream_t s,
                      int *iterations, const Config3D &cfg);
 int thin3d_host(int dtype, int w, int h, int d, const void *in, void *out, const Config3D &cfg);
-// thins a host uint8 mask; the w x h x d skeleton stays on the device, in this
-// thread's I/O workspace (valid until the thread's next thinning call)
-int thin3d_mask_keep(int w, int h, int d, const uint8_t *mask, const uint8_t **d_skel,
-                     const Config3D &cfg);
-int threshold_int_positive(int dtype, const void *in, size_t n, uint8_t *mask);
+// thins the foreground int(v) > 0 of a host volume of any pixel type (the
+// 3D centerline's threshold); the w x h x d uint8 skeleton stays on the
+// device, in this thread's I/O workspace (valid until its next thinning call)
+int thin3d_keep_int_positive(int dtype, int w, int h, int d, const void *in,
+                             const uint8_t **d_skel, const Config3D &cfg);
 
 const Config3D &default_config3d();
 

@@ -39,11 +39,23 @@ __device__ __forceinline__ bool fg3d<bool>(bool v)
 // ---------------------------------------------------------------------------
 
 // Thinning3D<T> input -> uint8 0/1 (v > 0 in T), and back (0/1 in T).
+// 3D CenterlineExtraction's foreground: int(v) > 0 (3D/CenterlineExtraction.cpp:44).
+// Floating values outside int's range convert to INT_MIN on x86 and are
+// background here.
 template <typename T>
+__device__ __forceinline__ bool int_pos(T v)
+{
+    if (std::is_floating_point<T>::value) {
+        const double dv = (double)v;
+        return dv >= 1.0 && dv < 2147483648.0;
+    }
+    return (int)v > 0;
+}
+template <typename T, bool IntPos>
 __global__ void mask3d_from(const T *__restrict__ in, uint8_t *__restrict__ m, long n)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) m[t] = fg3d<T>(in[t]) ? 1 : 0;
+    if (t < n) m[t] = (IntPos ? int_pos<T>(in[t]) : fg3d<T>(in[t])) ? 1 : 0;
 }
 
 template <typename T>
@@ -204,7 +216,7 @@ int thin3d_device_u8(const uint8_t *d_in, int w, int h, int d, uint8_t *d_out, h
     return BGT_OK;
 }
 
-template <typename T>
+template <typename T, bool IntPos = false>
 static int thin3d_host_t(int w, int h, int d, const T *in, T *out, const Config3D &cfg,
                          const uint8_t **dev_mask = nullptr)
 {
@@ -221,7 +233,7 @@ static int thin3d_host_t(int w, int h, int d, const T *in, T *out, const Config3
     BGT_HIP(hipMemcpyAsync(io.ptr, in, bytes, hipMemcpyHostToDevice, s));
     uint8_t *mask = direct ? (uint8_t *)io.ptr : (uint8_t *)io.ptr + round_up(bytes, 256);
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
-    if (!direct) hipLaunchKernelGGL(mask3d_from<T>, g, b, 0, s, (const T *)io.ptr, mask, n);
+    if (!direct) hipLaunchKernelGGL((mask3d_from<T, IntPos>), g, b, 0, s, (const T *)io.ptr, mask, n);
     BGT_HIP(hipGetLastError());
     OneVolume ov;
     BGT_CHECK(one_volume_setup(w, h, d, mask, mask, thread_workspace(2), s, ov));
@@ -238,11 +250,22 @@ static int thin3d_host_t(int w, int h, int d, const T *in, T *out, const Config3
     return BGT_OK;
 }
 
-int thin3d_mask_keep(int w, int h, int d, const uint8_t *mask, const uint8_t **d_skel,
-                     const Config3D &cfg)
+int thin3d_keep_int_positive(int dtype, int w, int h, int d, const void *in,
+                             const uint8_t **d_skel, const Config3D &cfg)
 {
-    if (!mask || !d_skel) return fail(BGT_ERR_INVALID, "null pointer");
-    return thin3d_host_t<uint8_t>(w, h, d, mask, nullptr, cfg, d_skel);
+    if (!in || !d_skel) return fail(BGT_ERR_INVALID, "null pointer");
+    switch (dtype) {
+    case BGT_U8: return thin3d_host_t<uint8_t, true>(w, h, d, (const uint8_t *)in, nullptr, cfg, d_skel);
+    case BGT_I8: return thin3d_host_t<int8_t, true>(w, h, d, (const int8_t *)in, nullptr, cfg, d_skel);
+    case BGT_U16: return thin3d_host_t<uint16_t, true>(w, h, d, (const uint16_t *)in, nullptr, cfg, d_skel);
+    case BGT_I16: return thin3d_host_t<int16_t, true>(w, h, d, (const int16_t *)in, nullptr, cfg, d_skel);
+    case BGT_U32: return thin3d_host_t<uint32_t, true>(w, h, d, (const uint32_t *)in, nullptr, cfg, d_skel);
+    case BGT_I32: return thin3d_host_t<int32_t, true>(w, h, d, (const int32_t *)in, nullptr, cfg, d_skel);
+    case BGT_F32: return thin3d_host_t<float, true>(w, h, d, (const float *)in, nullptr, cfg, d_skel);
+    case BGT_F64: return thin3d_host_t<double, true>(w, h, d, (const double *)in, nullptr, cfg, d_skel);
+    case BGT_BOOL: return thin3d_host_t<uint8_t, true>(w, h, d, (const uint8_t *)in, nullptr, cfg, d_skel);
+    default: return fail(BGT_ERR_INVALID, "unknown pixel type %d", dtype);
+    }
 }
 
 int thin3d_host(int dtype, int w, int h, int d, const void *in, void *out, const Config3D &cfg)
@@ -260,40 +283,6 @@ int thin3d_host(int dtype, int w, int h, int d, const void *in, void *out, const
     case BGT_BOOL: return thin3d_host_t<bool>(w, h, d, (const bool *)in, (bool *)out, cfg);
     default: return fail(BGT_ERR_INVALID, "unknown pixel type %d", dtype);
     }
-}
-
-// 3D CenterlineExtraction conversion: foreground iff int(v) > 0
-// (3D/CenterlineExtraction.cpp:44).  Floating values outside int's range
-// convert to INT_MIN on x86 and are background here.
-template <typename T>
-static void thr_int_pos(const T *in, size_t n, uint8_t *m)
-{
-    for (size_t i = 0; i < n; ++i) {
-        const T v = in[i];
-        if (std::is_floating_point<T>::value) {
-            const double dv = (double)v;
-            m[i] = (dv >= 1.0 && dv < 2147483648.0) ? 1 : 0;
-        } else {
-            m[i] = (int)v > 0;
-        }
-    }
-}
-
-int threshold_int_positive(int dtype, const void *in, size_t n, uint8_t *mask)
-{
-    switch (dtype) {
-    case BGT_U8: thr_int_pos((const uint8_t *)in, n, mask); break;
-    case BGT_I8: thr_int_pos((const int8_t *)in, n, mask); break;
-    case BGT_U16: thr_int_pos((const uint16_t *)in, n, mask); break;
-    case BGT_I16: thr_int_pos((const int16_t *)in, n, mask); break;
-    case BGT_U32: thr_int_pos((const uint32_t *)in, n, mask); break;
-    case BGT_I32: thr_int_pos((const int32_t *)in, n, mask); break;
-    case BGT_F32: thr_int_pos((const float *)in, n, mask); break;
-    case BGT_F64: thr_int_pos((const double *)in, n, mask); break;
-    case BGT_BOOL: thr_int_pos((const uint8_t *)in, n, mask); break;
-    default: return fail(BGT_ERR_INVALID, "unknown pixel type %d", dtype);
-    }
-    return BGT_OK;
 }
 
 }  // namespace bgt

@@ -61,3 +61,29 @@ def test_extract_segments_on_gpu_skeleton():
     ref = O.extract2d(skel)
     for k in ("x", "y", "start", "is_end"):
         assert np.array_equal(got[k], ref[k]), k
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64, np.uint32, np.int16, np.int8, np.uint16])
+def test_centerline3d_pixel_types(dt):
+    """3D CenterlineExtraction keeps int(v) > 0 (3D/CenterlineExtraction.cpp:44),
+    evaluated on the GPU: fractions below one, negatives, NaN, infinities,
+    values past int's range and unsigned values that wrap negative are
+    background."""
+    import begputhinning_amd as B
+    from begputhinning_amd import synth
+    base = synth.tree3d(60, 50, 40, 99).astype(bool)
+    rng = np.random.default_rng(7)
+    if np.issubdtype(dt, np.floating):
+        pool = np.array([0.5, 0.99, 1.0, 1.5, 7.0, -3.0, np.nan, np.inf, -np.inf, 3e9, 2147483647.0])
+        vals = rng.choice(pool, size=base.shape).astype(dt)
+        mask = (vals.astype(np.float64) >= 1.0) & (vals.astype(np.float64) < 2147483648.0)
+    else:
+        info = np.iinfo(dt)
+        vals = rng.integers(info.min, int(info.max) + 1, size=base.shape, dtype=np.int64).astype(dt)
+        mask = vals.astype(np.int64).astype(np.int32) > 0 if dt == np.uint32 else vals > 0
+    data = np.where(base, vals, np.zeros((), dt)).astype(dt)
+    mask = mask & base
+    ref = O.centerline3d(mask.astype(np.uint8), 6, 3)
+    x, y, z, idx = B.CenterlineExtraction(data, 6, 3)
+    assert np.array_equal(idx, ref["idx"])
+    assert same_floats(x, ref["x"]) and same_floats(y, ref["y"]) and same_floats(z, ref["z"])
