@@ -1,6 +1,7 @@
 """Pins of the oracle's thinning drivers (or2d_thin / or3d_thin), the
 iteration semantics of 2D/CUDA_ThinningKernelMultiBlock2D.cuh:99-154 and
-3D/CUDA_ThinningKernel3D_Linear.cuh:690-782:
+3D/CUDA_ThinningKernel3D_Linear.cuh:690-782 (synchronous subiterations, zero
+outside the frame, the 3D protection mask taken at the iteration's start):
 
   * brute force: a per-pixel restatement -- every subiteration looks each
     pixel's 3x3 (3x3x3) neighbourhood up in the predicate tables (pinned on
