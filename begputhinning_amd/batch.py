@@ -47,7 +47,7 @@ class ThinningBatch:
         s = stream if stream is not None else torch.cuda.current_stream()
         check(lib().bgt_batch_run(self._h, ctypes.c_void_p(s.cuda_stream), 1 if use_graph else 0))
 
-    def autotune(self, stream=None, reps=2):
+    def autotune(self, stream=None, reps=3):
         """Pick the fastest launch shapes for this batch (the outputs hold a
         complete result afterwards); returns the chosen shapes and time."""
         import torch

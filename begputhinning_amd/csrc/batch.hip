@@ -1079,8 +1079,8 @@ int batch_set_config(Batch &b, const Config2D &c2, const Config3D &c3)
 // Every shape gives the same results, so only time is compared; shapes that
 // do not fit an item (a band thinner than its halo) are skipped.  Each
 // candidate is run once to lay the batch out and then `reps` times, taking
-// the fastest wall time of a whole run.  The batch's current shape is timed
-// first and is replaced only by a shape at least 3% faster, so that the
+// the median wall time of a whole run.  The batch's current shape is timed
+// first and is replaced only by a shape at least 5% faster, so that the
 // choice does not flip between near-equal shapes from run to run.
 int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D *best3,
                    double *best_ms, int *tried)
@@ -1102,16 +1102,19 @@ int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D 
             return;  // the current shape, already timed
         if (batch_set_config(b, t2, t3) != BGT_OK) return;
         if (batch_run(b, s, false) != BGT_OK) return;  // lays out; shape may not fit
-        double t = 1e30;
+        // the median of the runs: a shape's one lucky run must not win it
+        std::vector<double> ts;
         for (int r = 0; r < reps; ++r) {
             const auto t0 = std::chrono::steady_clock::now();
             if (batch_run(b, s, false) != BGT_OK) return;
             const auto t1 = std::chrono::steady_clock::now();
-            t = std::min(t, std::chrono::duration<double, std::milli>(t1 - t0).count());
+            ts.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
         }
+        std::sort(ts.begin(), ts.end());
+        const double t = ts.empty() ? 1e30 : ts[ts.size() / 2];
         const bool first = n == 0;
         ++n;
-        if (first || t < best * 0.97) {
+        if (first || t < best * 0.95) {
             best = t;
             c2 = t2;
             c3 = t3;

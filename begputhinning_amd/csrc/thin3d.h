@@ -10,7 +10,7 @@ namespace bgt {
 
 // Launch shape of the 3D word-list engine.
 struct Config3D {
-    int blocks_per_cu = 0;  // step-kernel workgroups per CU; 0 = occupancy limit
+    int blocks_per_cu = 2;  // step-kernel workgroups per CU; 0 = occupancy limit
     int max_chunk = 16;     // most iterations queued between two host polls
     // run the remaining iterations in one launch (sparse3d_tail, one
     // workgroup per volume) once at most this many words per volume are

@@ -178,7 +178,7 @@ def set_config2d(waves=4, rows=16, iters_per_pass=3, blocks_per_cu=2):
     check(lib().bgt_set_config2d(waves, rows, iters_per_pass, blocks_per_cu))
 
 
-def set_config3d(blocks_per_cu=0, max_chunk=16, tail_words=2048):
+def set_config3d(blocks_per_cu=2, max_chunk=16, tail_words=2048):
     """Launch shape of the 3D subiteration kernel for later calls: workgroups
     per CU (0 = occupancy limit), iterations queued per host poll, and the
     average active words per volume at which the remaining iterations run in

@@ -7,7 +7,8 @@ uint8 {0,1}, all resident in HBM before timing.  One step = thinning the whole
 batch to convergence (pack, passes, unpack) with one batched launch sequence.
 Reported beside it: the same items looped through the plain per-image
 drop-in path, p50/p99 latency of 1024x1024 tree frames through the per-image
-path, the dominant kernel's roofline, and the CPU baseline (the oracle, the
+path and of the drop-in CenterlineExtraction on the 1024x1024 frame and the
+256^3 volume, the dominant kernel's roofline, and the CPU baseline (the oracle, the
 reference's rules restated word-parallel in C, one core) on the whole
 workload, which also checks every skeleton bit for bit.
 
@@ -229,7 +230,7 @@ def main():
     tuned = None
     if not args.no_autotune:
         t_tune = time.perf_counter()
-        tuned = batch.autotune(stream, reps=2)  # untimed; results do not depend on it
+        tuned = batch.autotune(stream, reps=3)  # untimed; results do not depend on it
         log(f"autotune: {tuned} in {time.perf_counter() - t_tune:.2f}s")
     for _ in range(args.warmup):
         batch.run(stream, use_graph=bool(args.graph))
@@ -275,7 +276,7 @@ def main():
     log(f"kernels: {DOMINANT} {ms2d:.2f} ms / {l2d} launches, 3D iterations {ms3d:.2f} ms / "
         f"{l3d}; timed step {timed_step_ms:.2f} ms")
 
-    per_image = latency = None
+    per_image = latency = centerline = None
     outs_pi = None
     if not args.only_batch:
         # ---- per-image drop-in path on the same items -----------------------
@@ -310,6 +311,22 @@ def main():
         log(f"1024x1024 frame latency p50 {p50:.3f} ms p99 {p99:.3f} ms")
         latency = {"p50": round(p50, 3), "p99": round(p99, 3), "frames": args.frames,
                    "path": "per-image device-resident"}
+
+        # ---- the drop-in CenterlineExtraction on host arrays (C2, C3) -------
+        cl = {"path": "drop-in CenterlineExtraction(data, 10, 3) on host uint8 arrays: "
+                      "copy in, GPU thinning, GPU segment pre-pass, host trace/prune/smooth"}
+        for key, a, reps in [("2d_1024x1024", synth.workload2(), 40), ("3d_256^3", synth.workload3(), 8)]:
+            B.CenterlineExtraction(a, 10, 3)
+            ts = []
+            for _ in range(reps):
+                t3 = time.perf_counter()
+                B.CenterlineExtraction(a, 10, 3)
+                ts.append((time.perf_counter() - t3) * 1e3)
+            cl[key] = {"p50": round(float(np.percentile(ts, 50)), 3),
+                       "p99": round(float(np.percentile(ts, 99)), 3), "calls": reps}
+        log(f"CenterlineExtraction: 1024^2 p50 {cl['2d_1024x1024']['p50']} ms, "
+            f"256^3 p50 {cl['3d_256^3']['p50']} ms")
+        centerline = cl
 
     serving = None
     if args.serve_qps > 0:
@@ -374,6 +391,7 @@ def main():
                    "launch_shapes": tuned or "defaults"},
         "per_image": per_image,
         "latency_1024x1024_ms": latency,
+        "centerline_ms": centerline,
         "serving": serving,
         "bit_exact": {"vs_oracle": exact, "vs_committed_hashes": gold_ok,
                       "batched_equals_per_image": same_pi},

@@ -168,7 +168,8 @@ int bgt_batch_kernel_time(bgt_batch b, double *ms2d, int *launches2d, double *ms
  * its shapes, 2D/LibThinning2D.cu:59-60, 3D/LibThinning3D.cu:50): runs the
  * batch under candidate 2D shapes (waves, rows, iterations per pass,
  * workgroups per CU) and 3D step grids, `reps` timed runs each after one
- * layout run, keeps the fastest for the batch and reports it (any output
+ * layout run (their median counts), keeps the fastest for the batch and
+ * reports it (any output
  * pointer may be NULL).  Results do not depend on the shape.  The batch's
  * outputs hold a complete result afterwards. */
 int bgt_batch_autotune(bgt_batch b, void *stream, int reps, int *waves, int *rows,

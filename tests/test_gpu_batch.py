@@ -46,7 +46,7 @@ def test_3d_tail_in_batch(tail_words):
             synth.tree3d(130, 60, 50, 7, target_fraction=0.02)]
     ins = [torch.from_numpy(a).cuda() for a in vols]
     outs = [torch.full_like(t, 9) for t in ins]
-    B.set_config3d(0, 16, tail_words)
+    B.set_config3d(2, 16, tail_words)
     try:
         b = ThinningBatch()
         for t, o in zip(ins, outs):

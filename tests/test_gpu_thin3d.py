@@ -15,9 +15,9 @@ def _gpu():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     import begputhinning_amd as B
-    B.set_config3d(0, 16)
+    B.set_config3d()
     yield
-    B.set_config3d(0, 16)
+    B.set_config3d()
 
 
 def check(vol, label):
@@ -70,7 +70,7 @@ def test_tail_launch(tail_words):
     the first poll on; the result and iteration count stay the oracle's."""
     import begputhinning_amd as B
     from begputhinning_amd import synth
-    B.set_config3d(0, 16, tail_words)
+    B.set_config3d(2, 16, tail_words)
     try:
         check(synth.tree3d(70, 90, 80, 99), f"tail {tail_words}")
         check(synth.workload3(), f"tail {tail_words} 256^3")
@@ -80,7 +80,7 @@ def test_tail_launch(tail_words):
         B.set_config3d()
 
 
-@pytest.mark.parametrize("cfg", [(1, 2), (2, 4), (8, 64)])
+@pytest.mark.parametrize("cfg", [(1, 2), (2, 4), (8, 64), (0, 16)])
 def test_launch_shapes(cfg):
     import begputhinning_amd as B
     from begputhinning_amd import synth
@@ -88,7 +88,7 @@ def test_launch_shapes(cfg):
     try:
         check(synth.tree3d(70, 90, 80, 99), f"config {cfg}")
     finally:
-        B.set_config3d(0, 16)
+        B.set_config3d()
 
 
 @pytest.mark.parametrize("dt", [np.uint8, np.int16, np.float32, np.bool_])
