@@ -452,6 +452,15 @@ struct Batch {
 // arena); with size_only, only computes b.bytes (the sizing sweep sets the
 // device pointers from a null base, so a batch that is not ready is laid out
 // again by its next run).
+// Workgroups per CU of the batch's 2D passes: the configured count, or when
+// automatic two beside volumes (the 3D chain's kernels take the other slots)
+// and as many as fit in a batch of images alone.
+static int blocks2(const Batch &b)
+{
+    if (b.c2.blocks_per_cu >= 0) return b.c2.blocks_per_cu;
+    return b.in3.empty() ? 0 : 2;
+}
+
 static int prepare(Batch &b, hipStream_t s, bool size_only = false)
 {
     // host tables, uploaded together through pinned memory on the run stream
@@ -645,8 +654,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
         b.grid2 = occupancy_grid(b.c2.rows == 8 ? (const void *)thin2d_queue_pass<8>
                                                 : (const void *)thin2d_queue_pass<16>,
                                  b.c2.waves * 64, lds2,
-                                 (int)std::min<long>(b.ntiles2 * kParts, 1L << 30),
-                                 b.c2.blocks_per_cu);
+                                 (int)std::min<long>(b.ntiles2 * kParts, 1L << 30), blocks2(b));
     if (!b.h_live && hipHostMalloc((void **)&b.h_live, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(BGT_ERR_NOMEM, "pinned allocation failed");
     if (!b.h_rep) {
@@ -1090,7 +1098,7 @@ int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D 
     static const int k2[][3] = {{4, 16, 4}, {4, 16, 3}, {4, 16, 2}, {4, 16, 6}, {4, 8, 2},
                                 {4, 8, 3},  {4, 8, 4},  {2, 16, 3}, {2, 16, 4}, {4, 16, 8},
                                 {8, 8, 3},  {8, 8, 4},  {8, 8, 6},  {8, 16, 3}, {8, 16, 6}};
-    static const int kb2[] = {1, 2, 3, 0};
+    static const int kb2[] = {1, 2, 3, 4, 0};
     static const int k3[] = {0, 2, 4, 8, 16};
     Config2D c2 = b.c2;
     Config3D c3 = b.c3;
@@ -1121,10 +1129,12 @@ int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D 
         }
     };
     const Config3D start3 = b.c3;
-    trial(b.c2, start3);  // the current shape first
+    Config2D cur2 = b.c2;
+    cur2.blocks_per_cu = blocks2(b);  // an automatic count, as it resolves here
+    trial(cur2, start3);  // the current shape first
     if (!b.in2.empty()) {
         for (const auto &k : k2) {
-            Config2D t2 = b.c2;
+            Config2D t2 = cur2;
             t2.waves = k[0];
             t2.rows = k[1];
             t2.iters_per_pass = k[2];

@@ -16,7 +16,10 @@ struct Config2D {
     int waves = 4;           // wave64s per workgroup
     int rows = 16;           // register rows per lane (8 or 16); 16 fits every width
     int iters_per_pass = 3;  // K: iterations fused per HBM pass (halo 4K rows)
-    int blocks_per_cu = 2;   // batch queue workgroups per CU (0 = as many as fit)
+    // batch queue workgroups per CU: 0 = as many as fit, -1 = automatic (two in
+    // a batch that also holds volumes, whose chain needs the remaining slots,
+    // else as many as fit)
+    int blocks_per_cu = -1;
 };
 
 struct Plan2D {

@@ -232,7 +232,7 @@ int band_rows_for(int L, const Config2D &cfg)
 int check_config2d(const Config2D &cfg)
 {
     if (cfg.waves < 1 || cfg.waves > 8 || (cfg.rows != 8 && cfg.rows != 16) ||
-        cfg.iters_per_pass < 1 || cfg.iters_per_pass > 16 || cfg.blocks_per_cu < 0 ||
+        cfg.iters_per_pass < 1 || cfg.iters_per_pass > 16 || cfg.blocks_per_cu < -1 ||
         cfg.blocks_per_cu > 32)
         return fail(BGT_ERR_INVALID, "bad 2D config (waves=%d rows=%d k=%d blocks/CU=%d)", cfg.waves,
                     cfg.rows, cfg.iters_per_pass, cfg.blocks_per_cu);

@@ -171,10 +171,11 @@ def thin3d_device(vol, out=None, stream=None):
     return out, it.value
 
 
-def set_config2d(waves=4, rows=16, iters_per_pass=3, blocks_per_cu=2):
+def set_config2d(waves=4, rows=16, iters_per_pass=3, blocks_per_cu=-1):
     """Launch shape of the 2D pass kernel for later calls: waves per
     workgroup, register rows per lane, iterations per HBM pass and batch
-    workgroups per CU (0 = occupancy limit)."""
+    workgroups per CU (0 = occupancy limit, -1 = automatic: 2 in a batch
+    that also holds volumes, else the occupancy limit)."""
     check(lib().bgt_set_config2d(waves, rows, iters_per_pass, blocks_per_cu))
 
 

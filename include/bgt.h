@@ -45,7 +45,8 @@ int bgt_version(void);
  * 2D/LibThinning2D.cu:59-60, 3D/LibThinning3D.cu:50).  2D: waves per
  * workgroup (1-8), register rows per lane (8 or 16), iterations fused per
  * HBM pass (1-16), workgroups per CU of the batch pass kernel (0-32, 0 = the
- * occupancy limit).  3D: workgroups per CU of the subiteration kernel (0-32,
+ * occupancy limit, -1 = automatic, the default: 2 in a batch that also holds
+ * volumes, whose kernels take the other slots, else the occupancy limit).  3D: workgroups per CU of the subiteration kernel (0-32,
  * 0 = the occupancy limit), the most iterations queued between two host
  * convergence polls (1-64), and the average active words per volume at or
  * below which the remaining iterations run in one launch, one workgroup per

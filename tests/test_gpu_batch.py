@@ -183,13 +183,18 @@ def test_split_band_tail(cfg):
         B.set_config2d()
 
 
-@pytest.mark.parametrize("blocks_per_cu", [1, 3, 0])
-def test_pass_grid_sizes(blocks_per_cu):
+@pytest.mark.parametrize("blocks_per_cu,volumes", [(1, True), (3, True), (0, True), (-1, True),
+                                                   (-1, False), (4, False)])
+def test_pass_grid_sizes(blocks_per_cu, volumes):
     """The batch pass kernel pulls bands from a queue: any number of
-    workgroups per CU gives the same skeletons and iteration counts."""
+    workgroups per CU gives the same skeletons and iteration counts, also
+    the automatic count (-1), which differs between batches with and
+    without volumes."""
     import begputhinning_amd as B
     from begputhinning_amd.batch import ThinningBatch
     items = make_items(4, n2=16, n3=0)
+    if not volumes:
+        items = [a for a in items if a.ndim == 2]
     ins = [torch.from_numpy(a).cuda() for a in items]
     outs = [torch.full_like(t, 9) for t in ins]
     B.set_config2d(4, 16, 3, blocks_per_cu)
