@@ -448,10 +448,6 @@ struct Batch {
     }
 };
 
-// Lays the batch out in device memory (its own allocation, or the caller's
-// arena); with size_only, only computes b.bytes (the sizing sweep sets the
-// device pointers from a null base, so a batch that is not ready is laid out
-// again by its next run).
 // Workgroups per CU of the batch's 2D passes: the configured count, or when
 // automatic two beside volumes (the 3D chain's kernels take the other slots)
 // and as many as fit in a batch of images alone.
@@ -461,6 +457,10 @@ static int blocks2(const Batch &b)
     return b.in3.empty() ? 0 : 2;
 }
 
+// Lays the batch out in device memory (its own allocation, or the caller's
+// arena); with size_only, only computes b.bytes (the sizing sweep sets the
+// device pointers from a null base, so a batch that is not ready is laid out
+// again by its next run).
 static int prepare(Batch &b, hipStream_t s, bool size_only = false)
 {
     // host tables, uploaded together through pinned memory on the run stream
