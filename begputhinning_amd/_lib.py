@@ -119,3 +119,34 @@ def check(code):
         msg = lib().bgt_last_error()
         raise BGTError(code, msg.decode() if msg else "")
     return code
+
+
+def check_device_pair(inp, out, ndim, what):
+    """Validates a device-resident (input, output) pair before any kernel
+    runs: both uint8 tensors on the current GPU, `ndim` dimensions, the same
+    shape, rows contiguous (2D) or fully contiguous (3D).  The library writes
+    exactly w*h(*d) bytes through `out` at the given pitch, so a smaller,
+    differently typed or host `out` would be written past its end."""
+    import torch
+    for name, t in (("input", inp), ("out", out)):
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{what}: {name} must be a torch tensor")
+        if not t.is_cuda:
+            raise TypeError(f"{what}: {name} must be a GPU tensor (got device {t.device})")
+        if t.dtype != torch.uint8:
+            raise TypeError(f"{what}: {name} must be uint8 (got {t.dtype})")
+        if t.dim() != ndim:
+            raise ValueError(f"{what}: {name} must have {ndim} dimensions (got {t.dim()})")
+    if out.device != inp.device or inp.device.index != torch.cuda.current_device():
+        raise ValueError(f"{what}: input and out must be on the current GPU "
+                         f"(cuda:{torch.cuda.current_device()}), got {inp.device} and {out.device}")
+    if tuple(out.shape) != tuple(inp.shape):
+        raise ValueError(f"{what}: out shape {tuple(out.shape)} != input shape {tuple(inp.shape)}")
+    if ndim == 2:
+        for name, t in (("input", inp), ("out", out)):
+            if t.stride(1) != 1 or (t.shape[0] > 1 and t.stride(0) < t.shape[1]):
+                raise ValueError(f"{what}: {name} rows must be contiguous")
+    else:
+        for name, t in (("input", inp), ("out", out)):
+            if not t.is_contiguous():
+                raise ValueError(f"{what}: {name} must be contiguous")

@@ -4,7 +4,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, lib
+from ._lib import check, check_device_pair, lib
 
 
 class ThinningBatch:
@@ -25,8 +25,7 @@ class ThinningBatch:
         check(lib().bgt_batch_use_current_config(self._h))
 
     def add2d(self, img, out):
-        if img.dim() != 2 or img.stride(1) != 1 or out.stride(1) != 1:
-            raise ValueError("2D items need (h, w) tensors with contiguous rows")
+        check_device_pair(img, out, 2, "ThinningBatch.add2d")
         h, w = img.shape
         check(lib().bgt_batch_add2d(self._h, ctypes.c_void_p(img.data_ptr()), img.stride(0), w, h,
                                     ctypes.c_void_p(out.data_ptr()), out.stride(0)))
@@ -34,8 +33,7 @@ class ThinningBatch:
         self.n += 1
 
     def add3d(self, vol, out):
-        if vol.dim() != 3 or not vol.is_contiguous() or not out.is_contiguous():
-            raise ValueError("3D items need contiguous (d, h, w) tensors")
+        check_device_pair(vol, out, 3, "ThinningBatch.add3d")
         d, h, w = vol.shape
         check(lib().bgt_batch_add3d(self._h, ctypes.c_void_p(vol.data_ptr()), w, h, d,
                                     ctypes.c_void_p(out.data_ptr())))

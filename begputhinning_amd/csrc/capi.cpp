@@ -6,7 +6,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <sys/syscall.h>
+#include <unistd.h>
 #include <vector>
 
 #include "batch.h"
@@ -67,12 +70,38 @@ int workspace_reserve(Workspace &ws, size_t bytes)
     return BGT_OK;
 }
 
+bool on_main_thread() { return (long)syscall(SYS_gettid) == (long)getpid(); }
+
+Workspace::~Workspace()
+{
+    // at process exit the runtime may be going away: the main thread's memory
+    // is returned with the process
+    if (!ptr || on_main_thread()) return;
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != device) (void)hipSetDevice(device);
+    (void)hipFree(ptr);
+    if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+    ptr = nullptr;
+}
+
+PinnedBuffer::~PinnedBuffer()
+{
+    if (ptr && !on_main_thread()) (void)hipHostFree(ptr);
+    ptr = nullptr;
+}
+
 Workspace &thread_workspace(int slot)
 {
     // 0, 1: 2D state and I/O; 2, 3: 3D state and I/O; 5, 6: segment pre-pass
     thread_local Workspace ws[8];
     return ws[slot & 7];
 }
+
+// Process-wide launch shapes: read from the environment once (invalid values
+// are reported on stderr and the defaults kept), replaced by
+// bgt_set_config2d/3d, and handed to each entry point as a copy taken under
+// the lock, so a concurrent set_config never changes a running call's shape.
+static std::mutex g_config_mu;
 
 static Config2D &config2d_storage()
 {
@@ -82,12 +111,20 @@ static Config2D &config2d_storage()
         if (const char *e = std::getenv("BGT_2D_ROWS")) v.rows = std::atoi(e);
         if (const char *e = std::getenv("BGT_2D_K")) v.iters_per_pass = std::atoi(e);
         if (const char *e = std::getenv("BGT_2D_BLOCKS")) v.blocks_per_cu = std::atoi(e);
+        if (check_config2d(v) != BGT_OK) {
+            std::fprintf(stderr, "libbgt: ignoring BGT_2D_* environment: %s\n", g_error.c_str());
+            v = Config2D{};
+        }
         return v;
     }();
     return c;
 }
 
-const Config2D &default_config2d() { return config2d_storage(); }
+Config2D default_config2d()
+{
+    std::lock_guard<std::mutex> lk(g_config_mu);
+    return config2d_storage();
+}
 
 static Config3D &config3d_storage()
 {
@@ -96,12 +133,20 @@ static Config3D &config3d_storage()
         if (const char *e = std::getenv("BGT_3D_BLOCKS")) v.blocks_per_cu = std::atoi(e);
         if (const char *e = std::getenv("BGT_3D_CHUNK")) v.max_chunk = std::atoi(e);
         if (const char *e = std::getenv("BGT_3D_TAIL")) v.tail_words = std::atoi(e);
+        if (check_config3d(v) != BGT_OK) {
+            std::fprintf(stderr, "libbgt: ignoring BGT_3D_* environment: %s\n", g_error.c_str());
+            v = Config3D{};
+        }
         return v;
     }();
     return c;
 }
 
-const Config3D &default_config3d() { return config3d_storage(); }
+Config3D default_config3d()
+{
+    std::lock_guard<std::mutex> lk(g_config_mu);
+    return config3d_storage();
+}
 
 }  // namespace bgt
 
@@ -121,6 +166,8 @@ int bgt_set_config2d(int waves, int rows, int iters_per_pass, int blocks_per_cu)
     c.iters_per_pass = iters_per_pass;
     c.blocks_per_cu = blocks_per_cu;
     BGT_CHECK(check_config2d(c));
+    (void)default_config2d();  // environment read before the lock is taken
+    std::lock_guard<std::mutex> lk(g_config_mu);
     config2d_storage() = c;
     return BGT_OK;
 }
@@ -132,6 +179,8 @@ int bgt_set_config3d(int blocks_per_cu, int max_chunk, int tail_words)
     c.max_chunk = max_chunk;
     c.tail_words = tail_words;
     BGT_CHECK(check_config3d(c));
+    (void)default_config3d();
+    std::lock_guard<std::mutex> lk(g_config_mu);
     config3d_storage() = c;
     return BGT_OK;
 }
@@ -218,8 +267,16 @@ struct bgt_batch_s {
 int bgt_batch_create(bgt_batch *out)
 {
     if (!out) return fail(BGT_ERR_INVALID, "null pointer");
-    *out = new bgt_batch_s{batch_new()};
-    return batch_set_config(*(*out)->b, default_config2d(), default_config3d());
+    *out = nullptr;
+    bgt_batch h = new bgt_batch_s{batch_new()};
+    const int rc = batch_set_config(*h->b, default_config2d(), default_config3d());
+    if (rc != BGT_OK) {  // no handle escapes a failed create
+        batch_delete(h->b);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return BGT_OK;
 }
 
 int bgt_batch_use_current_config(bgt_batch b)

@@ -33,11 +33,21 @@ int fail(int code, const char *fmt, ...);
 // ---- device memory -------------------------------------------------------
 // Grow-only per-thread workspace used by the host-pointer drop-in entry points
 // so that repeated calls do not hipMalloc/hipFree every time.
+// A thread's workspaces and pinned poll buffers are released when the thread
+// exits (server workers, caller thread pools); the process's main thread
+// keeps them until the process ends.
 struct Workspace {
     void *ptr = nullptr;
     size_t bytes = 0;
     int device = -1;
+    ~Workspace();
 };
+// Pinned host memory of one thread (hipHostMalloc), released at thread exit.
+struct PinnedBuffer {
+    void *ptr = nullptr;
+    ~PinnedBuffer();
+};
+bool on_main_thread();
 int workspace_reserve(Workspace &ws, size_t bytes);
 Workspace &thread_workspace(int slot);
 

@@ -665,8 +665,9 @@ int sparse3d_layout(const int *w, const int *h, const int *d, int n, Layout3D &l
 
 int sparse3d_grid(int blocks_per_cu)
 {
-    static int cus = 0, occ = 0;
-    if (!cus) {
+    // device values, read once (a function-local static is initialised by
+    // exactly one thread; concurrent first calls wait for it)
+    static const int2 dev_cus_occ = [] {
         int dev = 0, c = 256, per_cu = 4;
         if (hipGetDevice(&dev) == hipSuccess) {
             (void)hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
@@ -674,10 +675,9 @@ int sparse3d_grid(int blocks_per_cu)
                                                              256, 0) != hipSuccess || per_cu < 1)
                 per_cu = 4;
         }
-        occ = std::max(1, per_cu);
-        cus = std::max(1, c);
-    }
-    return cus * (blocks_per_cu > 0 ? blocks_per_cu : occ);
+        return int2{std::max(1, c), std::max(1, per_cu)};
+    }();
+    return dev_cus_occ.x * (blocks_per_cu > 0 ? blocks_per_cu : dev_cus_occ.y);
 }
 
 int sparse3d_begin(const Sparse3D &e, bool zero, hipStream_t s)

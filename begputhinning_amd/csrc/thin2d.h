@@ -41,7 +41,7 @@ int thin2d_device_u8(const uint8_t *d_in, size_t in_pitch, int w, int h, uint8_t
                      size_t out_pitch, hipStream_t s, int *iterations, const Config2D &cfg);
 int thin2d_host(int dtype, int w, int h, const void *in, void *out, const Config2D &cfg);
 
-const Config2D &default_config2d();
+Config2D default_config2d();  // a snapshot of the current shape
 
 }  // namespace bgt
 

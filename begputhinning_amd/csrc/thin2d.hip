@@ -341,11 +341,11 @@ int setup_item2d(const Plan2D &pl, void *ws, hipStream_t s, Item2D **d_item, uns
 // Per-thread pinned word for the convergence poll.
 static int *poll_buffer()
 {
-    thread_local int *p = nullptr;
-    if (!p) {
-        if (hipHostMalloc((void **)&p, 64, hipHostMallocDefault) != hipSuccess) p = nullptr;
+    thread_local PinnedBuffer b;  // released when the thread exits
+    if (!b.ptr) {
+        if (hipHostMalloc(&b.ptr, 64, hipHostMallocDefault) != hipSuccess) b.ptr = nullptr;
     }
-    return p;
+    return (int *)b.ptr;
 }
 
 int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned *d_ctr,

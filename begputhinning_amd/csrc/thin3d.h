@@ -28,6 +28,6 @@ int thin3d_host(int dtype, int w, int h, int d, const void *in, void *out, const
 int thin3d_keep_int_positive(int dtype, int w, int h, int d, const void *in,
                              const uint8_t **d_skel, const Config3D &cfg);
 
-const Config3D &default_config3d();
+Config3D default_config3d();  // a snapshot of the current shape
 
 }  // namespace bgt

@@ -85,11 +85,11 @@ int check_config3d(const Config3D &c)
 
 static int *poll_buffer3()
 {
-    thread_local int *p = nullptr;
-    if (!p) {
-        if (hipHostMalloc((void **)&p, 256, hipHostMallocDefault) != hipSuccess) p = nullptr;
+    thread_local PinnedBuffer b;  // released when the thread exits
+    if (!b.ptr) {
+        if (hipHostMalloc(&b.ptr, 256, hipHostMallocDefault) != hipSuccess) b.ptr = nullptr;
     }
-    return p;
+    return (int *)b.ptr;
 }
 
 // Engine for one volume in a workspace: [item|geo|io] [states+ctl] [changed]

@@ -10,7 +10,7 @@ that produced them); outputs are complete when wait() returns.
 """
 import ctypes
 
-from ._lib import check, lib
+from ._lib import check, check_device_pair, lib
 
 
 class ThinningServer:
@@ -25,21 +25,19 @@ class ThinningServer:
         """Queue one image ((h, w), contiguous rows) or volume ((d, h, w),
         contiguous); returns a ticket."""
         t = ctypes.c_uint64(0)
-        if inp.dim() == 2:
-            if inp.stride(1) != 1 or out.stride(1) != 1:
-                raise ValueError("2D requests need tensors with contiguous rows")
+        ndim = inp.dim() if hasattr(inp, "dim") else -1
+        if ndim not in (2, 3):
+            raise ValueError("requests are 2D or 3D uint8 GPU tensors")
+        check_device_pair(inp, out, ndim, "ThinningServer.submit")
+        if ndim == 2:
             hh, w = inp.shape
             check(lib().bgt_server_submit2d(self._h, ctypes.c_void_p(inp.data_ptr()), inp.stride(0),
                                             w, hh, ctypes.c_void_p(out.data_ptr()), out.stride(0),
                                             ctypes.byref(t)))
-        elif inp.dim() == 3:
-            if not inp.is_contiguous() or not out.is_contiguous():
-                raise ValueError("3D requests need contiguous tensors")
+        else:
             d, hh, w = inp.shape
             check(lib().bgt_server_submit3d(self._h, ctypes.c_void_p(inp.data_ptr()), w, hh, d,
                                             ctypes.c_void_p(out.data_ptr()), ctypes.byref(t)))
-        else:
-            raise ValueError("requests are 2D or 3D tensors")
         self._keep[t.value] = (inp, out)
         return t.value
 

@@ -11,7 +11,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import DTYPES, Segments, check, lib
+from ._lib import DTYPES, Segments, check, check_device_pair, lib
 
 
 def _dtype_code(a):
@@ -142,13 +142,12 @@ def _stream_handle(stream):
 def thin2d_device(img, out=None, stream=None):
     """Thin a (h, w) uint8 CUDA tensor; returns (out, iterations)."""
     import torch
-    if img.dtype != torch.uint8 or img.dim() != 2 or not img.is_cuda:
+    if not isinstance(img, torch.Tensor) or img.dtype != torch.uint8 or img.dim() != 2 or not img.is_cuda:
         raise TypeError("thin2d_device expects a 2D uint8 GPU tensor")
-    if img.stride(1) != 1:
-        raise ValueError("rows must be contiguous")
     h, w = img.shape
     if out is None:
         out = torch.empty_like(img)
+    check_device_pair(img, out, 2, "thin2d_device")
     it = ctypes.c_int(0)
     check(lib().bgt_thin2d_device(ctypes.c_void_p(img.data_ptr()), img.stride(0), w, h,
                                   ctypes.c_void_p(out.data_ptr()), out.stride(0),
@@ -159,11 +158,12 @@ def thin2d_device(img, out=None, stream=None):
 def thin3d_device(vol, out=None, stream=None):
     """Thin a contiguous (d, h, w) uint8 CUDA tensor; returns (out, iterations)."""
     import torch
-    if vol.dtype != torch.uint8 or vol.dim() != 3 or not vol.is_cuda or not vol.is_contiguous():
+    if not isinstance(vol, torch.Tensor) or vol.dtype != torch.uint8 or vol.dim() != 3 or not vol.is_cuda:
         raise TypeError("thin3d_device expects a contiguous 3D uint8 GPU tensor")
     d, h, w = vol.shape
     if out is None:
         out = torch.empty_like(vol)
+    check_device_pair(vol, out, 3, "thin3d_device")
     it = ctypes.c_int(0)
     check(lib().bgt_thin3d_device(ctypes.c_void_p(vol.data_ptr()), w, h, d,
                                   ctypes.c_void_p(out.data_ptr()), _stream_handle(stream),

@@ -187,7 +187,10 @@ int bgt_batch_destroy(bgt_batch b);
 /* and batch engine), so requests arriving while others run overlap them on  */
 /* the device.  A result equals the single-item result whatever it ran with.  */
 /* Inputs must be ready on the device when submitted; outputs are complete    */
-/* when bgt_server_wait returns.                                              */
+/* when bgt_server_wait returns.  The budget bounds the engine memory of the  */
+/* requests in flight; each worker also keeps a grow-only arena (the largest  */
+/* batch it ran, plus a quarter) and per-thread workspaces until the server   */
+/* is destroyed, which are not counted against it.                            */
 /* ------------------------------------------------------------------------- */
 typedef struct bgt_server_s *bgt_server;
 

@@ -1,8 +1,8 @@
 """Segment tracing over skeletons in device memory: the GPU classifies and
 compacts the skeleton points, the host follows only them.  The segment lists
-must equal the host tracer's (itself pinned against the reference's compiled
-ExtractSegments.cpp) on the same zero-padded frame, bit for bit: coordinates,
-segment starts and end flags."""
+must equal the oracle's tracer (pinned against the reference's compiled
+ExtractSegments.cpp) and the library's host tracer on the same zero-padded
+frame, bit for bit: coordinates, segment starts and end flags."""
 import numpy as np
 import pytest
 
@@ -28,13 +28,16 @@ def padded(a, shape):
 def check(a, shape, label):
     import begputhinning_amd as B
     print(f"[seggraph] {label} {a.shape} -> {shape}", flush=True)
-    ref = B.extract_segments_flat(padded(a, shape))
+    frame = padded(a, shape)
+    oracle = O.extract2d(frame) if frame.ndim == 2 else O.extract3d(frame)
+    host = B.extract_segments_flat(frame)
     got = B.extract_segments_device(torch.from_numpy(np.ascontiguousarray(a)).cuda(), shape)
     for k in ("x", "y", "z", "start", "is_end"):
-        if ref[k] is None:
-            assert got[k] is None
+        if oracle[k] is None:
+            assert got[k] is None and host[k] is None
             continue
-        assert np.array_equal(ref[k], got[k]), f"{label}: {k} differs"
+        assert np.array_equal(oracle[k], got[k]), f"{label}: {k} differs from the oracle"
+        assert np.array_equal(oracle[k], host[k]), f"{label}: host tracer {k} differs from the oracle"
 
 
 @pytest.mark.parametrize("w,h,density", [(1, 1, 1.0), (3, 3, 1.0), (5, 9, 0.5), (64, 64, 0.2),

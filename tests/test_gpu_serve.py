@@ -116,3 +116,39 @@ def test_concurrent_submitters():
     for a, o in zip(items, outs):
         ref, _ = O.thin2d(a) if a.ndim == 2 else O.thin3d(a)
         assert np.array_equal(o.cpu().numpy(), ref)
+
+
+def test_server_memory_returned_on_destroy():
+    """Workers release their arenas, per-thread workspaces and pinned buffers
+    when the server is destroyed: device free memory is the same after each
+    create/submit/destroy cycle (single requests take the per-item path, so
+    the workers' thread workspaces are exercised too)."""
+    from begputhinning_amd import synth
+    from begputhinning_amd.serve import ThinningServer
+    items = [synth.tree3d(160, 150, 140, 41), synth.polylines(900, 700, 42),
+             synth.tree3d(60, 50, 40, 43), synth.polylines(300, 200, 44)]
+    ins = [torch.from_numpy(a).cuda() for a in items]
+    outs = [torch.empty_like(t) for t in ins]
+    torch.cuda.synchronize()
+
+    def cycle():
+        srv = ThinningServer(budget_bytes=4 << 30, workers2d=2, workers3d=2)
+        try:
+            for t, o in zip(ins, outs):  # one at a time: the per-item path
+                srv.wait(srv.submit(t, o))
+            tks = [srv.submit(t, o) for t, o in zip(ins, outs)]  # together: a batch
+            for tk in tks:
+                srv.wait(tk)
+        finally:
+            srv.close()
+        torch.cuda.synchronize()
+        return torch.cuda.mem_get_info()[0]
+
+    free = [cycle() for _ in range(4)]
+    print(f"[serve] free bytes after each cycle: {free}", flush=True)
+    # the first cycle may leave the runtime's own pools grown; later cycles
+    # must not lose memory
+    assert max(free[1:]) - min(free[1:]) < 64 << 20, free
+    for a, o in zip(items, outs):
+        ref, _ = O.thin2d(a) if a.ndim == 2 else O.thin3d(a)
+        assert np.array_equal(o.cpu().numpy(), ref)
