@@ -52,6 +52,8 @@ SIGNATURES = {
     "bgt_thinning3d": (_i, [_i, _i, _i, _i, _vp, _vp]),
     "bgt_thin2d_device": (_i, [_vp, _sz, _i, _i, _vp, _sz, _vp, _ip]),
     "bgt_thin3d_device": (_i, [_vp, _i, _i, _i, _vp, _vp, _ip]),
+    "bgt_thin2d_device_noalloc": (_i, [_vp, _sz, _i, _i, _vp, _sz, _vp, _ip]),
+    "bgt_thin2d_noalloc_max_bytes": (_sz, []),
     "bgt_extract_segments2d": (_i, [_i, _vp, _i, _i, ctypes.POINTER(Segments)]),
     "bgt_extract_segments3d": (_i, [_i, _vp, _i, _i, _i, ctypes.POINTER(Segments)]),
     "bgt_extract_segments2d_device": (_i, [_vp, _i, _i, _i, _i, _vp, ctypes.POINTER(Segments)]),

@@ -197,6 +197,15 @@ int bgt_thin2d_device(const uint8_t *d_in, size_t in_pitch, int width, int heigh
                             (hipStream_t)stream, iterations, default_config2d());
 }
 
+int bgt_thin2d_device_noalloc(const uint8_t *d_in, size_t in_pitch, int width, int height,
+                              uint8_t *d_out, size_t out_pitch, void *stream, int *iterations)
+{
+    return thin2d_device_u8_noalloc(d_in, in_pitch, width, height, d_out, out_pitch,
+                                    (hipStream_t)stream, iterations, default_config2d());
+}
+
+size_t bgt_thin2d_noalloc_max_bytes(void) { return thin2d_noalloc_max_bytes(); }
+
 int bgt_thinning3d(int dtype, int width, int height, int depth, const void *in, void *out)
 {
     return thin3d_host(dtype, width, height, depth, in, out, default_config3d());

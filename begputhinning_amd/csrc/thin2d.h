@@ -34,12 +34,17 @@ int band_rows_for(int L, const Config2D &cfg);
 size_t workspace_bytes2d(const Plan2D &pl);
 int setup_item2d(const Plan2D &pl, void *ws, hipStream_t s, Item2D **d_item, unsigned **d_ctr,
                  Item2D *host_item);
+// host_poll (may be null: a per-thread pinned buffer) receives ST_INTS words
 int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned *d_ctr,
-                 const int *d_state, hipStream_t s, int *iterations);
+                 const int *d_state, hipStream_t s, int *iterations, int *host_poll = nullptr);
 
 int thin2d_device_u8(const uint8_t *d_in, size_t in_pitch, int w, int h, uint8_t *d_out,
                      size_t out_pitch, hipStream_t s, int *iterations, const Config2D &cfg);
 int thin2d_host(int dtype, int w, int h, const void *in, void *out, const Config2D &cfg);
+// thin2d_device_u8 on the library's fixed per-GPU engine array: no allocation
+int thin2d_device_u8_noalloc(const uint8_t *d_in, size_t in_pitch, int w, int h, uint8_t *d_out,
+                             size_t out_pitch, hipStream_t s, int *iterations, const Config2D &cfg);
+size_t thin2d_noalloc_max_bytes();
 
 Config2D default_config2d();  // a snapshot of the current shape
 

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -349,9 +350,9 @@ static int *poll_buffer()
 }
 
 int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned *d_ctr,
-                 const int *d_state, hipStream_t s, int *iterations)
+                 const int *d_state, hipStream_t s, int *iterations, int *host_poll)
 {
-    int *hp = poll_buffer();
+    int *hp = host_poll ? host_poll : poll_buffer();
     if (!hp) return fail(BGT_ERR_NOMEM, "pinned poll buffer allocation failed");
     const long max_pass = (long)pl.w * pl.h / cfg.iters_per_pass + 2;  // each changing iteration deletes >= 1 px
     long launched = 0;
@@ -392,6 +393,73 @@ int thin2d_device_u8(const uint8_t *d_in, size_t in_pitch, int w, int h, uint8_t
                        in_pitch, w, h, pl.L, hv.buf[0]);
     BGT_HIP(hipGetLastError());
     BGT_CHECK(run_passes2d(cfg, pl, d_item, d_ctr, hv.st, s, iterations));
+    launch_unpack2d_u8(d_item, d_out, w, h, out_pitch, s);
+    BGT_HIP(hipGetLastError());
+    BGT_HIP(hipStreamSynchronize(s));
+    return BGT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device-resident thinning without allocation
+// ---------------------------------------------------------------------------
+
+// Engine memory of the allocation-free entry point: one array per GPU, held
+// until the process ends.  Host code never uses its name as an address (on
+// the host it is a stand-in): the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+constexpr size_t kScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+static int scratch_address(void **out)
+{
+    static std::mutex mu;
+    static void *ptrs[64] = {};
+    int dev = 0;
+    BGT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(BGT_ERR_UNSUPPORTED, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptrs[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(BGT_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        ptrs[dev] = p;
+    }
+    *out = ptrs[dev];
+    return BGT_OK;
+}
+
+size_t thin2d_noalloc_max_bytes() { return kScratchBytes; }
+
+int thin2d_device_u8_noalloc(const uint8_t *d_in, size_t in_pitch, int w, int h, uint8_t *d_out,
+                             size_t out_pitch, hipStream_t s, int *iterations, const Config2D &cfg)
+{
+    if (!d_in || !d_out) return fail(BGT_ERR_INVALID, "null device pointer");
+    if (in_pitch < (size_t)w || out_pitch < (size_t)w)
+        return fail(BGT_ERR_INVALID, "pitch smaller than width");
+    Plan2D pl;
+    BGT_CHECK(plan2d(w, h, cfg, pl));
+    const size_t need = workspace_bytes2d(pl);
+    if (need > kScratchBytes)
+        return fail(BGT_ERR_UNSUPPORTED,
+                    "%d x %d image needs %zu bytes of engine memory; the allocation-free entry "
+                    "point holds %zu",
+                    w, h, need, kScratchBytes);
+    void *ws = nullptr;
+    BGT_CHECK(scratch_address(&ws));
+    Item2D *d_item;
+    unsigned *d_ctr;
+    Item2D hv;
+    // the init kernel clears the item state, pass counter and band flags that
+    // an earlier call left in the array; the first pass writes every row of
+    // the other state buffer
+    BGT_CHECK(setup_item2d(pl, ws, s, &d_item, &d_ctr, &hv));
+    const long nw = (long)pl.L * h;
+    hipLaunchKernelGGL(pack2d_u8, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, d_in,
+                       in_pitch, w, h, pl.L, hv.buf[0]);
+    BGT_HIP(hipGetLastError());
+    int poll[ST_INTS];  // pageable: the poll copies synchronise anyway
+    BGT_CHECK(run_passes2d(cfg, pl, d_item, d_ctr, hv.st, s, iterations, poll));
     launch_unpack2d_u8(d_item, d_out, w, h, out_pitch, s);
     BGT_HIP(hipGetLastError());
     BGT_HIP(hipStreamSynchronize(s));

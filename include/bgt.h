@@ -82,6 +82,20 @@ int bgt_thin2d_device(const uint8_t *d_in, size_t in_pitch, int width, int heigh
 int bgt_thin3d_device(const uint8_t *d_in, int width, int height, int depth, uint8_t *d_out,
                       void *stream, int *iterations);
 
+/* bgt_thin2d_device without any allocation (no counterpart in the reference,
+ * which allocates on every call, 2D/LibThinning2D.cu:73-75).  For callers
+ * that cannot allocate device memory: code under stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  The engine state lives in
+ * one fixed array per GPU of bgt_thin2d_noalloc_max_bytes() bytes (16 MiB,
+ * held until the process ends); an image whose state needs more
+ * (2 * ceil(width / 64) * height * 8 bytes plus a few KiB) is rejected with
+ * BGT_ERR_UNSUPPORTED.  Same results as bgt_thin2d_device.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (from several
+ * threads or streams) race and must be serialised by the caller. */
+int bgt_thin2d_device_noalloc(const uint8_t *d_in, size_t in_pitch, int width, int height,
+                              uint8_t *d_out, size_t out_pitch, void *stream, int *iterations);
+size_t bgt_thin2d_noalloc_max_bytes(void);
+
 /* ------------------------------------------------------------------------- */
 /* Segment tracing (host), ExtractSegmentsV.                                  */
 /* ------------------------------------------------------------------------- */
