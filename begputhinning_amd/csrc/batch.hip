@@ -38,6 +38,7 @@
 #include "thin3d_kernels.h"
 #include "pack.h"
 #include "sparse3d.h"
+#include "tile3d.h"
 
 namespace bgt {
 
@@ -352,6 +353,7 @@ struct Batch {
     };
     std::vector<In2> in2;
     std::vector<In3> in3;
+    std::vector<int> order;  // the items in the order added: 2D index, or ~3D index
 
     bool ready = false;
     bool ran = false;  // a run completed since the layout was made
@@ -378,7 +380,11 @@ struct Batch {
     hipGraphExec_t graph2 = nullptr;
     hipStream_t graph2_stream = nullptr;
 
-    // 3D pipeline: non-empty word list engine
+    // 3D pipeline: the non-empty word list engine and/or the tile engine
+    // (Config3D::engine)
+    Tile3D t3;
+    int titer3 = 0;        // tile engine: next iteration to queue
+    bool on_tiles3 = false;  // this run's 3D chain is on the tile engine
     Sparse3D e3;
     bool zeroed3 = false;  // state margins cleared since allocation
     int bound3 = 0;        // upper bound of the current 3D list's count
@@ -491,6 +497,12 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     Layout3D lay3;
     if (n3) BGT_CHECK(sparse3d_layout(w3v.data(), h3v.data(), d3v.data(), n3, lay3));
     const long nwords3 = lay3.nwords, data3 = lay3.data_words;
+    // engine 0: word lists (one-launch tail); 1: word lists, then tiles; 2: tiles
+    const bool words3 = n3 > 0 && b.c3.engine <= 1;
+    const bool tiles3 = n3 > 0 && b.c3.engine >= 1;
+    std::vector<TileItem3D> titems3;
+    const long ntiles3 = tiles3 ? tile3d_items(lay3, titems3) : 0;
+    if (ntiles3 >= 0x7fffffffL) return fail(BGT_ERR_UNSUPPORTED, "3D tiles exceed 2^31");
     const size_t lds2 = (size_t)2 * 2 * b.c2.waves * 64 * sizeof(uint64_t);
 
     std::vector<Tile2D> t2;
@@ -544,35 +556,55 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
         SIO *d_io3 = cv.take<SIO>(sizeof(SIO) * n3);
         int *d_blk3 = cv.take<int>(sizeof(int) * lay3.blk.size());
         int *d_ublk3 = cv.take<int>(sizeof(int) * lay3.ublk.size());
-        e.items = d_items3;
-        e.geo = d_geo3;
-        e.io = d_io3;
-        e.blk = d_blk3;
-        e.nblk = (int)lay3.blk.size();
-        e.ublk = d_ublk3;
-        e.nublk = (int)lay3.ublk.size();
-        e.state[0] = cv.take<uint32_t>(sizeof(uint32_t) * nwords3);
-        e.state[1] = cv.take<uint32_t>(sizeof(uint32_t) * nwords3);
-        e.nwords = nwords3;
-        e.ctl = cv.take<int>(sizeof(int) * SC_INTS);
-        for (int k = 0; k < 2; ++k) {
-            e.list[k].item = cv.take<int>(sizeof(int) * data3);
-            e.list[k].word = cv.take<uint32_t>(sizeof(uint32_t) * data3);
-            e.list[k].blk = cv.take<int>(sizeof(int) * data3);
-            e.list[k].count = e.ctl ? e.ctl + SC_COUNT0 + k : nullptr;
+        if (words3) {
+            e.items = d_items3;
+            e.geo = d_geo3;
+            e.io = d_io3;
+            e.blk = d_blk3;
+            e.nblk = (int)lay3.blk.size();
+            e.ublk = d_ublk3;
+            e.nublk = (int)lay3.ublk.size();
+            e.state[0] = cv.take<uint32_t>(sizeof(uint32_t) * nwords3);
+            e.state[1] = cv.take<uint32_t>(sizeof(uint32_t) * nwords3);
+            e.nwords = nwords3;
+            e.ctl = cv.take<int>(sizeof(int) * SC_INTS);
+            for (int k = 0; k < 2; ++k) {
+                e.list[k].item = cv.take<int>(sizeof(int) * data3);
+                e.list[k].word = cv.take<uint32_t>(sizeof(uint32_t) * data3);
+                e.list[k].blk = cv.take<int>(sizeof(int) * data3);
+                e.list[k].count = e.ctl ? e.ctl + SC_COUNT0 + k : nullptr;
+            }
+            e.act.item = cv.take<int>(sizeof(int) * data3);
+            e.act.word = cv.take<uint32_t>(sizeof(uint32_t) * data3);
+            e.act.blk = cv.take<int>(sizeof(int) * data3);
+            e.act.count = nullptr;  // in ctl, by iteration parity
+            e.prot = cv.take<uint32_t>(sizeof(uint32_t) * data3);
+            e.first = cv.take<uint32_t>(sizeof(uint32_t) * data3);
+            e.capacity = (int)data3;
+            e.nblocks = (int)lay3.nblocks;
+            e.bstamp = cv.take<int>(sizeof(int) * lay3.nblocks);
+            e.dstamp = cv.take<int>(sizeof(int) * lay3.nblocks);
+            e.states = cv.take<int>(sizeof(int) * (size_t)n3 * ST_INTS);
+            e.changed = cv.take<int>(sizeof(int) * n3);
         }
-        e.act.item = cv.take<int>(sizeof(int) * data3);
-        e.act.word = cv.take<uint32_t>(sizeof(uint32_t) * data3);
-        e.act.blk = cv.take<int>(sizeof(int) * data3);
-        e.act.count = nullptr;  // in ctl, by iteration parity
-        e.prot = cv.take<uint32_t>(sizeof(uint32_t) * data3);
-        e.first = cv.take<uint32_t>(sizeof(uint32_t) * data3);
-        e.capacity = (int)data3;
-        e.nblocks = (int)lay3.nblocks;
-        e.bstamp = cv.take<int>(sizeof(int) * lay3.nblocks);
-        e.dstamp = cv.take<int>(sizeof(int) * lay3.nblocks);
-        e.states = cv.take<int>(sizeof(int) * (size_t)n3 * ST_INTS);
-        e.changed = cv.take<int>(sizeof(int) * n3);
+        TileItem3D *d_titems3 = nullptr;
+        b.t3 = Tile3D{};
+        if (tiles3) {
+            // the tile engine takes over the word engine's state arrays
+            // (engine 1) or has its own (engine 2)
+            d_titems3 = cv.take<TileItem3D>(sizeof(TileItem3D) * n3);
+            Tile3D &t = b.t3;
+            uint32_t *const *shared = words3 ? e.state : nullptr;
+            char *tb = cv.take<char>(tile3d_carve(t, nullptr, n3, nwords3, ntiles3, lay3.nblocks, shared));
+            if (tb) tile3d_carve(t, tb, n3, nwords3, ntiles3, lay3.nblocks, shared);
+            t.items = d_titems3;
+            t.sitems = d_items3;
+            t.io = d_io3;
+            t.blk = d_blk3;
+            t.nblk = (int)lay3.blk.size();
+            t.ublk = d_ublk3;
+            t.nublk = (int)lay3.ublk.size();
+        }
         // 2D control words: [0..2] queue counts, [3..5] next slots, [6] pass base
         int *ctl = b.d_ctl;
         for (int k = 0; k < 3; ++k) b.q2.list[k] = lists2 ? lists2 + k * b.ntiles2 : nullptr;
@@ -637,6 +669,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
             stage(d_io3, io3.data(), sizeof(SIO) * n3);
             stage(d_blk3, lay3.blk.data(), sizeof(int) * lay3.blk.size());
             stage(d_ublk3, lay3.ublk.data(), sizeof(int) * lay3.ublk.size());
+            if (tiles3) stage(d_titems3, titems3.data(), sizeof(TileItem3D) * n3);
         }
     }
     if (host_stage.size() > b.h_stage_bytes) {
@@ -861,7 +894,12 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         hipLaunchKernelGGL(batch_build_list, dim3((unsigned)((b.ntiles2 + 255) / 256)), dim3(256),
                            0, s, b.q2);
     BGT_HIP(hipGetLastError());
-    if (n3) {
+    b.on_tiles3 = n3 > 0 && b.c3.engine == 2;
+    if (b.on_tiles3) {
+        BGT_CHECK(tile3d_begin(b.t3, !b.zeroed3, s3));
+        b.zeroed3 = true;
+        b.titer3 = 0;
+    } else if (n3) {
         BGT_CHECK(sparse3d_begin(b.e3, !b.zeroed3, s3));
         b.zeroed3 = true;
         b.bound3 = b.abound3 = b.e3.capacity;
@@ -921,7 +959,21 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     auto enqueue3 = [&](Chain &c) -> int {
         const int slot = (c.head + c.inflight) & 1;
         int *hs = b.h_live + slot * 32;
-        if (b.c3.tail_words > 0 && b.iter3 >= 1 &&
+        if (b.on_tiles3) {
+            // a chunk of iterations, one launch each; iteration t closes the
+            // items whose iteration t - 1 changed nothing
+            const int grid = tile3d_grid(b.c3.blocks_per_cu);
+            size_t mk = 0;
+            BGT_CHECK(timed_mark(b, s3, 1, true, &mk));
+            for (int i = 0; i < kChunk; ++i) BGT_CHECK(tile3d_iteration(b.t3, b.titer3++, grid, s3));
+            BGT_CHECK(timed_mark(b, s3, 1, false, &mk));
+            BGT_HIP(hipMemcpyAsync(hs + 8, b.t3.ctl, TC_INTS * sizeof(int), hipMemcpyDeviceToHost, s3));
+            BGT_HIP(hipEventRecord(b.ev_poll3[slot], s3));
+            ++c.inflight;
+            if (++c.chunks > max_chunks) return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
+            return BGT_OK;
+        }
+        if (b.c3.engine == 0 && b.c3.tail_words > 0 && b.iter3 >= 1 &&
             (long)b.abound3 <= (long)b.c3.tail_words * n3) {
             // few active words per volume left: the remaining iterations in one
             // launch, one workgroup per volume
@@ -942,6 +994,12 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     // reads the oldest pending poll (its event has completed)
     auto consume3 = [&](Chain &c) {
         const int *hp = b.h_live + c.head * 32;
+        if (b.on_tiles3) {
+            if (hp[8 + TC_LIVE] <= 0) c.live = false;
+            c.head ^= 1;
+            --c.inflight;
+            return;
+        }
         // chunks hold an even number of iterations: list 0 is current
         b.bound3 = hp[8 + SC_COUNT0];
         b.abound3 = hp[8 + SC_ACOUNT0 + (hp[8 + SC_ITER] & 1)];
@@ -956,7 +1014,17 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     // both.
     auto drive3 = [&](Chain &c) -> int {
         while (c.live) {
-            if (c.inflight < 2 && !c.final) {
+            // engine 1: once few words are active, the tile engine takes over
+            // after the chunks in flight (the host then knows the iteration)
+            const bool to_tiles = b.c3.engine == 1 && !b.on_tiles3 && b.c3.tail_words > 0 &&
+                                  b.iter3 >= 1 && (long)b.abound3 <= (long)b.c3.tail_words * n3;
+            if (to_tiles && c.inflight == 0) {
+                BGT_CHECK(tile3d_from_words(b.t3, b.e3, b.iter3, b.bound3, s3));
+                b.on_tiles3 = true;
+                b.titer3 = b.iter3;
+                continue;
+            }
+            if (c.inflight < 2 && !c.final && !to_tiles) {
                 BGT_CHECK(enqueue3(c));
                 continue;
             }
@@ -983,7 +1051,7 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     auto run3 = [&]() -> int {
         if (!n3) return BGT_OK;
         BGT_CHECK(drive3(c3));
-        BGT_CHECK(sparse3d_end(b.e3, s3));
+        BGT_CHECK(b.on_tiles3 ? tile3d_end(b.t3, s3) : sparse3d_end(b.e3, s3));
         if (fork) BGT_HIP(hipEventRecord(b.ev_join, s3));
         return BGT_OK;
     };
@@ -1023,7 +1091,7 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         BGT_HIP(hipMemcpyAsync(b.h_states, b.d_states2, (size_t)n2 * ST_INTS * sizeof(int),
                                hipMemcpyDeviceToHost, s));
     if (n3)
-        BGT_HIP(hipMemcpyAsync(b.h_states + (size_t)n2 * ST_INTS, b.e3.states,
+        BGT_HIP(hipMemcpyAsync(b.h_states + (size_t)n2 * ST_INTS, b.on_tiles3 ? b.t3.states : b.e3.states,
                                (size_t)n3 * ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
     b.ran = true;
@@ -1041,8 +1109,13 @@ void batch_kernel_time(const Batch &b, double *ms2d, int *n2d, double *ms3d, int
 int batch_iterations(Batch &b, int *out)
 {
     if (!b.ready || !b.ran) return fail(BGT_ERR_INVALID, "batch has not run");
-    const int n = (int)(b.in2.size() + b.in3.size());
-    for (int i = 0; i < n; ++i) out[i] = b.h_states[(size_t)i * ST_INTS + ST_ITERS];
+    // the state copy holds the 2D items, then the 3D items
+    const size_t n2 = b.in2.size();
+    for (size_t k = 0; k < b.order.size(); ++k) {
+        const int o = b.order[k];
+        const size_t i = o >= 0 ? (size_t)o : n2 + (size_t)~o;
+        out[k] = b.h_states[i * ST_INTS + ST_ITERS];
+    }
     return BGT_OK;
 }
 
@@ -1057,6 +1130,7 @@ int batch_add2d(Batch &b, const uint8_t *in, size_t in_pitch, int w, int h, uint
         return fail(BGT_ERR_INVALID, "bad 2D item (%d x %d)", w, h);
     if (w > 4096) return fail(BGT_ERR_UNSUPPORTED, "2D width %d exceeds 4096", w);
     b.release();
+    b.order.push_back((int)b.in2.size());
     b.in2.push_back({{in, in_pitch, out, out_pitch}, w, h});
     return BGT_OK;
 }
@@ -1066,6 +1140,7 @@ int batch_add3d(Batch &b, const uint8_t *in, int w, int h, int d, uint8_t *out)
     if (!in || !out) return fail(BGT_ERR_INVALID, "null device pointer");
     if (w <= 0 || h <= 0 || d <= 0) return fail(BGT_ERR_INVALID, "bad 3D item");
     b.release();
+    b.order.push_back(~(int)b.in3.size());
     b.in3.push_back({{in, out}, w, h, d});
     return BGT_OK;
 }
@@ -1185,6 +1260,7 @@ void batch_clear(Batch &b)
     b.release();
     b.in2.clear();
     b.in3.clear();
+    b.order.clear();
 }
 
 int batch_size(const Batch &b) { return (int)(b.in2.size() + b.in3.size()); }

@@ -133,6 +133,7 @@ static Config3D &config3d_storage()
         if (const char *e = std::getenv("BGT_3D_BLOCKS")) v.blocks_per_cu = std::atoi(e);
         if (const char *e = std::getenv("BGT_3D_CHUNK")) v.max_chunk = std::atoi(e);
         if (const char *e = std::getenv("BGT_3D_TAIL")) v.tail_words = std::atoi(e);
+        if (const char *e = std::getenv("BGT_3D_ENGINE")) v.engine = std::atoi(e);
         if (check_config3d(v) != BGT_OK) {
             std::fprintf(stderr, "libbgt: ignoring BGT_3D_* environment: %s\n", g_error.c_str());
             v = Config3D{};
@@ -174,12 +175,21 @@ int bgt_set_config2d(int waves, int rows, int iters_per_pass, int blocks_per_cu)
 
 int bgt_set_config3d(int blocks_per_cu, int max_chunk, int tail_words)
 {
-    Config3D c;
+    Config3D c = default_config3d();  // the engine choice is kept
     c.blocks_per_cu = blocks_per_cu;
     c.max_chunk = max_chunk;
     c.tail_words = tail_words;
     BGT_CHECK(check_config3d(c));
-    (void)default_config3d();
+    std::lock_guard<std::mutex> lk(g_config_mu);
+    config3d_storage() = c;
+    return BGT_OK;
+}
+
+int bgt_set_engine3d(int engine)
+{
+    Config3D c = default_config3d();
+    c.engine = engine;
+    BGT_CHECK(check_config3d(c));
     std::lock_guard<std::mutex> lk(g_config_mu);
     config3d_storage() = c;
     return BGT_OK;

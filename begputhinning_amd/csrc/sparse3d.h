@@ -32,6 +32,10 @@
 
 namespace bgt {
 
+// 64-voxel runs each thread of a pack grid handles (the pack block tables of
+// sparse3d_layout give every item whole 256-thread blocks of such threads)
+constexpr int kPackRuns3D = 8;
+
 struct SItem3D {
     long base;     // first word of the item's padded region
     int w, h, d;   // voxels

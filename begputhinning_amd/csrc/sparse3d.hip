@@ -36,7 +36,7 @@ namespace bgt {
 namespace {
 
 const int kCompactPer = 1024;  // list entries per compaction workgroup (4 per thread)
-const int kPackRuns = 8;       // 64-voxel runs per pack thread
+const int kPackRuns = kPackRuns3D;
 
 // Appends the flagged entries (item, word[j], blk[j]) of a workgroup to
 // `out`, up to N per thread, keeping their order, with one atomic per

@@ -12,10 +12,15 @@ namespace bgt {
 struct Config3D {
     int blocks_per_cu = 2;  // step-kernel workgroups per CU; 0 = occupancy limit
     int max_chunk = 16;     // most iterations queued between two host polls
-    // run the remaining iterations in one launch (sparse3d_tail, one
-    // workgroup per volume) once at most this many words per volume are
-    // active on average (0 = never)
-    int tail_words = 2048;
+    // once at most this many words per volume are active on average, the
+    // remaining iterations switch to the tile engine (engine 1) or run in one
+    // launch, one workgroup per volume (engine 0); 0 = never
+    int tail_words = 16384;
+    // 0: the non-empty-word list engine (sparse3d.h, six step launches per
+    // iteration) with its one-launch tail; 1: that engine, then the tile
+    // engine (tile3d.h, one launch per iteration) from tail_words on; 2: the
+    // tile engine from the first iteration
+    int engine = 1;
 };
 
 int check_config3d(const Config3D &c);

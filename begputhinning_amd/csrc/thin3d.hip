@@ -18,6 +18,7 @@
 #include "thin3d_kernels.h"
 #include "pack.h"
 #include "sparse3d.h"
+#include "tile3d.h"
 
 namespace bgt {
 
@@ -77,9 +78,10 @@ __global__ void init_one3d(SItem3D *item, int2 *geo, SIO *io, SItem3D v, SIO f)
 int check_config3d(const Config3D &c)
 {
     if (c.blocks_per_cu < 0 || c.blocks_per_cu > 32 || c.max_chunk < 1 || c.max_chunk > 64 ||
-        c.tail_words < 0)
-        return fail(BGT_ERR_INVALID, "bad 3D config (blocks_per_cu=%d max_chunk=%d tail_words=%d)",
-                    c.blocks_per_cu, c.max_chunk, c.tail_words);
+        c.tail_words < 0 || c.engine < 0 || c.engine > 2)
+        return fail(BGT_ERR_INVALID,
+                    "bad 3D config (blocks_per_cu=%d max_chunk=%d tail_words=%d engine=%d)",
+                    c.blocks_per_cu, c.max_chunk, c.tail_words, c.engine);
     return BGT_OK;
 }
 
@@ -98,7 +100,14 @@ static int *poll_buffer3()
 struct OneVolume {
     Sparse3D e;
     SItem3D item;
+    Tile3D t;            // the tile engine that takes over (engine 1)
+    bool tiles = false;
 };
+
+__global__ void init_one3d_titem(TileItem3D *titem, TileItem3D tv)
+{
+    if (threadIdx.x == 0) *titem = tv;
+}
 
 static size_t one_volume_bytes(long nwords, long data, long nblocks)
 {
@@ -108,12 +117,18 @@ static size_t one_volume_bytes(long nwords, long data, long nblocks)
 }
 
 static int one_volume_setup(int w, int h, int d, const uint8_t *in, uint8_t *out, Workspace &ws,
-                            hipStream_t s, OneVolume &ov)
+                            hipStream_t s, OneVolume &ov, bool with_tiles)
 {
     Layout3D lay;
     BGT_CHECK(sparse3d_layout(&w, &h, &d, 1, lay));
     const long nwords = lay.nwords, data = lay.data_words;
-    BGT_CHECK(workspace_reserve(ws, one_volume_bytes(nwords, data, lay.nblocks)));
+    std::vector<TileItem3D> ti;
+    const long ntiles = with_tiles ? tile3d_items(lay, ti) : 0;
+    uint32_t *no_state[2] = {nullptr, nullptr};  // sizing only: the word engine's arrays are shared
+    Tile3D sizing;
+    const size_t tile_bytes =
+        with_tiles ? 256 + tile3d_carve(sizing, nullptr, 1, nwords, ntiles, lay.nblocks, no_state) : 0;
+    BGT_CHECK(workspace_reserve(ws, one_volume_bytes(nwords, data, lay.nblocks) + tile_bytes));
     char *p = (char *)ws.ptr;
     auto take = [&](size_t bytes) {
         char *r = p;
@@ -158,8 +173,41 @@ static int one_volume_setup(int w, int h, int d, const uint8_t *in, uint8_t *out
     ov.item = lay.items[0];
     hipLaunchKernelGGL(init_one3d, dim3(1), dim3(64), 0, s, d_item, d_geo, d_io, lay.items[0],
                        SIO{in, out, 0, 0});
+    ov.tiles = with_tiles;
+    if (with_tiles) {
+        TileItem3D *d_titem = (TileItem3D *)take(sizeof(TileItem3D));
+        Tile3D &t = ov.t;
+        t = Tile3D{};
+        tile3d_carve(t, p, 1, nwords, ntiles, lay.nblocks, e.state);
+        t.items = d_titem;
+        t.sitems = d_item;
+        t.io = d_io;
+        t.blk = nullptr;
+        t.nblk = e.nblk;
+        t.ublk = nullptr;
+        t.nublk = e.nublk;
+        hipLaunchKernelGGL(init_one3d_titem, dim3(1), dim3(64), 0, s, d_titem, ti[0]);
+    }
     BGT_HIP(hipGetLastError());
     return BGT_OK;
+}
+
+// The tile engine's iterations from t on (it has taken over from the word
+// engine), polling item 0 after chunks of 2, 4, ... up to max_chunk.  The
+// launch of iteration t closes the item when iteration t - 1 changed nothing.
+static int tiles_to_convergence(const Tile3D &e, int t, long max_iter, const Config3D &cfg,
+                                hipStream_t s, int *hp)
+{
+    const int grid = tile3d_grid(cfg.blocks_per_cu);
+    int chunk = std::min(2, cfg.max_chunk);
+    for (;;) {
+        for (int i = 0; i < chunk; ++i) BGT_CHECK(tile3d_iteration(e, t++, grid, s));
+        BGT_HIP(hipMemcpyAsync(hp, e.states, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
+        BGT_HIP(hipStreamSynchronize(s));
+        if (hp[ST_DONE]) return BGT_OK;
+        if (t > max_iter) return fail(BGT_ERR_NOT_CONVERGED, "3D thinning did not converge");
+        chunk = std::min(chunk * 2, cfg.max_chunk);
+    }
 }
 
 // Iterate one volume to convergence, polling the item state every chunk of
@@ -187,7 +235,13 @@ static int one_volume_run(OneVolume &ov, const Config3D &cfg, hipStream_t s, int
         if (done_iters > max_iter) return fail(BGT_ERR_NOT_CONVERGED, "3D thinning did not converge");
         bound = hp[ST_INTS + SC_COUNT0 + cur];
         abound = hp[ST_INTS + SC_ACOUNT0 + (hp[ST_INTS + SC_ITER] & 1)];
-        if (cfg.tail_words > 0 && abound <= cfg.tail_words) {
+        if (ov.tiles && cfg.tail_words > 0 && abound <= cfg.tail_words && !(done_iters & 1)) {
+            // few active words left: the tile engine, one launch per iteration
+            BGT_CHECK(tile3d_from_words(ov.t, e, (int)done_iters, bound, s));
+            BGT_CHECK(tiles_to_convergence(ov.t, (int)done_iters, max_iter, cfg, s, hp));
+            break;
+        }
+        if (!ov.tiles && cfg.tail_words > 0 && abound <= cfg.tail_words) {
             // few active words left: the remaining iterations in one launch
             BGT_CHECK(sparse3d_tail(e, s));
             BGT_HIP(hipMemcpyAsync(hp, e.states, (ST_INTS + SC_INTS) * sizeof(int),
@@ -202,6 +256,97 @@ static int one_volume_run(OneVolume &ov, const Config3D &cfg, hipStream_t s, int
     return sparse3d_end(e, s);
 }
 
+// ---------------------------------------------------------------------------
+// per-volume path on the tile engine
+// ---------------------------------------------------------------------------
+
+// One volume in a workspace: [sitem|titem|io] [engine: states, tiles, lists,
+// state arrays].
+struct OneVolumeTiles {
+    Tile3D e;
+    SItem3D item;
+};
+
+__global__ void init_one3d_tiles(SItem3D *item, TileItem3D *titem, SIO *io, SItem3D v, TileItem3D tv,
+                                 SIO f)
+{
+    if (threadIdx.x == 0) {
+        *item = v;
+        *titem = tv;
+        *io = f;
+    }
+}
+
+static int one_volume_tiles_setup(int w, int h, int d, const uint8_t *in, uint8_t *out,
+                                  Workspace &ws, hipStream_t s, OneVolumeTiles &ov)
+{
+    Layout3D lay;
+    BGT_CHECK(sparse3d_layout(&w, &h, &d, 1, lay));
+    std::vector<TileItem3D> ti;
+    const long ntiles = tile3d_items(lay, ti);
+    Tile3D &e = ov.e;
+    e = Tile3D{};
+    const size_t head = 3 * 256;
+    const size_t need = head + tile3d_carve(e, nullptr, 1, lay.nwords, ntiles, lay.nblocks);
+    BGT_CHECK(workspace_reserve(ws, need));
+    char *p = (char *)ws.ptr;
+    SItem3D *d_item = (SItem3D *)p;
+    TileItem3D *d_titem = (TileItem3D *)(p + 256);
+    SIO *d_io = (SIO *)(p + 512);
+    tile3d_carve(e, p + head, 1, lay.nwords, ntiles, lay.nblocks);
+    e.items = d_titem;
+    e.sitems = d_item;
+    e.io = d_io;
+    e.blk = nullptr;  // every block belongs to item 0
+    e.nblk = (int)lay.blk.size();
+    e.ublk = nullptr;
+    e.nublk = (int)lay.ublk.size();
+    ov.item = lay.items[0];
+    hipLaunchKernelGGL(init_one3d_tiles, dim3(1), dim3(64), 0, s, d_item, d_titem, d_io, lay.items[0],
+                       ti[0], SIO{in, out, 0, 0});
+    BGT_HIP(hipGetLastError());
+    return BGT_OK;
+}
+
+// Iterate one volume to convergence, polling the item state after chunks of
+// 2, 4, ... up to max_chunk iterations.  The launch of iteration t closes the
+// item when iteration t - 1 changed nothing, so a poll after iterations
+// 0 .. t - 1 sees every convergence up to iteration t - 2.
+static int one_volume_tiles_run(OneVolumeTiles &ov, const Config3D &cfg, hipStream_t s,
+                                int *iterations)
+{
+    const Tile3D &e = ov.e;
+    BGT_CHECK(tile3d_begin(e, true, s));
+    int *hp = poll_buffer3();
+    if (!hp) return fail(BGT_ERR_NOMEM, "pinned poll buffer allocation failed");
+    const long max_iter = (long)ov.item.w * ov.item.h * ov.item.d + 3;
+    const int grid = tile3d_grid(cfg.blocks_per_cu);
+    int t = 0, chunk = std::min(2, cfg.max_chunk);
+    for (;;) {
+        for (int i = 0; i < chunk; ++i) BGT_CHECK(tile3d_iteration(e, t++, grid, s));
+        BGT_HIP(hipMemcpyAsync(hp, e.states, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
+        BGT_HIP(hipStreamSynchronize(s));
+        if (hp[ST_DONE]) break;
+        if (t > max_iter) return fail(BGT_ERR_NOT_CONVERGED, "3D thinning did not converge");
+        chunk = std::min(chunk * 2, cfg.max_chunk);
+    }
+    if (iterations) *iterations = hp[ST_ITERS];
+    return tile3d_end(e, s);
+}
+
+static int thin3d_one(const uint8_t *in, int w, int h, int d, uint8_t *out, hipStream_t s,
+                      int *iterations, const Config3D &cfg)
+{
+    if (cfg.engine == 2) {
+        OneVolumeTiles ov;
+        BGT_CHECK(one_volume_tiles_setup(w, h, d, in, out, thread_workspace(2), s, ov));
+        return one_volume_tiles_run(ov, cfg, s, iterations);
+    }
+    OneVolume ov;
+    BGT_CHECK(one_volume_setup(w, h, d, in, out, thread_workspace(2), s, ov, cfg.engine == 1));
+    return one_volume_run(ov, cfg, s, iterations);
+}
+
 int thin3d_device_u8(const uint8_t *d_in, int w, int h, int d, uint8_t *d_out, hipStream_t s,
                      int *iterations, const Config3D &cfg)
 {
@@ -209,9 +354,7 @@ int thin3d_device_u8(const uint8_t *d_in, int w, int h, int d, uint8_t *d_out, h
     if (!d_in || !d_out) return fail(BGT_ERR_INVALID, "null device pointer");
     if (w <= 0 || h <= 0 || d <= 0)
         return fail(BGT_ERR_INVALID, "3D size must be >= 1 (got %d x %d x %d)", w, h, d);
-    OneVolume ov;
-    BGT_CHECK(one_volume_setup(w, h, d, d_in, d_out, thread_workspace(2), s, ov));
-    BGT_CHECK(one_volume_run(ov, cfg, s, iterations));
+    BGT_CHECK(thin3d_one(d_in, w, h, d, d_out, s, iterations, cfg));
     BGT_HIP(hipStreamSynchronize(s));
     return BGT_OK;
 }
@@ -235,9 +378,7 @@ static int thin3d_host_t(int w, int h, int d, const T *in, T *out, const Config3
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
     if (!direct) hipLaunchKernelGGL((mask3d_from<T, IntPos>), g, b, 0, s, (const T *)io.ptr, mask, n);
     BGT_HIP(hipGetLastError());
-    OneVolume ov;
-    BGT_CHECK(one_volume_setup(w, h, d, mask, mask, thread_workspace(2), s, ov));
-    BGT_CHECK(one_volume_run(ov, cfg, s, nullptr));
+    BGT_CHECK(thin3d_one(mask, w, h, d, mask, s, nullptr, cfg));
     if (dev_mask) {  // the skeleton stays in this thread's I/O workspace
         *dev_mask = mask;
         BGT_HIP(hipStreamSynchronize(s));

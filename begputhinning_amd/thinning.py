@@ -179,12 +179,19 @@ def set_config2d(waves=4, rows=16, iters_per_pass=3, blocks_per_cu=-1):
     check(lib().bgt_set_config2d(waves, rows, iters_per_pass, blocks_per_cu))
 
 
-def set_config3d(blocks_per_cu=2, max_chunk=16, tail_words=2048):
+def set_config3d(blocks_per_cu=2, max_chunk=16, tail_words=16384):
     """Launch shape of the 3D subiteration kernel for later calls: workgroups
     per CU (0 = occupancy limit), iterations queued per host poll, and the
     average active words per volume at which the remaining iterations run in
     one launch, one workgroup per volume (0 = never)."""
     check(lib().bgt_set_config3d(blocks_per_cu, max_chunk, tail_words))
+
+
+def set_engine3d(engine=1):
+    """3D engine for later calls (results do not depend on it): 0 = word
+    lists with a one-launch tail, 1 = word lists then tiles from tail_words
+    on, 2 = tiles throughout."""
+    check(lib().bgt_set_engine3d(engine))
 
 
 def extract_segments_device(skel, padded=None, stream=None):

@@ -54,6 +54,12 @@ int bgt_version(void);
  * created afterwards. */
 int bgt_set_config2d(int waves, int rows, int iters_per_pass, int blocks_per_cu);
 int bgt_set_config3d(int blocks_per_cu, int max_chunk, int tail_words);
+/* 3D engine (no reference counterpart; results do not depend on it): 0 = the
+ * non-empty-word lists, six step launches per iteration, finishing in one
+ * launch per batch; 1 (default) = the word lists, then from tail_words on the
+ * tiles, one launch per iteration with the whole iteration staged in LDS;
+ * 2 = the tiles from the first iteration. */
+int bgt_set_engine3d(int engine);
 
 /* ------------------------------------------------------------------------- */
 /* Drop-in thinning on host buffers.                                          */
