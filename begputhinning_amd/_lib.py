@@ -63,6 +63,7 @@ SIGNATURES = {
     "bgt_centerline2d": (_i, [_i, _vp, _i, _i, _i, _i, _fpp, _fpp, _upp, _up]),
     "bgt_centerline3d": (_i, [_i, _vp, _i, _i, _i, _i, _i, _fpp, _fpp, _fpp, _upp, _up]),
     "bgt_free": (None, [_vp]),
+    "bgt_mex_columns": (_i, [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp]),
     "bgt_synth_item_seed": (ctypes.c_uint64, [_i, _i]),
     "bgt_synth_polylines": (_i, [_vp, _i, _i, ctypes.c_uint64, _i, ctypes.c_double]),
     "bgt_synth_tree2d": (_i, [_vp, _i, _i, ctypes.c_uint64, _i, ctypes.c_double, ctypes.c_double]),

@@ -279,6 +279,12 @@ void bgt_segments_free(bgt_segments *s)
 
 void bgt_free(void *p) { std::free(p); }
 
+int bgt_mex_columns(const float *x, const float *y, const float *z, const uint32_t *out_idx,
+                    uint32_t nsegs, double *out)
+{
+    return mex_columns(x, y, z, out_idx, nsegs, out);
+}
+
 struct bgt_batch_s {
     bgt::Batch *b;
 };

@@ -12,9 +12,14 @@
 // organised: the first pass scans 64-bit words and skips empty ones, and the
 // list and coordinates are kept in flat arrays.
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -243,12 +248,131 @@ int extract_segments(int dtype, const void *data, int w, int h, int d, bool thre
 
 // ---- prune, smooth, flatten (2D/CenterlineExtraction.cpp:51-131) ----------
 
+namespace {
+
+// A few host threads kept for the process's lifetime, so that a call's
+// parallel section costs a wake-up rather than thread creation.  run(n, fn)
+// calls fn(i) for i in [0, n) on the pool and the calling thread, and returns
+// when all have finished.  Calls are serialised (one parallel section at a
+// time).
+class HostPool {
+  public:
+    static HostPool &get()
+    {
+        static HostPool p;
+        return p;
+    }
+    int threads() const { return (int)workers_.size() + 1; }
+    void run(int n, const std::function<void(int)> &fn)
+    {
+        std::lock_guard<std::mutex> one(call_mu_);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fn_ = &fn;
+            n_ = n;
+            next_.store(0);
+            busy_ = (int)workers_.size();
+            ++gen_;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return busy_ == 0; });
+        fn_ = nullptr;
+    }
+    ~HostPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+            ++gen_;
+        }
+        cv_.notify_all();
+        for (auto &t : workers_) t.join();
+    }
+
+  private:
+    HostPool()
+    {
+        const unsigned hc = std::thread::hardware_concurrency();
+        const int nw = (int)std::min<unsigned>(7u, hc > 1 ? hc - 1 : 0u);
+        for (int k = 0; k < nw; ++k) workers_.emplace_back([this] { loop(); });
+    }
+    void drain()
+    {
+        for (int i; (i = next_.fetch_add(1)) < n_;) (*fn_)(i);
+    }
+    void loop()
+    {
+        unsigned seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (stop_) return;
+            }
+            drain();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex mu_, call_mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)> *fn_ = nullptr;
+    std::atomic<int> next_{0};
+    int n_ = 0, busy_ = 0;
+    unsigned gen_ = 0;
+    bool stop_ = false;
+};
+
+// The moving average of one segment (2D/CenterlineExtraction.cpp:60-104):
+// a window that grows from the first point, slides, then shrinks into the
+// last point; float32 running sums in the reference's order of additions and
+// subtractions.  Writes n values of each coordinate from X, Y (, Z).
+void smooth_segment(const int16_t *px, const int16_t *py, const int16_t *pz, int n, int span_cap,
+                    float *X, float *Y, float *Z)
+{
+    const int span = std::min(span_cap, (n - 1) / 2);
+    float ax = px[0], ay = py[0], az = pz ? pz[0] : 0.0f;
+    int cnt = 1, o = 0;
+    X[o] = (float)px[0]; Y[o] = (float)py[0]; if (Z) Z[o] = (float)pz[0];
+    ++o;
+    for (int j = 1; j <= span; ++j) {
+        ax += px[2 * j - 1]; ay += py[2 * j - 1]; if (pz) az += pz[2 * j - 1];
+        ax += px[2 * j];     ay += py[2 * j];     if (pz) az += pz[2 * j];
+        cnt += 2;
+        X[o] = ax / (float)cnt; Y[o] = ay / (float)cnt; if (Z) Z[o] = az / (float)cnt;
+        ++o;
+    }
+    for (int j = span + 1; j < n - span; ++j) {
+        ax += px[j + span]; ay += py[j + span]; if (pz) az += pz[j + span];
+        ax -= px[j - span - 1]; ay -= py[j - span - 1]; if (pz) az -= pz[j - span - 1];
+        X[o] = ax / (float)cnt; Y[o] = ay / (float)cnt; if (Z) Z[o] = az / (float)cnt;
+        ++o;
+    }
+    for (int j = n - span, dd = span; j < n; ++j, --dd) {
+        ax -= px[j - dd - 1]; ay -= py[j - dd - 1]; if (pz) az -= pz[j - dd - 1];
+        ax -= px[j - dd];     ay -= py[j - dd];     if (pz) az -= pz[j - dd];
+        cnt -= 2;
+        X[o] = ax / (float)cnt; Y[o] = ay / (float)cnt; if (Z) Z[o] = az / (float)cnt;
+        ++o;
+    }
+}
+
+// points at or above which the segments are smoothed on the host pool
+constexpr size_t kParallelPoints = size_t(1) << 14;
+
+}  // namespace
+
 int centerline_from_segments(const bgt_segments &s, bool three_d, int max_segment_length,
                              int smoothing_span, float **ox, float **oy, float **oz,
                              uint32_t **oidx, uint32_t *nsegs)
 {
     const size_t lim = (size_t)(long)max_segment_length;  // negative -> huge, as in the reference
     const int span_cap = smoothing_span < 0 ? 0 : smoothing_span;
+    // prune (2D/CenterlineExtraction.cpp:51-57): short end segments go
     std::vector<int> keep;
     size_t npts = 0;
     for (int i = 0; i < s.nseg; ++i) {
@@ -265,47 +389,58 @@ int centerline_from_segments(const bgt_segments &s, bool three_d, int max_segmen
         std::free(X); std::free(Y); std::free(Z); std::free(I);
         return fail(BGT_ERR_NOMEM, "centerline output allocation failed");
     }
+    // flatten (:107-131): each kept segment's points start where the previous
+    // one's end, so the segments are independent and smoothed in parallel
     size_t o = 0;
     for (size_t k = 0; k < keep.size(); ++k) {
+        I[k] = (uint32_t)o;
+        o += s.seg_start[keep[k] + 1] - s.seg_start[keep[k]];
+    }
+    I[keep.size()] = (uint32_t)npts;
+    auto one = [&](size_t k) {
         const int sidx = keep[k];
         const int st = (int)s.seg_start[sidx];
         const int n = (int)(s.seg_start[sidx + 1] - s.seg_start[sidx]);
-        const int16_t *px = s.x + st, *py = s.y + st, *pz = three_d ? s.z + st : nullptr;
-        I[k] = (uint32_t)o;
-        // moving average with a window that grows from the first point, slides,
-        // then shrinks into the last point; float32 running sums in the
-        // reference's order of additions and subtractions
-        const int span = std::min(span_cap, (n - 1) / 2);
-        float ax = px[0], ay = py[0], az = pz ? pz[0] : 0.0f;
-        int cnt = 1;
-        X[o] = (float)px[0]; Y[o] = (float)py[0]; if (Z) Z[o] = (float)pz[0];
-        ++o;
-        for (int j = 1; j <= span; ++j) {
-            ax += px[2 * j - 1]; ay += py[2 * j - 1]; if (pz) az += pz[2 * j - 1];
-            ax += px[2 * j];     ay += py[2 * j];     if (pz) az += pz[2 * j];
-            cnt += 2;
-            X[o] = ax / (float)cnt; Y[o] = ay / (float)cnt; if (Z) Z[o] = az / (float)cnt;
-            ++o;
-        }
-        for (int j = span + 1; j < n - span; ++j) {
-            ax += px[j + span]; ay += py[j + span]; if (pz) az += pz[j + span];
-            ax -= px[j - span - 1]; ay -= py[j - span - 1]; if (pz) az -= pz[j - span - 1];
-            X[o] = ax / (float)cnt; Y[o] = ay / (float)cnt; if (Z) Z[o] = az / (float)cnt;
-            ++o;
-        }
-        for (int j = n - span, dd = span; j < n; ++j, --dd) {
-            ax -= px[j - dd - 1]; ay -= py[j - dd - 1]; if (pz) az -= pz[j - dd - 1];
-            ax -= px[j - dd];     ay -= py[j - dd];     if (pz) az -= pz[j - dd];
-            cnt -= 2;
-            X[o] = ax / (float)cnt; Y[o] = ay / (float)cnt; if (Z) Z[o] = az / (float)cnt;
-            ++o;
-        }
+        smooth_segment(s.x + st, s.y + st, three_d ? s.z + st : nullptr, n, span_cap, X + I[k],
+                       Y + I[k], Z ? Z + I[k] : nullptr);
+    };
+    HostPool &pool = HostPool::get();
+    if (npts >= kParallelPoints && pool.threads() > 1) {
+        // chunks of about npts / (4 threads) points, whole segments each
+        const int nchunks = pool.threads() * 4;
+        std::vector<size_t> bounds(1, 0);
+        const size_t per = npts / (size_t)nchunks + 1;
+        for (size_t k = 0; k < keep.size(); ++k)
+            if (I[k + 1] >= per * bounds.size()) bounds.push_back(k + 1);
+        if (bounds.back() != keep.size()) bounds.push_back(keep.size());
+        pool.run((int)bounds.size() - 1, [&](int c) {
+            for (size_t k = bounds[c]; k < bounds[c + 1]; ++k) one(k);
+        });
+    } else {
+        for (size_t k = 0; k < keep.size(); ++k) one(k);
     }
-    I[keep.size()] = (uint32_t)npts;
     *ox = X; *oy = Y;
     if (oz) *oz = Z;
     *oidx = I;
     *nsegs = (uint32_t)keep.size();
+    return BGT_OK;
+}
+
+int mex_columns(const float *x, const float *y, const float *z, const uint32_t *idx,
+                uint32_t nsegs, double *out)
+{
+    if (!x || !y || !idx || (!out && idx[nsegs] > 0)) return fail(BGT_ERR_INVALID, "null pointer");
+    const uint32_t nc = z ? 3 : 2;
+    for (uint32_t i = 0; i < nsegs; ++i) {
+        if (idx[i + 1] < idx[i]) return fail(BGT_ERR_INVALID, "segment starts are not ascending");
+        const uint32_t n = idx[i + 1] - idx[i];
+        double *blk = out + (size_t)idx[i] * nc;  // this segment's n x nc matrix, column-major
+        for (uint32_t t = 0; t < n; ++t) {
+            blk[t] = (double)y[idx[i] + t];
+            blk[n + t] = (double)x[idx[i] + t];
+            if (z) blk[2 * n + t] = (double)z[idx[i] + t];
+        }
+    }
     return BGT_OK;
 }
 

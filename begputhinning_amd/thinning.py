@@ -217,3 +217,26 @@ def extract_segments_device(skel, padded=None, stream=None):
                                                   _stream_handle(stream), ctypes.byref(s)))
         return _take_segments(s, True)
     raise ValueError("expects a 2D or 3D tensor")
+
+
+def mex_columns(centerline):
+    """The MEX gateways' output (2D/Matlab2DThinning.cpp:163-175,
+    3D/Matlab3DThinning.cpp:157-170) from CenterlineExtraction's arrays:
+    one (n, 2) [y x] or (n, 3) [y x z] float64 matrix per segment, laid out
+    column-major as MATLAB holds it."""
+    if len(centerline) == 3:
+        x, y, idx = centerline
+        z = None
+    else:
+        x, y, z, idx = centerline
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    z = None if z is None else np.ascontiguousarray(z, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    nc = 2 if z is None else 3
+    nsegs = len(idx) - 1
+    out = np.empty(max(1, int(idx[-1]) * nc), np.float64)
+    check(lib().bgt_mex_columns(_ptr(x), _ptr(y), _ptr(z) if z is not None else None, _ptr(idx),
+                                nsegs, _ptr(out)))
+    return [out[idx[i] * nc:idx[i + 1] * nc].reshape((nc, idx[i + 1] - idx[i])).T
+            for i in range(nsegs)]

@@ -156,6 +156,16 @@ int bgt_centerline3d(int dtype, const void *vol, int width, int height, int dept
                      float **out_z, uint32_t **out_idx, uint32_t *nsegs);
 void bgt_free(void *p);
 
+/* The MEX gateways' output layout (2D/Matlab2DThinning.cpp:163-175,
+ * 3D/Matlab3DThinning.cpp:157-170): segment i of a centerline (out_idx as
+ * returned above) becomes an n_i x 2 double matrix with columns [y x], or
+ * n_i x 3 with columns [y x z] when z is non-NULL, stored column-major.  The
+ * matrices are written one after another into `out`, which holds
+ * out_idx[nsegs] * (z ? 3 : 2) doubles; matrix i starts at out_idx[i] * (z ?
+ * 3 : 2).  A gateway copies each into its mxCreateDoubleMatrix cell. */
+int bgt_mex_columns(const float *x, const float *y, const float *z, const uint32_t *out_idx,
+                    uint32_t nsegs, double *out);
+
 /* ------------------------------------------------------------------------- */
 /* Batched thinning (no reference counterpart: the reference thins one image  */
 /* per call, 2D/LibThinning2D.cu:39-130).  Many device-resident images and    */

@@ -118,3 +118,18 @@ def test_server_serving_sized_requests_vs_oracle():
         ref, rit = O.thin2d(a) if a.ndim == 2 else O.thin3d(a)
         assert its[k] == rit, k
         assert np.array_equal(outs[k].cpu().numpy(), ref), k
+
+
+def test_centerline2d_large_parallel_smoothing_vs_oracle():
+    """A 2048x2048 polyline frame: enough points that the segments are
+    smoothed on the host pool; the float32 outputs equal the oracle's bits."""
+    import begputhinning_amd as B
+    from begputhinning_amd import synth
+    img = synth.polylines(2048, 2048, synth.item_seed(1, 2048))
+    print("[configs] CenterlineExtraction 2048^2", flush=True)
+    ref = O.centerline2d(img, 10, 3)
+    assert ref["idx"][-1] >= 1 << 14  # above the parallel threshold
+    x, y, idx = B.CenterlineExtraction(img, 10, 3)
+    assert np.array_equal(idx, ref["idx"])
+    for a, r in ((x, ref["x"]), (y, ref["y"])):
+        assert a.dtype == np.float32 and np.array_equal(a.view(np.uint32), r.view(np.uint32))
