@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libbgt.so")
 ARCH = os.environ.get("BGT_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["thin2d.hip", "thin3d.hip", "sparse3d.hip", "tile3d.hip", "batch.hip", "seggraph.hip", "capi.cpp", "segments.cpp", "synth.cpp",
+SOURCES = ["thin2d.hip", "thin3d.hip", "sparse3d.hip", "tile3d.hip", "batch.hip", "stream2d.hip", "seggraph.hip", "capi.cpp", "segments.cpp", "synth.cpp",
            "server.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

@@ -37,6 +37,7 @@
 #include "thin3d.h"
 #include "thin3d_kernels.h"
 #include "pack.h"
+#include "queue2d.h"
 #include "sparse3d.h"
 #include "tile3d.h"
 
@@ -54,35 +55,6 @@ struct IO2D {
     int ublk0;          // first block of the item in the unpack grid
     uint32_t uchunks;   // unpack chunks per segment
     int flat;           // rows contiguous: the whole image is one segment
-};
-
-// Work queue of the 2D bands.  Tile ids index `tiles`, `stamp` and `occ`.
-// Pass P reads list P % 3 and appends to list (P + 1) % 3; list (P + 2) % 3
-// is idle during pass P (read by pass P - 1, appended to by pass P + 1), so
-// pass P clears its counters without racing any reader or writer.
-struct Queue {
-    int *list[3];   // active tile ids, by pass % 3
-    int *cnt;       // [3] entries in each list
-    int *wctr;      // [3] next list slot to hand out, by pass % 3
-    int *stamp;     // per tile: last pass it was queued for
-    uint8_t *occ;   // per tile: interior non-empty
-    int ntiles;
-};
-
-// Per-item convergence of the 2D batch.  The tiles of pass P OR their
-// iteration masks into hist[P % 4][item]; pass P + 1 skips the tiles of an
-// item whose pass-P mask misses an iteration (that iteration deleted nothing
-// in the whole image, so it is a fixed point), and pass P + 2 closes the item
-// from that mask.  Slot (P + 1) % 4 is cleared during pass P: its last reader
-// is pass P - 1 and its next writer pass P + 1.
-struct Close2D {
-    unsigned *hist;  // [4][nitems]
-    int *states;     // ST_INTS words per item
-    int nitems;
-    // host-mapped [16] (or null): at P % 16, (P + 1) << 32 | bands queued for
-    // pass P, stored when pass P starts
-    unsigned long long *report;
-    int ncu;         // compute units of the device
 };
 
 namespace {
@@ -162,12 +134,23 @@ __device__ __forceinline__ void close_item(const Close2D &c, int i, int K, int P
 {
     int *st = c.states + (size_t)i * ST_INTS;
     if (st[ST_DONE]) return;
+    const int base = st[ST_BASE];  // the item's first pass
+    if (P < base) return;
     const int m = __popc(c.hist[(size_t)(P & 3) * c.nitems + i]);
     if (m < K) {
-        st[ST_ITERS] = P * K + m + 1;  // counting the iteration that deleted nothing
+        st[ST_ITERS] = (P - base) * K + m + 1;  // counting the iteration that deleted nothing
         st[ST_PARITY] = (P + 1) & 1;
         st[ST_DONE] = 1;
+        if (c.closed)
+            __hip_atomic_store(&c.closed[i], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+}
+
+// the item skips pass P when an iteration of its pass P - 1 deleted nothing
+__device__ __forceinline__ bool converged_before(const Close2D &c, const unsigned *hprev, int i,
+                                                 int K, int P)
+{
+    return P >= 1 + c.states[(size_t)i * ST_INTS + ST_BASE] && __popc(hprev[i]) < K;
 }
 
 // tail passes: the most workgroups a band is split over
@@ -225,7 +208,7 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         if (e >= n) return;
         const int tile = list[e];
         const Tile2D tl = tiles[tile];
-        if (P >= 1 && __popc(hprev[tl.item]) < K) return;  // converged in pass P - 1
+        if (converged_before(c, hprev, tl.item, K, P)) return;
         const Item2D it = items[tl.item];
         const int row0 = tl.band * it.band_rows;
         const int seg = (int)(blockDim.x >> 6) * (64 / it.L);  // segments per window
@@ -275,7 +258,7 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         const int tile = list[t];
         const Tile2D tl = tiles[tile];
         // converged in pass P - 1: an iteration of that pass deleted nothing
-        if (P >= 1 && __popc(hprev[tl.item]) < K) continue;
+        if (converged_before(c, hprev, tl.item, K, P)) continue;
         const Item2D it = items[tl.item];
         // a single-band item also runs at most K iterations per pass, so that no
         // pass waits on one item iterating to convergence while CUs idle
@@ -503,7 +486,6 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     std::vector<TileItem3D> titems3;
     const long ntiles3 = tiles3 ? tile3d_items(lay3, titems3) : 0;
     if (ntiles3 >= 0x7fffffffL) return fail(BGT_ERR_UNSUPPORTED, "3D tiles exceed 2^31");
-    const size_t lds2 = (size_t)2 * 2 * b.c2.waves * 64 * sizeof(uint64_t);
 
     std::vector<Tile2D> t2;
     std::vector<int> tile0(n2);
@@ -617,6 +599,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
         b.close2.hist = hist2;
         b.close2.states = b.d_states2;
         b.close2.nitems = n2;
+        b.close2.closed = nullptr;
 
         std::vector<Item2D> h2(n2);
         for (int i = 0; i < n2; ++i) {
@@ -684,10 +667,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     for (const Staged &t : staged)
         BGT_HIP(hipMemcpyAsync(t.dst, b.h_stage + t.off, t.n, hipMemcpyHostToDevice, s));
     if (n2)
-        b.grid2 = occupancy_grid(b.c2.rows == 8 ? (const void *)thin2d_queue_pass<8>
-                                                : (const void *)thin2d_queue_pass<16>,
-                                 b.c2.waves * 64, lds2,
-                                 (int)std::min<long>(b.ntiles2 * kParts, 1L << 30), blocks2(b));
+        b.grid2 = queue_pass_grid2d(b.c2, (int)std::min<long>(b.ntiles2 * kParts, 1L << 30), blocks2(b));
     if (!b.h_live && hipHostMalloc((void **)&b.h_live, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(BGT_ERR_NOMEM, "pinned allocation failed");
     if (!b.h_rep) {
@@ -737,20 +717,35 @@ static int timed_mark(Batch &b, hipStream_t s, int kind, bool begin, size_t *ope
     return BGT_OK;
 }
 
+int launch_queue_pass2d(const Config2D &c2, int grid, const Item2D *items, const Tile2D *tiles,
+                        const Queue &q, const Close2D &c, const int *pbase, int p, hipStream_t s)
+{
+    const size_t lds = (size_t)2 * 2 * c2.waves * 64 * sizeof(uint64_t);
+    const dim3 g(grid + 1), blk(c2.waves * 64);
+    if (c2.rows == 8)
+        hipLaunchKernelGGL(thin2d_queue_pass<8>, g, blk, lds, s, items, tiles, q, c,
+                           c2.iters_per_pass, pbase, p);
+    else
+        hipLaunchKernelGGL(thin2d_queue_pass<16>, g, blk, lds, s, items, tiles, q, c,
+                           c2.iters_per_pass, pbase, p);
+    BGT_HIP(hipGetLastError());
+    return BGT_OK;
+}
+
+int queue_pass_grid2d(const Config2D &c2, int ntiles, int per_cu_cap)
+{
+    const size_t lds = (size_t)2 * 2 * c2.waves * 64 * sizeof(uint64_t);
+    return occupancy_grid(c2.rows == 8 ? (const void *)thin2d_queue_pass<8>
+                                       : (const void *)thin2d_queue_pass<16>,
+                          c2.waves * 64, lds, ntiles, per_cu_cap);
+}
+
 // one 2D pass, number (pbase ? *pbase : 0) + p: one launch, bookkeeping included
 static int launch_pass2d(Batch &b, hipStream_t s, const int *pbase, int p)
 {
-    const size_t lds = (size_t)2 * 2 * b.c2.waves * 64 * sizeof(uint64_t);
-    const dim3 g(b.grid2 + 1), blk(b.c2.waves * 64);
     size_t mk = 0;
     BGT_CHECK(timed_mark(b, s, 0, true, &mk));
-    if (b.c2.rows == 8)
-        hipLaunchKernelGGL(thin2d_queue_pass<8>, g, blk, lds, s, b.d_items2, b.d_tiles2, b.q2,
-                           b.close2, b.c2.iters_per_pass, pbase, p);
-    else
-        hipLaunchKernelGGL(thin2d_queue_pass<16>, g, blk, lds, s, b.d_items2, b.d_tiles2, b.q2,
-                           b.close2, b.c2.iters_per_pass, pbase, p);
-    BGT_HIP(hipGetLastError());
+    BGT_CHECK(launch_queue_pass2d(b.c2, b.grid2, b.d_items2, b.d_tiles2, b.q2, b.close2, pbase, p, s));
     BGT_CHECK(timed_mark(b, s, 0, false, &mk));
     return BGT_OK;
 }

@@ -397,6 +397,9 @@ int bgt_server_create(bgt_server *out, size_t hbm_budget, int workers2d, int wor
     cfg.workers2d = workers2d;
     cfg.workers3d = workers3d;
     cfg.max_items = max_items;
+    // BGT_SERVE_QUEUE=0: frames are batched by the 2D workers only
+    const char *q = std::getenv("BGT_SERVE_QUEUE");
+    cfg.stream_slots = (q && std::atoi(q) == 0) ? 0 : max_items;
     Server *s = nullptr;
     BGT_CHECK(server_create(cfg, default_config2d(), default_config3d(), &s));
     *out = new bgt_server_s{s};

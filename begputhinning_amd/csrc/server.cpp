@@ -30,6 +30,7 @@
 #include "batch.h"
 #include "common.h"
 #include "server.h"
+#include "stream2d.h"
 #include "thin2d.h"
 #include "thin3d.h"
 
@@ -74,6 +75,9 @@ struct Server {
     uint64_t next_id = 1;
     bool stop = false;
     std::vector<std::thread> threads;
+    // frames that fit its slots join the continuously running 2D queue at the
+    // next pass boundary; other frames and all volumes go to the workers
+    Stream2D *q2d = nullptr;
     // statistics
     uint64_t batches = 0, completed = 0;
     size_t peak_in_use = 0;
@@ -193,6 +197,37 @@ int server_create(const ServerConfig &cfg, const Config2D &c2, const Config3D &c
     sv->c2 = c2;
     sv->c3 = c3;
     BGT_HIP(hipGetDevice(&sv->device));
+    if (cfg.workers2d > 0 && cfg.stream_slots > 0) {
+        // slots for frames up to 2048 x 2048, within half of the budget
+        Stream2DConfig qc;
+        qc.max_w = qc.max_h = 2048;
+        Plan2D big;
+        BGT_CHECK(plan2d(qc.max_w, qc.max_h, c2, big));
+        const size_t slot_bytes = 2 * big.state_words * sizeof(uint64_t) + 4096;
+        qc.slots = (int)std::min<size_t>((size_t)std::min(cfg.stream_slots, 64), cfg.budget / 2 / slot_bytes);
+        if (qc.slots >= 1) {
+            Server *raw = sv.get();
+            BGT_CHECK(stream2d_create(qc, c2,
+                                      [raw](void *tag, int status, int iters) {
+                                          Request *r = (Request *)tag;
+                                          const std::string err =
+                                              status == BGT_OK ? std::string() : last_error_message();
+                                          {
+                                              std::lock_guard<std::mutex> lk(raw->m);
+                                              r->status = status;
+                                              r->error = err;
+                                              r->iterations = iters;
+                                              r->t_done = now_ms();
+                                              r->done = true;
+                                              ++raw->completed;
+                                          }
+                                          raw->cv_done.notify_all();
+                                      },
+                                      &sv->q2d));
+            sv->in_use = stream2d_device_bytes(sv->q2d);  // held for the server's lifetime
+            sv->peak_in_use = sv->in_use;
+        }
+    }
     for (int k = 0; k < cfg.workers2d; ++k) sv->threads.emplace_back(&Server::worker, sv.get(), 0);
     for (int k = 0; k < cfg.workers3d; ++k) sv->threads.emplace_back(&Server::worker, sv.get(), 1);
     *out = sv.release();
@@ -217,6 +252,7 @@ void server_destroy(Server *sv)
     }
     sv->cv_work.notify_all();
     sv->cv_done.notify_all();
+    stream2d_destroy(sv->q2d);  // its running frames finish, waiting ones fail
     for (auto &t : sv->threads) t.join();
     delete sv;
 }
@@ -246,6 +282,23 @@ int server_submit(Server *sv, int cls, const uint8_t *in, size_t in_pitch, uint8
     r->w = w;
     r->h = h;
     r->d = d;
+    if (cls == 0 && sv->q2d && stream2d_fits(sv->q2d, w, h)) {
+        // the frame joins the running queue (its slot is already allocated)
+        Request *rp = r.get();
+        {
+            std::lock_guard<std::mutex> lk(sv->m);
+            r->id = sv->next_id++;
+            r->t_submit = t_submit;
+            *ticket = r->id;
+            sv->reqs.emplace(r->id, std::move(r));
+        }
+        const int rc = stream2d_submit(sv->q2d, in, in_pitch, w, h, out, out_pitch, rp);
+        if (rc != BGT_OK) {
+            std::lock_guard<std::mutex> lk(sv->m);
+            sv->reqs.erase(*ticket);
+        }
+        return rc;
+    }
     BGT_CHECK(item_bytes(*r, sv->c2, sv->c3, &r->bytes));
     if (r->bytes > sv->cfg.budget)
         return fail(BGT_ERR_ADMISSION, "request needs %zu bytes of device memory, budget is %zu",

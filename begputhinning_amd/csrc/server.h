@@ -14,6 +14,9 @@ struct ServerConfig {
     int workers2d = 2;    // worker threads (each a stream + batch engine) for 2D
     int workers3d = 1;    // and for 3D requests
     int max_items = 64;   // most requests per batch
+    // frames held by the continuously running 2D queue at once (0: none; all
+    // frames then go to the 2D workers)
+    int stream_slots = 64;
 };
 
 struct Server;

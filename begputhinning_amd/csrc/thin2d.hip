@@ -16,6 +16,7 @@
 #include "common.h"
 #include "thin2d.h"
 #include "pack.h"
+#include "queue2d.h"
 #include "thin2d_kernels.h"
 
 namespace bgt {
@@ -94,8 +95,8 @@ __global__ __launch_bounds__(256) void unpack2d_u8(const Item2D *__restrict__ it
 }
 
 // launch of unpack2d_u8 for a w x h image with row pitch `pitch`
-static void launch_unpack2d_u8(const Item2D *d_item, uint8_t *out, int w, int h, size_t pitch,
-                               hipStream_t s)
+void launch_unpack_item2d(const Item2D *d_item, uint8_t *out, int w, int h, size_t pitch,
+                          hipStream_t s)
 {
     int flat = 0;
     uint32_t uchunks = 0;
@@ -393,7 +394,7 @@ int thin2d_device_u8(const uint8_t *d_in, size_t in_pitch, int w, int h, uint8_t
                        in_pitch, w, h, pl.L, hv.buf[0]);
     BGT_HIP(hipGetLastError());
     BGT_CHECK(run_passes2d(cfg, pl, d_item, d_ctr, hv.st, s, iterations));
-    launch_unpack2d_u8(d_item, d_out, w, h, out_pitch, s);
+    launch_unpack_item2d(d_item, d_out, w, h, out_pitch, s);
     BGT_HIP(hipGetLastError());
     BGT_HIP(hipStreamSynchronize(s));
     return BGT_OK;
@@ -460,7 +461,7 @@ int thin2d_device_u8_noalloc(const uint8_t *d_in, size_t in_pitch, int w, int h,
     BGT_HIP(hipGetLastError());
     int poll[ST_INTS];  // pageable: the poll copies synchronise anyway
     BGT_CHECK(run_passes2d(cfg, pl, d_item, d_ctr, hv.st, s, iterations, poll));
-    launch_unpack2d_u8(d_item, d_out, w, h, out_pitch, s);
+    launch_unpack_item2d(d_item, d_out, w, h, out_pitch, s);
     BGT_HIP(hipGetLastError());
     BGT_HIP(hipStreamSynchronize(s));
     return BGT_OK;
@@ -494,7 +495,7 @@ static int thin2d_host_t(int w, int h, const T *in, OutT *out, const Config2D &c
     BGT_HIP(hipGetLastError());
     BGT_CHECK(run_passes2d(cfg, pl, d_item, d_ctr, hv.st, s, nullptr));
     if (std::is_same<OutT, uint8_t>::value)
-        launch_unpack2d_u8(d_item, (uint8_t *)io.ptr, w, h, (size_t)w, s);
+        launch_unpack_item2d(d_item, (uint8_t *)io.ptr, w, h, (size_t)w, s);
     else
         hipLaunchKernelGGL(unpack2d_t<OutT>, g, b, 0, s, d_item, (OutT *)io.ptr);
     BGT_HIP(hipGetLastError());

@@ -27,8 +27,13 @@ namespace bgt {
 // Per-item state words.  ST_MASK collects, over the tiles of a pass, bit j =
 // "iteration j of this pass deleted something"; ST_RUN is set by a tile that
 // iterated its item to convergence on its own (single-tile items) and then
-// holds the number of iterations that deleted something.
-enum { ST_PASS = 0, ST_DONE = 1, ST_ITERS = 2, ST_MASK = 3, ST_PARITY = 4, ST_RUN = 5, ST_INTS = 8 };
+// holds the number of iterations that deleted something; ST_BASE is the pass
+// of a work queue in which the item's first iteration runs (0 in a batch; the
+// pass an item joined at in a continuously running queue).
+enum {
+    ST_PASS = 0, ST_DONE = 1, ST_ITERS = 2, ST_MASK = 3, ST_PARITY = 4, ST_RUN = 5, ST_BASE = 6,
+    ST_INTS = 8
+};
 
 struct Item2D {
     uint64_t *buf[2];   // ping-pong packed state (L words per row)

@@ -221,6 +221,11 @@ int bgt_batch_destroy(bgt_batch b);
 /* requests in flight; each worker also keeps a grow-only arena (the largest  */
 /* batch it ran, plus a quarter) and per-thread workspaces until the server   */
 /* is destroyed, which are not counted against it.                            */
+/* Frames up to 2048 x 2048 join a continuously running 2D band queue at its  */
+/* next pass boundary (up to max_items at once, in slots allocated at create  */
+/* within half the budget and counted against it); larger frames and volumes */
+/* go to the workers.  BGT_SERVE_QUEUE=0 in the environment sends every frame */
+/* to the workers.                                                            */
 /* ------------------------------------------------------------------------- */
 typedef struct bgt_server_s *bgt_server;
 

@@ -152,3 +152,54 @@ def test_server_memory_returned_on_destroy():
     for a, o in zip(items, outs):
         ref, _ = O.thin2d(a) if a.ndim == 2 else O.thin3d(a)
         assert np.array_equal(o.cpu().numpy(), ref)
+
+
+def test_frames_join_the_running_queue():
+    """Frames submitted while others are mid-run (staggered, from several
+    threads; deep small frames beside large shallow ones) join the running
+    2D queue at a pass boundary; every result and iteration count equals
+    the oracle's and the single-image path's."""
+    import time
+    import begputhinning_amd as B
+    from begputhinning_amd import synth
+    from begputhinning_amd.serve import ThinningServer
+    rng = np.random.default_rng(123)
+    items = []
+    for i in range(40):
+        w, h = (int(v) for v in rng.integers(64, 1400, 2))
+        items.append((synth.polylines if i % 3 else synth.tree2d)(w, h, 900 + i))
+    items += [np.ones((150, 130), np.uint8), np.zeros((9, 700), np.uint8), np.ones((1, 1), np.uint8)]
+    ins = [torch.from_numpy(a).cuda() for a in items]
+    outs = [torch.full_like(t, 4) for t in ins]
+    torch.cuda.synchronize()
+    srv = ThinningServer(budget_bytes=4 << 30, workers2d=2, workers3d=1, max_items=16)
+    res = [None] * len(items)
+    errors = []
+
+    def client(ks, seed):
+        r = np.random.default_rng(seed)
+        try:
+            tks = []
+            for k in ks:
+                time.sleep(float(r.uniform(0, 0.002)))
+                tks.append((k, srv.submit(ins[k], outs[k])))
+            for k, tk in tks:
+                res[k] = srv.wait(tk)[0]
+        except Exception as exc:  # reported below
+            errors.append(exc)
+
+    threads = [threading.Thread(target=client, args=(list(range(j, len(items), 3)), j)) for j in range(3)]
+    try:
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(120)
+    finally:
+        srv.close()
+    assert not errors, errors
+    for k, a in enumerate(items):
+        ref, rit = O.thin2d(a)
+        assert res[k] == rit, (k, a.shape, res[k], rit)
+        assert np.array_equal(outs[k].cpu().numpy(), ref), (k, a.shape)
+        single, sit = B.thin2d_device(ins[k])
+        assert sit == rit and torch.equal(single, outs[k]), k
