@@ -160,7 +160,7 @@ constexpr int kParts = 16;
 // bands.  Workgroups 0 .. gridDim.x - 2 take bands; the last workgroup does
 // the pass's bookkeeping (closes the items of pass P - 2, clears the idle
 // queue counters and history slot) and takes no band.
-template <int R>
+template <int R, int V>
 __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restrict__ items,
                                                           const Tile2D *__restrict__ tiles,
                                                           Queue q, Close2D c, int K,
@@ -211,21 +211,21 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         if (converged_before(c, hprev, tl.item, K, P)) return;
         const Item2D it = items[tl.item];
         const int row0 = tl.band * it.band_rows;
-        const int seg = (int)(blockDim.x >> 6) * (64 / it.L);  // segments per window
+        const int seg = (int)(blockDim.x >> 6) * (64 / lanes_for(it.L, V));  // segments per window
         const int in4 = seg * 4 - 8 * K, in8 = seg * 8 - 8 * K;  // interior rows of a part
         const int np4 = in4 >= 1 ? (it.band_rows + in4 - 1) / in4 : kParts + 1;
         const int np8 = in8 >= 1 ? (it.band_rows + in8 - 1) / in8 : kParts + 1;
         if (np4 <= kp) {
             if (part >= np4 || row0 + part * in4 >= it.h) return;
-            band_work<4>(it, row0 + part * in4, min(in4, it.band_rows - part * in4), false, K,
+            band_work<4, V>(it, row0 + part * in4, min(in4, it.band_rows - part * in4), false, K,
                          P & 1, edges, &info);
         } else if (R > 8 && np8 <= kp) {
             if (part >= np8 || row0 + part * in8 >= it.h) return;
-            band_work<8>(it, row0 + part * in8, min(in8, it.band_rows - part * in8), false, K,
+            band_work<8, V>(it, row0 + part * in8, min(in8, it.band_rows - part * in8), false, K,
                          P & 1, edges, &info);
         } else {  // windows too short for parts: one workgroup per band
             if (part != 0) return;
-            band_work<R>(it, row0, it.band_rows, false, K, P & 1, edges, &info);
+            band_work<R, V>(it, row0, it.band_rows, false, K, P & 1, edges, &info);
         }
         if (tid == 0) {
             // parts only set the flag: a band emptied in a tail pass stays
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         const Item2D it = items[tl.item];
         // a single-band item also runs at most K iterations per pass, so that no
         // pass waits on one item iterating to convergence while CUs idle
-        band_work<R>(it, tl.band * it.band_rows, it.band_rows, it.nbands == 1, K, P & 1, edges,
+        band_work<R, V>(it, tl.band * it.band_rows, it.band_rows, it.nbands == 1, K, P & 1, edges,
                      &info, K);
         if (tid == 0) {
             q.occ[tile] = info.occ != 0;
@@ -357,6 +357,7 @@ struct Batch {
     Queue q2{};
     Close2D close2{};
     int grid2 = 0;          // band workgroups of a pass (one more does the bookkeeping)
+    int V2 = 1;             // words per lane of every 2D item's bands
     int *d_states2 = nullptr;
     int *d_gpass2 = nullptr;  // pass number base of graph replays
     int pass2 = 0;            // 2D passes queued in this run
@@ -468,8 +469,16 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     if (b.in2.empty() && b.in3.empty()) return fail(BGT_ERR_INVALID, "empty batch");
     BGT_CHECK(check_config2d(b.c2));
     const int n2 = (int)b.in2.size(), n3 = (int)b.in3.size();
+    // one pass kernel runs every item's bands, so all take the words per lane
+    // of the widest
+    b.V2 = 1;
+    for (int i = 0; i < n2; ++i) {
+        const int v = lane_words2d(b.in2[i].w, b.c2);
+        if (!v) return fail(BGT_ERR_UNSUPPORTED, "2D width %d exceeds %d", b.in2[i].w, kMaxWidth2D);
+        b.V2 = std::max(b.V2, v);
+    }
     std::vector<Plan2D> p2(n2);
-    for (int i = 0; i < n2; ++i) BGT_CHECK(plan2d(b.in2[i].w, b.in2[i].h, b.c2, p2[i]));
+    for (int i = 0; i < n2; ++i) BGT_CHECK(plan2d(b.in2[i].w, b.in2[i].h, b.c2, p2[i], b.V2));
     BGT_CHECK(check_config3d(b.c3));
     std::vector<int> w3v(n3), h3v(n3), d3v(n3);
     for (int i = 0; i < n3; ++i) {
@@ -667,7 +676,8 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     for (const Staged &t : staged)
         BGT_HIP(hipMemcpyAsync(t.dst, b.h_stage + t.off, t.n, hipMemcpyHostToDevice, s));
     if (n2)
-        b.grid2 = queue_pass_grid2d(b.c2, (int)std::min<long>(b.ntiles2 * kParts, 1L << 30), blocks2(b));
+        b.grid2 = queue_pass_grid2d(b.c2, b.V2, (int)std::min<long>(b.ntiles2 * kParts, 1L << 30),
+                                    blocks2(b));
     if (!b.h_live && hipHostMalloc((void **)&b.h_live, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(BGT_ERR_NOMEM, "pinned allocation failed");
     if (!b.h_rep) {
@@ -717,27 +727,34 @@ static int timed_mark(Batch &b, hipStream_t s, int kind, bool begin, size_t *ope
     return BGT_OK;
 }
 
-int launch_queue_pass2d(const Config2D &c2, int grid, const Item2D *items, const Tile2D *tiles,
-                        const Queue &q, const Close2D &c, const int *pbase, int p, hipStream_t s)
+// the queue pass kernel for `rows` register rows and V words per lane
+static const void *queue_pass_fn(int rows, int V)
 {
-    const size_t lds = (size_t)2 * 2 * c2.waves * 64 * sizeof(uint64_t);
+    if (V == 4)
+        return rows == 8 ? (const void *)thin2d_queue_pass<8, 4> : (const void *)thin2d_queue_pass<16, 4>;
+    if (V == 2)
+        return rows == 8 ? (const void *)thin2d_queue_pass<8, 2> : (const void *)thin2d_queue_pass<16, 2>;
+    return rows == 8 ? (const void *)thin2d_queue_pass<8, 1> : (const void *)thin2d_queue_pass<16, 1>;
+}
+
+int launch_queue_pass2d(const Config2D &c2, int V, int grid, const Item2D *items,
+                        const Tile2D *tiles, const Queue &q, const Close2D &c, const int *pbase,
+                        int p, hipStream_t s)
+{
     const dim3 g(grid + 1), blk(c2.waves * 64);
-    if (c2.rows == 8)
-        hipLaunchKernelGGL(thin2d_queue_pass<8>, g, blk, lds, s, items, tiles, q, c,
-                           c2.iters_per_pass, pbase, p);
-    else
-        hipLaunchKernelGGL(thin2d_queue_pass<16>, g, blk, lds, s, items, tiles, q, c,
-                           c2.iters_per_pass, pbase, p);
-    BGT_HIP(hipGetLastError());
+    int K = c2.iters_per_pass;
+    Queue qq = q;
+    Close2D cc = c;
+    void *args[] = {(void *)&items, (void *)&tiles, (void *)&qq, (void *)&cc,
+                    (void *)&K,     (void *)&pbase, (void *)&p};
+    BGT_HIP(hipLaunchKernel(queue_pass_fn(c2.rows, V), g, blk, args, pass_lds_bytes2d(c2, V), s));
     return BGT_OK;
 }
 
-int queue_pass_grid2d(const Config2D &c2, int ntiles, int per_cu_cap)
+int queue_pass_grid2d(const Config2D &c2, int V, int ntiles, int per_cu_cap)
 {
-    const size_t lds = (size_t)2 * 2 * c2.waves * 64 * sizeof(uint64_t);
-    return occupancy_grid(c2.rows == 8 ? (const void *)thin2d_queue_pass<8>
-                                       : (const void *)thin2d_queue_pass<16>,
-                          c2.waves * 64, lds, ntiles, per_cu_cap);
+    return occupancy_grid(queue_pass_fn(c2.rows, V), c2.waves * 64, pass_lds_bytes2d(c2, V), ntiles,
+                          per_cu_cap);
 }
 
 // one 2D pass, number (pbase ? *pbase : 0) + p: one launch, bookkeeping included
@@ -745,7 +762,7 @@ static int launch_pass2d(Batch &b, hipStream_t s, const int *pbase, int p)
 {
     size_t mk = 0;
     BGT_CHECK(timed_mark(b, s, 0, true, &mk));
-    BGT_CHECK(launch_queue_pass2d(b.c2, b.grid2, b.d_items2, b.d_tiles2, b.q2, b.close2, pbase, p, s));
+    BGT_CHECK(launch_queue_pass2d(b.c2, b.V2, b.grid2, b.d_items2, b.d_tiles2, b.q2, b.close2, pbase, p, s));
     BGT_CHECK(timed_mark(b, s, 0, false, &mk));
     return BGT_OK;
 }
@@ -1123,7 +1140,7 @@ int batch_add2d(Batch &b, const uint8_t *in, size_t in_pitch, int w, int h, uint
     if (!in || !out) return fail(BGT_ERR_INVALID, "null device pointer");
     if (w <= 0 || h <= 0 || in_pitch < (size_t)w || out_pitch < (size_t)w)
         return fail(BGT_ERR_INVALID, "bad 2D item (%d x %d)", w, h);
-    if (w > 4096) return fail(BGT_ERR_UNSUPPORTED, "2D width %d exceeds 4096", w);
+    if (w > kMaxWidth2D) return fail(BGT_ERR_UNSUPPORTED, "2D width %d exceeds %d", w, kMaxWidth2D);
     b.release();
     b.order.push_back((int)b.in2.size());
     b.in2.push_back({{in, in_pitch, out, out_pitch}, w, h});
@@ -1176,7 +1193,8 @@ int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D 
     int n = 0;
     auto trial = [&](const Config2D &t2, const Config3D &t3) {
         if (n && t2.waves == c2.waves && t2.rows == c2.rows && t2.iters_per_pass == c2.iters_per_pass &&
-            t2.blocks_per_cu == c2.blocks_per_cu && t3.blocks_per_cu == c3.blocks_per_cu)
+            t2.blocks_per_cu == c2.blocks_per_cu && t2.lane_words == c2.lane_words &&
+            t3.blocks_per_cu == c3.blocks_per_cu)
             return;  // the current shape, already timed
         if (batch_set_config(b, t2, t3) != BGT_OK) return;
         if (batch_run(b, s, false) != BGT_OK) return;  // lays out; shape may not fit

@@ -111,6 +111,7 @@ static Config2D &config2d_storage()
         if (const char *e = std::getenv("BGT_2D_ROWS")) v.rows = std::atoi(e);
         if (const char *e = std::getenv("BGT_2D_K")) v.iters_per_pass = std::atoi(e);
         if (const char *e = std::getenv("BGT_2D_BLOCKS")) v.blocks_per_cu = std::atoi(e);
+        if (const char *e = std::getenv("BGT_2D_LANE_WORDS")) v.lane_words = std::atoi(e);
         if (check_config2d(v) != BGT_OK) {
             std::fprintf(stderr, "libbgt: ignoring BGT_2D_* environment: %s\n", g_error.c_str());
             v = Config2D{};
@@ -161,13 +162,22 @@ int bgt_version(void) { return 1; }
 
 int bgt_set_config2d(int waves, int rows, int iters_per_pass, int blocks_per_cu)
 {
-    Config2D c;
+    Config2D c = default_config2d();  // the words per lane are kept
     c.waves = waves;
     c.rows = rows;
     c.iters_per_pass = iters_per_pass;
     c.blocks_per_cu = blocks_per_cu;
     BGT_CHECK(check_config2d(c));
-    (void)default_config2d();  // environment read before the lock is taken
+    std::lock_guard<std::mutex> lk(g_config_mu);
+    config2d_storage() = c;
+    return BGT_OK;
+}
+
+int bgt_set_lane_words2d(int words)
+{
+    Config2D c = default_config2d();
+    c.lane_words = words;
+    BGT_CHECK(check_config2d(c));
     std::lock_guard<std::mutex> lk(g_config_mu);
     config2d_storage() = c;
     return BGT_OK;

@@ -43,12 +43,13 @@ struct Close2D {
 };
 
 // One pass P = (pbase ? *pbase : 0) + p of the band queue (thin2d_queue_pass
-// in batch.hip) on `grid` band workgroups plus the bookkeeping one.
-int launch_queue_pass2d(const Config2D &c2, int grid, const Item2D *items, const Tile2D *tiles,
+// in batch.hip, every item's bands planned with V words per lane) on `grid`
+// band workgroups plus the bookkeeping one.
+int launch_queue_pass2d(const Config2D &c2, int V, int grid, const Item2D *items, const Tile2D *tiles,
                         const Queue &q, const Close2D &c, const int *pbase, int p, hipStream_t s);
 // Band workgroups of a pass: as many as fit (at most per_cu_cap per CU when
 // > 0) and no more than ntiles.
-int queue_pass_grid2d(const Config2D &c2, int ntiles, int per_cu_cap);
+int queue_pass_grid2d(const Config2D &c2, int V, int ntiles, int per_cu_cap);
 
 // kernels of thin2d.hip shared with the serving queue
 void launch_unpack_item2d(const Item2D *d_item, uint8_t *out, int w, int h, size_t pitch,

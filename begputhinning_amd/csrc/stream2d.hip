@@ -102,6 +102,7 @@ struct Stream2D {
 
     Stream2DConfig cfg;
     Config2D c2;
+    int V = 1;  // words per lane of every slot's bands (those of the largest image)
     Stream2DDone done;
     int device = 0;
     int mb = 0;                 // bands per slot
@@ -140,14 +141,14 @@ struct Stream2D {
 bool stream2d_fits(const Stream2D *e, int w, int h)
 {
     Plan2D pl;
-    if (w <= 0 || h <= 0 || plan2d(w, h, e->c2, pl) != BGT_OK) return false;
+    if (w <= 0 || h <= 0 || plan2d(w, h, e->c2, pl, e->V) != BGT_OK) return false;
     return pl.nbands <= e->mb && pl.state_words <= e->slot_words;
 }
 
 int Stream2D::admit(int k, const Req &r)
 {
     Plan2D pl;
-    BGT_CHECK(plan2d(r.w, r.h, c2, pl));
+    BGT_CHECK(plan2d(r.w, r.h, c2, pl, V));
     Item2D v{};
     v.buf[0] = d_bufs + (size_t)k * 2 * slot_words;
     v.buf[1] = v.buf[0] + slot_words;
@@ -221,7 +222,7 @@ void Stream2D::run()
         }
         // ---- keep the passes flowing while images are in the queue
         while (live > 0 && pass_next - reported < kAhead) {
-            const int rc = launch_queue_pass2d(c2, grid, d_items, d_tiles, q, c, nullptr, pass_next, s);
+            const int rc = launch_queue_pass2d(c2, V, grid, d_items, d_tiles, q, c, nullptr, pass_next, s);
             if (rc != BGT_OK) {
                 // the queue cannot go on: every image in it or waiting fails
                 const std::string why = last_error_message();
@@ -301,6 +302,7 @@ int stream2d_create(const Stream2DConfig &cfg, const Config2D &c2, Stream2DDone 
     BGT_HIP(hipGetDevice(&e->device));
     Plan2D big;
     BGT_CHECK(plan2d(cfg.max_w, cfg.max_h, c2, big));
+    e->V = big.V;
     // the fewest interior rows per band any admitted width gets (widest rows)
     e->mb = (cfg.max_h + big.band_rows - 1) / big.band_rows;
     e->slot_words = big.state_words;
@@ -369,7 +371,7 @@ int stream2d_create(const Stream2DConfig &cfg, const Config2D &c2, Stream2DDone 
     BGT_HIP(hipEventCreateWithFlags(&e->ev_pass, hipEventDisableTiming));
     e->slots.resize(ns);
     for (auto &sl : e->slots) BGT_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
-    e->grid = queue_pass_grid2d(c2, nt * 16, cfg.blocks_per_cu);
+    e->grid = queue_pass_grid2d(c2, e->V, nt * 16, cfg.blocks_per_cu);
     const int ninit = std::max({ns * ST_INTS, 4 * ns, nt, 16});
     hipLaunchKernelGGL(stream2d_init, dim3((ninit + 255) / 256), dim3(256), 0, e->s, e->d_states, ns,
                        e->c.hist, e->d_tiles, e->q.occ, e->q.stamp, nt, e->mb, ctl, 16);

@@ -20,17 +20,29 @@ struct Config2D {
     // a batch that also holds volumes, whose chain needs the remaining slots,
     // else as many as fit)
     int blocks_per_cu = -1;
+    // 64-bit words per lane (1, 2 or 4), 0 = automatic: the fewest that hold
+    // a row in one wave64.  Rows wider than the choice take more.
+    int lane_words = 0;
 };
+
+// Widest 2D image: a row in one wave64 of lanes of four 64-bit words.
+constexpr int kMaxWidth2D = 64 * 64 * 4;
 
 struct Plan2D {
     int w = 0, h = 0, L = 1;  // L = 64-bit words per row
+    int V = 1;                // words per lane
     int band_rows = 0, nbands = 0;
     size_t state_words = 0;
 };
 
-int plan2d(int w, int h, const Config2D &cfg, Plan2D &pl);
+// Words per lane a w-wide image is planned with under cfg (0: too wide).
+int lane_words2d(int w, const Config2D &cfg);
+// V = 0: lane_words2d's choice; else V words per lane (at least the image needs)
+int plan2d(int w, int h, const Config2D &cfg, Plan2D &pl, int V = 0);
 int check_config2d(const Config2D &cfg);
-int band_rows_for(int L, const Config2D &cfg);
+int band_rows_for(int L, int V, const Config2D &cfg);
+// LDS bytes of a pass workgroup with V words per lane
+size_t pass_lds_bytes2d(const Config2D &cfg, int V);
 size_t workspace_bytes2d(const Plan2D &pl);
 int setup_item2d(const Plan2D &pl, void *ws, hipStream_t s, Item2D **d_item, unsigned **d_ctr,
                  Item2D *host_item);

@@ -144,7 +144,7 @@ __global__ void init_item2d(Item2D *item, Item2D val, unsigned *ctr)
 // the band's result is the same while a pass's latency, bound by the rows one
 // wave works through per subiteration, drops; where no such window fits,
 // part 0 runs the whole band.
-template <int R>
+template <int R, int V>
 __global__ __launch_bounds__(512) void thin2d_pass(const Item2D *__restrict__ items, int K,
                                                     unsigned *__restrict__ ctr, int kp)
 {
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(512) void thin2d_pass(const Item2D *__restrict__ it
     }
     int row0 = band * it.band_rows, nrows = it.band_rows, win = R;
     if (work && kp > 1) {
-        const int seg = (int)(blockDim.x >> 6) * (64 / it.L);  // segments per window
+        const int seg = (int)(blockDim.x >> 6) * (64 / lanes_for(it.L, V));  // segments per window
         const int in4 = seg * 4 - 8 * K, in8 = seg * 8 - 8 * K;  // interior rows of a part
         const int np4 = in4 >= 1 ? (it.band_rows + in4 - 1) / in4 : kp + 1;
         const int np8 = in8 >= 1 ? (it.band_rows + in8 - 1) / in8 : kp + 1;
@@ -190,11 +190,11 @@ __global__ __launch_bounds__(512) void thin2d_pass(const Item2D *__restrict__ it
     if (work) {
         int run = 0;
         if (win == 4)
-            band_work<4>(it, row0, nrows, false, K, par, edges, &info);
+            band_work<4, V>(it, row0, nrows, false, K, par, edges, &info);
         else if (win == 8 && R > 8)
-            band_work<8>(it, row0, nrows, false, K, par, edges, &info);
+            band_work<8, V>(it, row0, nrows, false, K, par, edges, &info);
         else
-            run = band_work<R>(it, row0, nrows, it.nbands == 1, K, par, edges, &info);
+            run = band_work<R, V>(it, row0, nrows, it.nbands == 1, K, par, edges, &info);
         if (tid == 0) {
             // flags are cleared before the pass (init, then each pass's last
             // workgroup), so the parts of a band only set them
@@ -225,9 +225,9 @@ __global__ __launch_bounds__(512) void thin2d_pass(const Item2D *__restrict__ it
 // host side
 // ---------------------------------------------------------------------------
 
-int band_rows_for(int L, const Config2D &cfg)
+int band_rows_for(int L, int V, const Config2D &cfg)
 {
-    const int G = 64 / L;
+    const int G = 64 / lanes_for(L, V);
     return cfg.waves * G * cfg.rows - 8 * cfg.iters_per_pass;
 }
 
@@ -235,27 +235,43 @@ int check_config2d(const Config2D &cfg)
 {
     if (cfg.waves < 1 || cfg.waves > 8 || (cfg.rows != 8 && cfg.rows != 16) ||
         cfg.iters_per_pass < 1 || cfg.iters_per_pass > 16 || cfg.blocks_per_cu < -1 ||
-        cfg.blocks_per_cu > 32)
-        return fail(BGT_ERR_INVALID, "bad 2D config (waves=%d rows=%d k=%d blocks/CU=%d)", cfg.waves,
-                    cfg.rows, cfg.iters_per_pass, cfg.blocks_per_cu);
+        cfg.blocks_per_cu > 32 ||
+        (cfg.lane_words != 0 && cfg.lane_words != 1 && cfg.lane_words != 2 && cfg.lane_words != 4))
+        return fail(BGT_ERR_INVALID, "bad 2D config (waves=%d rows=%d k=%d blocks/CU=%d lane words=%d)",
+                    cfg.waves, cfg.rows, cfg.iters_per_pass, cfg.blocks_per_cu, cfg.lane_words);
     return BGT_OK;
 }
 
-static int launch_image_pass2d(const Config2D &cfg, int nbands, int kp, const Item2D *item,
+int lane_words2d(int w, const Config2D &cfg)
+{
+    const int L = (w + 63) / 64;
+    const int need = L <= 64 ? 1 : L <= 128 ? 2 : L <= 256 ? 4 : 0;
+    if (!need) return 0;
+    return std::max(need, cfg.lane_words > 0 ? cfg.lane_words : 1);
+}
+
+// the per-image pass kernel for `rows` register rows and V words per lane
+static const void *image_pass_fn(int rows, int V)
+{
+    if (V == 4) return rows == 8 ? (const void *)thin2d_pass<8, 4> : (const void *)thin2d_pass<16, 4>;
+    if (V == 2) return rows == 8 ? (const void *)thin2d_pass<8, 2> : (const void *)thin2d_pass<16, 2>;
+    return rows == 8 ? (const void *)thin2d_pass<8, 1> : (const void *)thin2d_pass<16, 1>;
+}
+
+// LDS of a pass workgroup: the edge rows, [2 buffers][top/bottom][lanes][V words]
+size_t pass_lds_bytes2d(const Config2D &cfg, int V)
+{
+    return (size_t)2 * 2 * cfg.waves * 64 * V * sizeof(uint64_t);
+}
+
+static int launch_image_pass2d(const Config2D &cfg, int V, int nbands, int kp, const Item2D *item,
                                unsigned *ctr, hipStream_t s)
 {
-    const size_t lds = (size_t)2 * 2 * cfg.waves * 64 * sizeof(uint64_t);
+    const void *fn = image_pass_fn(cfg.rows, V);
     const dim3 grid(nbands * kp), blk(cfg.waves * 64);
-    const int K = cfg.iters_per_pass;
-    switch (cfg.rows) {
-    case 8:
-        hipLaunchKernelGGL(thin2d_pass<8>, grid, blk, lds, s, item, K, ctr, kp);
-        break;
-    default:
-        hipLaunchKernelGGL(thin2d_pass<16>, grid, blk, lds, s, item, K, ctr, kp);
-        break;
-    }
-    BGT_HIP(hipGetLastError());
+    int K = cfg.iters_per_pass;
+    void *args[] = {(void *)&item, (void *)&K, (void *)&ctr, (void *)&kp};
+    BGT_HIP(hipLaunchKernel(fn, grid, blk, args, pass_lds_bytes2d(cfg, V), s));
     return BGT_OK;
 }
 
@@ -276,16 +292,23 @@ static int image_parts(int nbands)
     return std::max(1, std::min(16, cus / nbands));
 }
 
-int plan2d(int w, int h, const Config2D &cfg, Plan2D &pl)
+int plan2d(int w, int h, const Config2D &cfg, Plan2D &pl, int V)
 {
     if (w <= 0 || h <= 0) return fail(BGT_ERR_INVALID, "2D size must be >= 1 (got %d x %d)", w, h);
-    if (w > 64 * 64)
-        return fail(BGT_ERR_UNSUPPORTED, "2D width %d exceeds 4096 (one wave64 of 64-bit words)", w);
+    if (w > kMaxWidth2D)
+        return fail(BGT_ERR_UNSUPPORTED,
+                    "2D width %d exceeds %d (one wave64 of lanes of four 64-bit words)", w, kMaxWidth2D);
     BGT_CHECK(check_config2d(cfg));
+    const int need = lane_words2d(w, cfg);
+    if (V == 0) V = need;
+    if (V != 1 && V != 2 && V != 4) return fail(BGT_ERR_INVALID, "2D lane words %d not 1, 2 or 4", V);
+    if (lanes_for((w + 63) / 64, V) > 64)
+        return fail(BGT_ERR_UNSUPPORTED, "2D width %d needs more than %d words per lane", w, V);
     pl.w = w;
     pl.h = h;
-    pl.L = (w + 63) / 64;  // words per row; a wave holds floor(64 / L) row segments
-    pl.band_rows = band_rows_for(pl.L, cfg);
+    pl.L = (w + 63) / 64;  // words per row; a wave holds floor(64 / ceil(L / V)) row segments
+    pl.V = V;
+    pl.band_rows = band_rows_for(pl.L, V, cfg);
     if (pl.band_rows < 4 * cfg.iters_per_pass)
         return fail(BGT_ERR_INVALID, "2D band of %d rows is thinner than its halo %d",
                     pl.band_rows, 4 * cfg.iters_per_pass);
@@ -363,7 +386,7 @@ int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned
     const int kp = image_parts(pl.nbands);
     for (;;) {
         for (int i = 0; i < chunk; ++i)
-            BGT_CHECK(launch_image_pass2d(cfg, pl.nbands, kp, d_item, d_ctr, s));
+            BGT_CHECK(launch_image_pass2d(cfg, pl.V, pl.nbands, kp, d_item, d_ctr, s));
         launched += chunk;
         BGT_HIP(hipMemcpyAsync(hp, d_state, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
         BGT_HIP(hipStreamSynchronize(s));

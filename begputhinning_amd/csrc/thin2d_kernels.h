@@ -39,11 +39,14 @@ struct Item2D {
     uint64_t *buf[2];   // ping-pong packed state (L words per row)
     uint8_t *chg;       // per band change flags, [2][nbands] (pass parity)
     int *st;            // item state (ST_*)
-    int w, h, L;        // L = 64-bit words (lanes) per row
+    int w, h, L;        // L = 64-bit words per row
     int nbands;         // bands covering the image
     int band_rows;      // interior rows per band
     int tile0;          // id of band 0 in the batch's tile arrays
 };
+
+// Lanes a row of L words takes with V words per lane.
+__host__ __device__ constexpr int lanes_for(int L, int V) { return (L + V - 1) / V; }
 
 struct Tile2D {
     int item;
@@ -64,21 +67,37 @@ __device__ __forceinline__ uint32_t from_upper_lane(uint32_t v)
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, true);
 }
 
-struct W64 {
-    uint32_t lo, hi;
+// The 2V 32-bit halves of the V consecutive 64-bit words a lane holds of one
+// row, low pixels first: half i holds pixels 32 i .. 32 i + 31 of the lane.
+template <int V>
+struct Row {
+    uint32_t h[2 * V];
 };
 
-// bit k <- pixel x-1 (the reference's ShiftLeft)
-__device__ __forceinline__ W64 shift_w(W64 v, uint32_t carry_mask)
+// bit k <- pixel x-1 (the reference's ShiftLeft): the halves shifted up by one
+// bit, the lowest taking the top bit of the lane below (masked at segment
+// starts)
+template <int V>
+__device__ __forceinline__ Row<V> shift_w(const Row<V> &v, uint32_t carry_mask)
 {
-    uint32_t c = from_lower_lane(v.hi) & carry_mask;
-    return {__builtin_amdgcn_alignbit(v.lo, c, 31), __builtin_amdgcn_alignbit(v.hi, v.lo, 31)};
+    Row<V> o;
+    const uint32_t c = from_lower_lane(v.h[2 * V - 1]) & carry_mask;
+    o.h[0] = __builtin_amdgcn_alignbit(v.h[0], c, 31);
+#pragma unroll
+    for (int i = 1; i < 2 * V; ++i) o.h[i] = __builtin_amdgcn_alignbit(v.h[i], v.h[i - 1], 31);
+    return o;
 }
-// bit k <- pixel x+1 (the reference's ShiftRight)
-__device__ __forceinline__ W64 shift_e(W64 v, uint32_t carry_mask)
+// bit k <- pixel x+1 (the reference's ShiftRight): shifted down by one bit, the
+// highest taking the bottom bit of the lane above (masked at segment ends)
+template <int V>
+__device__ __forceinline__ Row<V> shift_e(const Row<V> &v, uint32_t carry_mask)
 {
-    uint32_t c = from_upper_lane(v.lo) & carry_mask;
-    return {__builtin_amdgcn_alignbit(v.hi, v.lo, 1), __builtin_amdgcn_alignbit(c, v.hi, 1)};
+    Row<V> o;
+    const uint32_t c = from_upper_lane(v.h[0]) & carry_mask;
+#pragma unroll
+    for (int i = 0; i + 1 < 2 * V; ++i) o.h[i] = __builtin_amdgcn_alignbit(v.h[i + 1], v.h[i], 1);
+    o.h[2 * V - 1] = __builtin_amdgcn_alignbit(c, v.h[2 * V - 1], 1);
+    return o;
 }
 
 // ---- the deletion rule ---------------------------------------------------
@@ -139,19 +158,21 @@ __device__ __forceinline__ uint32_t keep(uint32_t ctr, uint32_t x0, uint32_t x1,
 // One subiteration of direction D over the R register rows of a segment.
 // up_edge / dn_edge are the rows just above / below the segment (new values
 // of the previous subiteration, from the neighbouring segments).
-template <int D, int R>
-__device__ __forceinline__ void subiteration(uint32_t (&lo)[R], uint32_t (&hi)[R], W64 up_edge,
-                                             W64 dn_edge, uint32_t mw, uint32_t me)
+template <int D, int R, int V>
+__device__ __forceinline__ void subiteration(Row<V> (&w)[R], const Row<V> &up_edge,
+                                             const Row<V> &dn_edge, uint32_t mw, uint32_t me)
 {
-    W64 up = up_edge, upw = shift_w(up, mw), upe = shift_e(up, me);
-    W64 cp = {lo[0], hi[0]}, cpw = shift_w(cp, mw), cpe = shift_e(cp, me);
+    Row<V> up = up_edge, upw = shift_w(up, mw), upe = shift_e(up, me);
+    Row<V> cp = w[0], cpw = shift_w(cp, mw), cpe = shift_e(cp, me);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        W64 dn = (r + 1 < R) ? W64{lo[r + 1 < R ? r + 1 : 0], hi[r + 1 < R ? r + 1 : 0]} : dn_edge;
-        W64 dnw = shift_w(dn, mw), dne = shift_e(dn, me);
+        const Row<V> dn = (r + 1 < R) ? w[r + 1 < R ? r + 1 : 0] : dn_edge;
+        const Row<V> dnw = shift_w(dn, mw), dne = shift_e(dn, me);
         // x0=N=up, x1=NE, x2=E, x3=SE, x4=S=dn, x5=SW, x6=W, x7=NW
-        lo[r] = keep<D>(cp.lo, up.lo, upe.lo, cpe.lo, dne.lo, dn.lo, dnw.lo, cpw.lo, upw.lo);
-        hi[r] = keep<D>(cp.hi, up.hi, upe.hi, cpe.hi, dne.hi, dn.hi, dnw.hi, cpw.hi, upw.hi);
+#pragma unroll
+        for (int i = 0; i < 2 * V; ++i)
+            w[r].h[i] = keep<D>(cp.h[i], up.h[i], upe.h[i], cpe.h[i], dne.h[i], dn.h[i], dnw.h[i],
+                                cpw.h[i], upw.h[i]);
         up = cp; upw = cpw; upe = cpe;
         cp = dn; cpw = dnw; cpe = dne;
         // rows are independent; keep the scheduler from interleaving all of
@@ -162,14 +183,17 @@ __device__ __forceinline__ void subiteration(uint32_t (&lo)[R], uint32_t (&hi)[R
 
 // Pixels in the rows of `rows` (bit r = register row r).  Subiterations only
 // clear pixels, so an iteration deleted one of them iff this count dropped.
-template <int R>
-__device__ __forceinline__ uint32_t count_rows(const uint32_t (&lo)[R], const uint32_t (&hi)[R],
-                                               uint32_t rows)
+template <int R, int V>
+__device__ __forceinline__ uint32_t count_rows(const Row<V> (&w)[R], uint32_t rows)
 {
     uint32_t n = 0;
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-        n += ((rows >> r) & 1u) * (uint32_t)(__popc(lo[r]) + __popc(hi[r]));
+    for (int r = 0; r < R; ++r) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 2 * V; ++i) c += (uint32_t)__popc(w[r].h[i]);
+        n += ((rows >> r) & 1u) * c;
+    }
     return n;
 }
 
@@ -201,29 +225,33 @@ struct TileInfo {
 };
 
 // Run one pass over the interior rows [row0, row0 + nrows) of item `it` (a
-// band, or a part of one): K iterations (4K subiterations) from buffer `par`
-// into buffer par^1, or -- when `whole`: the rows are the whole image, so that
-// the halo rows are all outside it -- iterations until one deletes nothing, at
-// most whole_max of them.  The window of the launch (waves x G segments x R
-// rows) must hold nrows + 8K rows.  Results land in `info` (bit j of the mask:
-// iteration j deleted an interior pixel); returns the number of iterations
-// that deleted something in the whole case (else 0).
-template <int R>
+// band, or a part of one) with V words per lane: K iterations (4K
+// subiterations) from buffer `par` into buffer par^1, or -- when `whole`: the
+// rows are the whole image, so that the halo rows are all outside it --
+// iterations until one deletes nothing, at most whole_max of them.  The window
+// of the launch (waves x G segments x R rows, G = floor(64 / lanes per row))
+// must hold nrows + 8K rows, and `edges` 4 * waves * 64 * V words.  Results
+// land in `info` (bit j of the mask: iteration j deleted an interior pixel);
+// returns the number of iterations that deleted something in the whole case
+// (else 0).
+template <int R, int V>
 __device__ int band_work(const Item2D &it, int row0, int nrows, bool whole, int K, int par,
                          uint64_t *edges, TileInfo *info, int whole_max = 0x7fffffff)
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int NW = blockDim.x >> 6;
-    // G = floor(64 / L) segments per wave; the lanes past G*L are idle: they
+    // a row takes LV = ceil(L / V) lanes, lane c holding words c V .. c V +
+    // V - 1 (those past the row read as zero and are never stored); G =
+    // floor(64 / LV) segments per wave; the lanes past G*LV are idle: they
     // hold zeros, load and store nothing and own no interior rows, and their
     // words reach no real lane (segment-end carries are masked)
-    const int L = it.L, G = 64 / L, S = NW * G, SL = NW * 64;
-    const int g = lane / L, c = lane - g * L, q = wv * G + g;
+    const int L = it.L, LV = lanes_for(L, V), G = 64 / LV, S = NW * G, SL = NW * 64;
+    const int g = lane / LV, c = lane - g * LV, q = wv * G + g;
     const bool active = g < G;
     const int H = 4 * K;
     const int y_reg0 = row0 - H + q * R;  // image row of register row 0
     uint32_t mw = c == 0 ? 0u : 0xffffffffu;
-    uint32_t me = c == L - 1 ? 0u : 0xffffffffu;
+    uint32_t me = c == LV - 1 ? 0u : 0xffffffffu;
     // opaque to the compiler, so `dpp(v) & mask` stays a VGPR AND that folds
     // into the DPP instruction instead of becoming a lane-mask select
     asm volatile("" : "+v"(mw), "+v"(me));
@@ -233,28 +261,50 @@ __device__ int band_work(const Item2D &it, int row0, int nrows, bool whole, int 
         (active && rb > ra) ? (uint32_t)(((1ull << rb) - 1ull) & ~((1ull << ra) - 1ull)) : 0u;
     const uint64_t *src = par ? it.buf[1] : it.buf[0];
     uint64_t *dst = par ? it.buf[0] : it.buf[1];
+    const int w0 = c * V;  // first word of this lane
 
-    uint32_t lo[R], hi[R];
+    Row<V> w[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int y = y_reg0 + r;
-        uint64_t v = 0;
-        if (active && y >= 0 && y < it.h) v = src[(size_t)y * L + c];
-        lo[r] = (uint32_t)v;
-        hi[r] = (uint32_t)(v >> 32);
+        const bool row_ok = active && y >= 0 && y < it.h;
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            uint64_t v = 0;
+            if (row_ok && w0 + j < L) v = src[(size_t)y * L + w0 + j];
+            w[r].h[2 * j] = (uint32_t)v;
+            w[r].h[2 * j + 1] = (uint32_t)(v >> 32);
+        }
     }
     // edges written after subiteration s live in buffer (s + 1) & 1; buffer 0
-    // holds the initial rows.  Layout [buf][top/bot][NW*64]; the entry of lane
-    // (g, c) of wave v is v*64 + g*L + c = tid, so segment q's column c is at
-    // (q / G)*64 + (q % G)*L + c.  Idle lanes only read their own entries.
+    // holds the initial rows.  Layout [buf][top/bot][NW*64][V]; the entry of
+    // lane (g, c) of wave v is v*64 + g*LV + c = tid, so segment q's column c
+    // is at (q / G)*64 + (q % G)*LV + c.  Idle lanes only read their own
+    // entries.
     const int me_idx = tid;
     const bool has_up = active && q > 0, has_dn = active && q + 1 < S;
-    const int up_idx = has_up ? ((q - 1) / G) * 64 + ((q - 1) % G) * L + c : me_idx;
-    const int dn_idx = has_dn ? ((q + 1) / G) * 64 + ((q + 1) % G) * L + c : me_idx;
+    const int up_idx = has_up ? ((q - 1) / G) * 64 + ((q - 1) % G) * LV + c : me_idx;
+    const int dn_idx = has_dn ? ((q + 1) / G) * 64 + ((q + 1) % G) * LV + c : me_idx;
     const uint32_t up_ok = has_up ? 0xffffffffu : 0u;
     const uint32_t dn_ok = has_dn ? 0xffffffffu : 0u;
-    edges[(0 * 2 + 0) * SL + me_idx] = ((uint64_t)hi[0] << 32) | lo[0];
-    edges[(0 * 2 + 1) * SL + me_idx] = ((uint64_t)hi[R - 1] << 32) | lo[R - 1];
+    auto put = [&](int b, int tb, const Row<V> &x) {
+        uint64_t *e = edges + ((size_t)(b * 2 + tb) * SL + me_idx) * V;
+#pragma unroll
+        for (int j = 0; j < V; ++j) e[j] = ((uint64_t)x.h[2 * j + 1] << 32) | x.h[2 * j];
+    };
+    auto get = [&](int b, int tb, int idx, uint32_t ok) {
+        const uint64_t *e = edges + ((size_t)(b * 2 + tb) * SL + idx) * V;
+        Row<V> x;
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            const uint64_t v = e[j];
+            x.h[2 * j] = (uint32_t)v & ok;
+            x.h[2 * j + 1] = (uint32_t)(v >> 32) & ok;
+        }
+        return x;
+    };
+    put(0, 0, w[0]);
+    put(0, 1, w[R - 1]);
     if (tid == 0) {
         info->mask = 0;
         info->occ = 0;
@@ -264,25 +314,23 @@ __device__ int band_work(const Item2D &it, int row0, int nrows, bool whole, int 
 
     unsigned bits = 0;
     int s = 0, changed_iters = 0;
-    uint32_t count = count_rows(lo, hi, interior);
+    uint32_t count = count_rows(w, interior);
     const int max_iters = whole ? whole_max : K;
 #pragma unroll 1
     for (int itn = 0; itn < max_iters; ++itn) {
-#define BGT_SUB(D)                                                                           \
-    {                                                                                        \
-        const int rb_ = s & 1, wb_ = (s + 1) & 1;                                            \
-        const uint64_t ue = edges[(rb_ * 2 + 1) * SL + up_idx];                              \
-        const uint64_t de = edges[(rb_ * 2 + 0) * SL + dn_idx];                              \
-        subiteration<D, R>(lo, hi, W64{(uint32_t)ue & up_ok, (uint32_t)(ue >> 32) & up_ok},  \
-                           W64{(uint32_t)de & dn_ok, (uint32_t)(de >> 32) & dn_ok}, mw, me); \
-        edges[(wb_ * 2 + 0) * SL + me_idx] = ((uint64_t)hi[0] << 32) | lo[0];                \
-        edges[(wb_ * 2 + 1) * SL + me_idx] = ((uint64_t)hi[R - 1] << 32) | lo[R - 1];        \
-        __syncthreads();                                                                     \
-        ++s;                                                                                 \
+#define BGT_SUB(D)                                                                  \
+    {                                                                               \
+        const int rb_ = s & 1, wb_ = (s + 1) & 1;                                   \
+        const Row<V> ue = get(rb_, 1, up_idx, up_ok), de = get(rb_, 0, dn_idx, dn_ok); \
+        subiteration<D, R, V>(w, ue, de, mw, me);                                   \
+        put(wb_, 0, w[0]);                                                          \
+        put(wb_, 1, w[R - 1]);                                                      \
+        __syncthreads();                                                            \
+        ++s;                                                                        \
     }
         BGT_SUB(0) BGT_SUB(1) BGT_SUB(2) BGT_SUB(3)
 #undef BGT_SUB
-        const uint32_t now = count_rows(lo, hi, interior);
+        const uint32_t now = count_rows(w, interior);
         const bool ch = now != count;
         count = now;
         if (ch && itn < 32) bits |= 1u << itn;
@@ -304,8 +352,12 @@ __device__ int band_work(const Item2D &it, int row0, int nrows, bool whole, int 
     for (int r = 0; r < R; ++r) {
         const int y = y_reg0 + r;
         if (((interior >> r) & 1u) && y >= 0 && y < it.h) {
-            dst[(size_t)y * L + c] = ((uint64_t)hi[r] << 32) | lo[r];
-            occ |= lo[r] | hi[r];
+#pragma unroll
+            for (int j = 0; j < V; ++j)
+                if (w0 + j < L) {
+                    dst[(size_t)y * L + w0 + j] = ((uint64_t)w[r].h[2 * j + 1] << 32) | w[r].h[2 * j];
+                    occ |= w[r].h[2 * j] | w[r].h[2 * j + 1];
+                }
         }
     }
     if (bits) atomicOr(&info->mask, bits);

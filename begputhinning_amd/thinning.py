@@ -194,6 +194,13 @@ def set_engine3d(engine=1):
     check(lib().bgt_set_engine3d(engine))
 
 
+def set_lane_words2d(words=0):
+    """64-bit words per lane of the 2D pass kernel for later calls (results do
+    not depend on it): 1, 2 or 4, 0 = automatic (the fewest that hold a row
+    in one wave64; widths up to 16384)."""
+    check(lib().bgt_set_lane_words2d(words))
+
+
 def extract_segments_device(skel, padded=None, stream=None):
     """ExtractSegmentsV over a contiguous uint8 torch tensor on the GPU (h x w
     or d x h x w), the top-left corner of a zero-padded frame of shape

@@ -60,6 +60,12 @@ int bgt_set_config3d(int blocks_per_cu, int max_chunk, int tail_words);
  * tiles, one launch per iteration with the whole iteration staged in LDS;
  * 2 = the tiles from the first iteration. */
 int bgt_set_engine3d(int engine);
+/* 64-bit words per lane of the 2D pass kernel (no reference counterpart;
+ * results do not depend on it): 1, 2 or 4, 0 = automatic (the default: the
+ * fewest that hold a row in one wave64, i.e. 1 up to 4096 pixels, 2 up to
+ * 8192, 4 up to 16384, the widest 2D image).  Images wider than the choice
+ * take more; a batch's images all take those of its widest. */
+int bgt_set_lane_words2d(int words);
 
 /* ------------------------------------------------------------------------- */
 /* Drop-in thinning on host buffers.                                          */
@@ -68,8 +74,9 @@ int bgt_set_engine3d(int engine);
 /* Replaces template<class T> void Thinning2D(int width, int height, T *data,
  * T *out) (2D/LibThinning2D.h:13-15, 2D/LibThinning2D.cu:39-130).
  * Foreground iff (unsigned)v > 0 with saturating conversion of floats
- * (2D/CUDA_DeviceHelperFunctions.cuh:101); out holds T(0) / T(1).  Any
- * width/height >= 1 (the reference needs padded widths <= 2048). */
+ * (2D/CUDA_DeviceHelperFunctions.cuh:101); out holds T(0) / T(1).  Widths
+ * 1..16384, heights >= 1 (the reference needs padded widths <= 2048); wider
+ * images fail with BGT_ERR_UNSUPPORTED. */
 int bgt_thinning2d(int dtype, int width, int height, const void *in, void *out);
 
 /* Replaces template<class T> void Thinning3D(int width, int height, int depth,
