@@ -103,9 +103,7 @@ __device__ __forceinline__ Row<V> shift_e(const Row<V> &v, uint32_t carry_mask)
 // ---- the deletion rule ---------------------------------------------------
 
 // v_bitop3_b32: any bitwise function of three operands in one instruction.
-// The truth table is written over the operand columns TA, TB, TC: bit
-// (a<<2 | b<<1 | c) of the table is the result for input bits a, b, c.
-constexpr unsigned TA = 0xf0u, TB = 0xccu, TC = 0xaau;
+// Bit (a<<2 | b<<1 | c) of the table T is the result for input bits a, b, c.
 template <unsigned T>
 __device__ __forceinline__ uint32_t lut3(uint32_t a, uint32_t b, uint32_t c)
 {
@@ -114,30 +112,30 @@ __device__ __forceinline__ uint32_t lut3(uint32_t a, uint32_t b, uint32_t c)
 
 // Pixels of `ctr` that survive the North subiteration, over the clockwise
 // neighbour labels x0..x7 = N, NE, E, SE, S, SW, W, NW.  The North template
-// of 2D/CUDA_SubiterationKernels2D.cuh:15-40 deletes a pixel when
-//   ~x0 & (a | b | c | d | e),  a = ~x1 ~x7 (x2^x3^~x4) (x3^x5) (x3^x6),
+// of 2D/CUDA_SubiterationKernels2D.cuh:15-40 deletes a pixel when ~x0 & F,
+//   F = a | b | c | d | e,  a = ~x1 ~x7 (x2^x3^~x4) (x3^x5) (x3^x6),
 //   b = x2 x4 (x6|~x7),  c = x4 x6 ~x1,  d = ~x6 ~x7 (x1^x5) (x2^x5) (x4^~x5),
-//   e = ~x1 ~x2 ~x3 x6 x7;
-// the result is ctr & ~that, as a network of 14 three-input gates.
+//   e = ~x1 ~x2 ~x3 x6 x7.
+// F, a function of x1..x7, is built here from 8 three-input gates, then
+// ctr & (x0 | ~F) from one more.  The network is a decomposition of F's
+// 128-entry truth table: F = G(x1, x2, W) with W = H(x4, x6, M), M = x5 ? B : A,
+// A and B of (x2, x3, x4, x6, x7) in two and three gates; each gate's table
+// takes its value where the outer gates read it and is free elsewhere.  The
+// tables are the hex constants (bit a<<2 | b<<1 | c of a table is the result
+// for inputs a, b, c); the network equals the template on all 512 inputs.
 __device__ __forceinline__ uint32_t north_keep(uint32_t ctr, uint32_t x0, uint32_t x1, uint32_t x2,
                                                uint32_t x3, uint32_t x4, uint32_t x5, uint32_t x6,
                                                uint32_t x7)
 {
-    const uint32_t g1 = lut3<TA ^ TB ^ ~TC>(x2, x3, x4);          // x2 ^ x3 ^ ~x4
-    const uint32_t g2 = lut3<(TA ^ TB) & (TA ^ TC)>(x3, x5, x6);  // (x3^x5) (x3^x6)
-    const uint32_t g3 = lut3<~TA & ~TB & TC>(x1, x7, g1);         // ~x1 ~x7 g1
-    const uint32_t g4 = lut3<(TA ^ TC) & (TB ^ TC)>(x1, x2, x5);  // (x1^x5) (x2^x5)
-    const uint32_t g5 = lut3<~(TA ^ TB) & ~TC>(x4, x5, x6);       // (x4^~x5) ~x6
-    const uint32_t g6 = lut3<~TA & ~TB & ~TC>(x1, x2, x3);        // ~x1 ~x2 ~x3
-    const uint32_t g7 = lut3<TA & (TB | ~TC)>(x2, x6, x7);        // x2 (x6|~x7)
-    const uint32_t g8 = lut3<(~TA & TB) | TC>(x1, x6, g7);        // b|c = x4 g8
-    const uint32_t d = lut3<TA & TB & ~TC>(g4, g5, x7);
-    const uint32_t e = lut3<TA & TB & TC>(g6, x6, x7);
-    const uint32_t ad = lut3<(TA & TB) | TC>(g2, g3, d);          // a | d
-    const uint32_t bce = lut3<(TA & TB) | TC>(x4, g8, e);         // b | c | e
-    // ctr & ~(~x0 & (ad | bce)) = ctr & (x0 | ~ad) & (x0 | ~bce)
-    const uint32_t k = lut3<TA & (TB | ~TC)>(ctr, x0, ad);
-    return lut3<TA & (TB | ~TC)>(k, x0, bce);
+    const uint32_t t1 = lut3<0x76>(x1, x4, x6);
+    const uint32_t t2 = lut3<0xe8>(x2, x3, t1);
+    const uint32_t ga = lut3<0x42>(x6, x7, t2);  // F where x5 = 0 (and x1 x2 leave it open)
+    const uint32_t t3 = lut3<0x02>(x2, x3, x6);
+    const uint32_t gb = lut3<0x1a>(x4, x7, t3);  // F where x5 = 1 (likewise)
+    const uint32_t m = lut3<0xca>(x5, gb, ga);   // x5 ? gb : ga
+    const uint32_t w = lut3<0xea>(x4, x6, m);
+    const uint32_t f = lut3<0x8a>(x1, x2, w);    // F
+    return lut3<0xd0>(ctr, x0, f);               // ctr & (x0 | ~F)
 }
 
 // direction D: 0 = N, 1 = S, 2 = E, 3 = W (the reference's order,
