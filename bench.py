@@ -35,6 +35,11 @@ DOMINANT = "thin2d_queue_pass"
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a copy of known size (see the file for the runs)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "thin2d_queue_pass.mixed_batch.pmc.txt")
+# VALU instructions per launch of the dominant kernel, from a rocprofv3 SQ
+# counter run (see the file for the run)
+VALU_FILE = os.path.join(ROOT, "profiles", "thin2d_queue_pass.mixed_batch.sq.txt")
+# one wave64 VALU instruction per cycle per CU: 256 CUs x 2.4 GHz
+VALU_CEIL = 256 * 2.4e9
 
 
 def log(*a):
@@ -54,13 +59,13 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()[:16]
 
 
-def measured_traffic():
-    """(HBM bytes per launch of the dominant kernel, launch shape it was
-    measured with) from the committed counter summary, or (None, None)."""
-    if not os.path.exists(TRAFFIC_FILE):
+def measured_per_launch(path, key):
+    """(figure `key` per launch of the dominant kernel, launch shape it was
+    measured with) from a committed counter summary, or (None, None)."""
+    if not os.path.exists(path):
         return None, None
-    text = open(TRAFFIC_FILE).read()
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
+    text = open(path).read()
+    m = re.search(r"^" + key + r":\s*([0-9.eE+]+)", text, re.M)
     sh = re.search(r"^launch_shape:\s*waves=(\d+) rows=(\d+) iters_per_pass=(\d+)", text, re.M)
     shape = tuple(int(v) for v in sh.groups()) if sh else None
     return (float(m.group(1)) if m else None), shape
@@ -206,6 +211,34 @@ def serve_leg(args, torch, B, synth, dev):
             "max_qps_1024x1024_p99_under_5ms": max(ok) if ok else None}
 
 
+def frame_leg(args, torch, B, synth, dev):
+    """p50/p99 latency of 1024x1024 tree frames (distinct seeds) through the
+    per-image device path, and its algorithmic HBM rate at p50: the frames'
+    mean 2N + 2 ceil(N/8) I bytes (N pixels, I iterations) per p50 latency."""
+    frames = [torch.from_numpy(synth.workload2(i)).to(dev) for i in range(min(args.frames, 64))]
+    fout = torch.empty_like(frames[0])
+    iters = [B.thin2d_device(f, out=fout)[1] for f in frames]
+    torch.cuda.synchronize()
+    lat = []
+    for k in range(args.frames):
+        f = frames[k % len(frames)]
+        t2 = time.perf_counter()
+        B.thin2d_device(f, out=fout)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t2) * 1e3)
+    p50 = float(np.percentile(lat, 50))
+    p99 = float(np.percentile(lat, 99))
+    n = frames[0].numel()
+    b_alg = float(np.mean([2 * n + 2 * ((n + 7) // 8) * i for i in iters]))
+    gbs = b_alg / (p50 / 1e3) / 1e9
+    log(f"1024x1024 frame latency p50 {p50:.3f} ms p99 {p99:.3f} ms; mean {np.mean(iters):.1f} "
+        f"iterations, {gbs:.1f} GB/s algorithmic at p50")
+    return {"p50": round(p50, 3), "p99": round(p99, 3), "frames": args.frames,
+            "path": "per-image device-resident", "mean_iterations": round(float(np.mean(iters)), 2),
+            "hbm_gbs_algorithmic_at_p50": round(gbs, 1),
+            "b_alg_bytes": int(b_alg)}
+
+
 def host_cores():
     """Threads the host gives this process: the affinity mask, capped by
     OMP_NUM_THREADS where the environment sets the process's CPU share."""
@@ -280,6 +313,9 @@ def main():
                     default=[1000.0, 2000.0, 4000.0, 6000.0, 8000.0, 12000.0],
                     help="Poisson arrival rates of the serving leg (none or 0 = skip it)")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--only-frames", action="store_true",
+                    help="run only the 1024x1024 per-image latency leg (for counter runs of the "
+                         "per-image path) and print its figures")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, to scale hardware byte counters)")
@@ -301,6 +337,9 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU"
     dev = torch.device("cuda:0")
+    if args.only_frames:
+        print(json.dumps({"latency_1024x1024_ms": frame_leg(args, torch, B, synth, dev)}))
+        return
     t0 = time.time()
     items, n2 = gen_items()
     total_vox = sum(a.size for a in items)
@@ -394,22 +433,7 @@ def main():
         per_image = {"value": round(per_image_value, 1), "unit": "Mvoxel/s",
                      "ratio_batched_over_per_image": round(value / per_image_value, 2)}
 
-        # ---- latency of 1024x1024 frames through the per-image path ---------
-        frames = [torch.from_numpy(synth.workload2(i)).to(dev) for i in range(min(args.frames, 64))]
-        fout = torch.empty_like(frames[0])
-        torch.cuda.synchronize()
-        lat = []
-        for k in range(args.frames):
-            f = frames[k % len(frames)]
-            t2 = time.perf_counter()
-            B.thin2d_device(f, out=fout)
-            torch.cuda.synchronize()
-            lat.append((time.perf_counter() - t2) * 1e3)
-        p50 = float(np.percentile(lat, 50))
-        p99 = float(np.percentile(lat, 99))
-        log(f"1024x1024 frame latency p50 {p50:.3f} ms p99 {p99:.3f} ms")
-        latency = {"p50": round(p50, 3), "p99": round(p99, 3), "frames": args.frames,
-                   "path": "per-image device-resident"}
+        latency = frame_leg(args, torch, B, synth, dev)
 
         # ---- the drop-in CenterlineExtraction on host arrays (C2, C3) -------
         cl = {"path": "drop-in CenterlineExtraction(data, 10, 3) on host uint8 arrays: "
@@ -449,7 +473,8 @@ def main():
     if not args.no_cpu_baseline:
         cpu, exact = cpu_baseline(items, got, iters, total_vox)
 
-    traffic, traffic_shape = measured_traffic()
+    traffic, traffic_shape = measured_per_launch(TRAFFIC_FILE, "traffic_bytes_per_launch")
+    valu, valu_shape = measured_per_launch(VALU_FILE, "valu_insts_per_launch")
     # the library's default 2D shape (4 waves, 16 rows, K = 3) unless set by its environment
     shape = ((tuned["waves"], tuned["rows"], tuned["iters_per_pass"]) if tuned
              else tuple(int(os.environ.get(k, v)) for k, v in
@@ -458,6 +483,12 @@ def main():
         log(f"traffic figure was measured at launch shape {traffic_shape}, this run uses "
             f"{shape}: not reported")
         traffic = None
+    if valu is not None and valu_shape != shape:
+        valu = None
+    # VALU issue: the profiled instructions per launch over what the CUs can
+    # issue in the live average launch time
+    valu_frac = (round(valu / (avg_launch_s * VALU_CEIL), 4)
+                 if valu is not None and avg_launch_s > 0 else None)
     line = {
         "metric": "bit-exact thinned Mvoxel/s, batched vs per-image, 1 MI355X; p50/p99 ms",
         "value": round(value, 1),
@@ -486,7 +517,9 @@ def main():
                      "traffic": traffic, "launches": l2d,
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                      "bytes_per_launch": int(bytes_per_launch),
-                     "timing": "HIP events around each launch on its stream, one extra step"},
+                     "timing": "HIP events around each launch on its stream, one extra step",
+                     "valu_issue_frac": valu_frac,
+                     "valu_insts_per_launch": valu},
         "three_d_iterations": {"launch_groups": l3d, "ms": round(ms3d, 3),
                                "avg_iteration_ms": round(ms3d / max(l3d, 1), 4)},
         "b_alg": {"bytes": int(b_alg), "frac_of_hbm_peak_per_step": round(
