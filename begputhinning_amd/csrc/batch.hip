@@ -121,6 +121,15 @@ __global__ void batch_build_list(Queue q)
     q.list[0][k] = t;
 }
 
+// queue of pass 0 = every tile (the passes pack the images themselves, so no
+// occupancy is known before it)
+__global__ void batch_list_all(Queue q)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < q.ntiles) q.list[0][t] = t;
+    if (t == 0) q.cnt[0] = q.ntiles;
+}
+
 __device__ __forceinline__ void enqueue(const Queue &q, int u, int next_pass, int *nlist,
                                         int *ncnt, bool force)
 {
@@ -146,6 +155,7 @@ __device__ __forceinline__ void close_item(const Close2D &c, int i, int K, int P
     }
 }
 
+
 // the item skips pass P when an iteration of its pass P - 1 deleted nothing
 __device__ __forceinline__ bool converged_before(const Close2D &c, const unsigned *hprev, int i,
                                                  int K, int P)
@@ -160,11 +170,21 @@ constexpr int kParts = 16;
 // bands.  Workgroups 0 .. gridDim.x - 2 take bands; the last workgroup does
 // the pass's bookkeeping (closes the items of pass P - 2, clears the idle
 // queue counters and history slot) and takes no band.
-template <int R, int V>
+
+// output in the pass that leaves them final (band_work), or, when their item
+// converged before the pass, from the packed state here: no pack or unpack
+// launch runs.
+// IO = 2 (V = 1 only): the kernel of pass 0 of a batch whose items all start
+// there, which packs the images from the caller's bytes (`io`) itself and
+// queues every neighbour of a changed band (band occupancy is only being
+// found in that pass): no pack launch runs.  A kernel of its own, so that
+// the other passes carry none of that code.  IO = 0: `io` is null.
+template <int R, int V, int IO>
 __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restrict__ items,
                                                           const Tile2D *__restrict__ tiles,
                                                           Queue q, Close2D c, int K,
-                                                          const int *pbase, int p)
+                                                          const int *pbase, int p,
+                                                          const ItemIO2D *__restrict__ io)
 {
     extern __shared__ uint64_t edges[];
     __shared__ TileInfo info;
@@ -210,6 +230,7 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         const Tile2D tl = tiles[tile];
         if (converged_before(c, hprev, tl.item, K, P)) return;
         const Item2D it = items[tl.item];
+        const ItemIO2D *fio = io ? io + tl.item : nullptr;
         const int row0 = tl.band * it.band_rows;
         const int seg = (int)(blockDim.x >> 6) * (64 / lanes_for(it.L, V));  // segments per window
         const int in4 = seg * 4 - 8 * K, in8 = seg * 8 - 8 * K;  // interior rows of a part
@@ -217,15 +238,16 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         const int np8 = in8 >= 1 ? (it.band_rows + in8 - 1) / in8 : kParts + 1;
         if (np4 <= kp) {
             if (part >= np4 || row0 + part * in4 >= it.h) return;
-            band_work<4, V>(it, row0 + part * in4, min(in4, it.band_rows - part * in4), false, K,
-                         P & 1, edges, &info);
+            band_work<4, V, IO>(it, row0 + part * in4, min(in4, it.band_rows - part * in4), false,
+                                   K, P & 1, edges, &info, 0x7fffffff, fio);
         } else if (R > 8 && np8 <= kp) {
             if (part >= np8 || row0 + part * in8 >= it.h) return;
-            band_work<8, V>(it, row0 + part * in8, min(in8, it.band_rows - part * in8), false, K,
-                         P & 1, edges, &info);
+            band_work<8, V, IO>(it, row0 + part * in8, min(in8, it.band_rows - part * in8), false,
+                                   K, P & 1, edges, &info, 0x7fffffff, fio);
         } else {  // windows too short for parts: one workgroup per band
             if (part != 0) return;
-            band_work<R, V>(it, row0, it.band_rows, false, K, P & 1, edges, &info);
+            band_work<R, V, IO>(it, row0, it.band_rows, false, K, P & 1, edges, &info, 0x7fffffff,
+                                   fio);
         }
         if (tid == 0) {
             // parts only set the flag: a band emptied in a tail pass stays
@@ -236,7 +258,8 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         // a part that changed queues its band and the band's neighbours
         if (info.mask && tid < 3) {
             const int b = tl.band - 1 + tid;
-            if (b >= 0 && b < it.nbands) enqueue(q, it.tile0 + b, P + 1, nlist, ncnt, tid == 1);
+            if (b >= 0 && b < it.nbands)
+                enqueue(q, it.tile0 + b, P + 1, nlist, ncnt, tid == 1 || IO == 2);
         }
         return;
     }
@@ -262,8 +285,8 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         const Item2D it = items[tl.item];
         // a single-band item also runs at most K iterations per pass, so that no
         // pass waits on one item iterating to convergence while CUs idle
-        band_work<R, V>(it, tl.band * it.band_rows, it.band_rows, it.nbands == 1, K, P & 1, edges,
-                     &info, K);
+        band_work<R, V, IO>(it, tl.band * it.band_rows, it.band_rows, it.nbands == 1, K, P & 1,
+                               edges, &info, K, io ? io + tl.item : nullptr);
         if (tid == 0) {
             q.occ[tile] = info.occ != 0;
             if (info.mask) atomicOr(&hcur[tl.item], info.mask);
@@ -273,7 +296,8 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
         }
         if (info.mask && it.nbands > 1 && tid < 3) {
             const int b = tl.band - 1 + tid;
-            if (b >= 0 && b < it.nbands) enqueue(q, it.tile0 + b, P + 1, nlist, ncnt, tid == 1);
+            if (b >= 0 && b < it.nbands)
+                enqueue(q, it.tile0 + b, P + 1, nlist, ncnt, tid == 1 || IO == 2);
         }
         __syncthreads();
     }
@@ -349,6 +373,7 @@ struct Batch {
     // 2D pipeline: band work queue
     Item2D *d_items2 = nullptr;
     IO2D *d_io2 = nullptr;
+    ItemIO2D *d_fio2 = nullptr;  // caller buffers for passes with fused pack/unpack
     int *d_blk2 = nullptr;   // pack block -> item
     int nblk2 = 0;
     int *d_ublk2 = nullptr;  // unpack block -> item
@@ -358,6 +383,9 @@ struct Batch {
     Close2D close2{};
     int grid2 = 0;          // band workgroups of a pass (one more does the bookkeeping)
     int V2 = 1;             // words per lane of every 2D item's bands
+    // pass 0 packs the images (Config2D::fused_io, one word per lane only;
+    // wider batches keep the pack launch)
+    bool fpack2 = false;
     int *d_states2 = nullptr;
     int *d_gpass2 = nullptr;  // pass number base of graph replays
     int pass2 = 0;            // 2D passes queued in this run
@@ -477,6 +505,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
         if (!v) return fail(BGT_ERR_UNSUPPORTED, "2D width %d exceeds %d", b.in2[i].w, kMaxWidth2D);
         b.V2 = std::max(b.V2, v);
     }
+    b.fpack2 = b.V2 == 1 && b.c2.fused_io != 0;
     std::vector<Plan2D> p2(n2);
     for (int i = 0; i < n2; ++i) BGT_CHECK(plan2d(b.in2[i].w, b.in2[i].h, b.c2, p2[i], b.V2));
     BGT_CHECK(check_config3d(b.c3));
@@ -530,6 +559,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
         // 2D
         b.d_items2 = cv.take<Item2D>(sizeof(Item2D) * n2);
         b.d_io2 = cv.take<IO2D>(sizeof(IO2D) * n2);
+        b.d_fio2 = cv.take<ItemIO2D>(sizeof(ItemIO2D) * n2);
         b.d_blk2 = cv.take<int>(sizeof(int) * blk2.size());
         b.d_ublk2 = cv.take<int>(sizeof(int) * ublk2.size());
         b.d_tiles2 = cv.take<Tile2D>(sizeof(Tile2D) * t2.size());
@@ -651,6 +681,10 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
         if (n2) {
             stage(b.d_items2, h2.data(), sizeof(Item2D) * n2);
             stage(b.d_io2, io2.data(), sizeof(IO2D) * n2);
+            std::vector<ItemIO2D> fio2(n2);
+            for (int i = 0; i < n2; ++i)
+                fio2[i] = ItemIO2D{io2[i].in, io2[i].in_pitch, io2[i].out, io2[i].out_pitch};
+            stage(b.d_fio2, fio2.data(), sizeof(ItemIO2D) * n2);
             stage(b.d_blk2, blk2.data(), sizeof(int) * blk2.size());
             stage(b.d_ublk2, ublk2.data(), sizeof(int) * ublk2.size());
             stage(b.d_tiles2, t2.data(), sizeof(Tile2D) * t2.size());
@@ -728,26 +762,31 @@ static int timed_mark(Batch &b, hipStream_t s, int kind, bool begin, size_t *ope
 }
 
 // the queue pass kernel for `rows` register rows and V words per lane
-static const void *queue_pass_fn(int rows, int V)
+// (the pass-0 kernel that packs from the caller's bytes when `first`; V = 1 only)
+static const void *queue_pass_fn(int rows, int V, bool first = false)
 {
     if (V == 4)
-        return rows == 8 ? (const void *)thin2d_queue_pass<8, 4> : (const void *)thin2d_queue_pass<16, 4>;
+        return rows == 8 ? (const void *)thin2d_queue_pass<8, 4, 0> : (const void *)thin2d_queue_pass<16, 4, 0>;
     if (V == 2)
-        return rows == 8 ? (const void *)thin2d_queue_pass<8, 2> : (const void *)thin2d_queue_pass<16, 2>;
-    return rows == 8 ? (const void *)thin2d_queue_pass<8, 1> : (const void *)thin2d_queue_pass<16, 1>;
+        return rows == 8 ? (const void *)thin2d_queue_pass<8, 2, 0> : (const void *)thin2d_queue_pass<16, 2, 0>;
+    if (first)
+        return rows == 8 ? (const void *)thin2d_queue_pass<8, 1, 2> : (const void *)thin2d_queue_pass<16, 1, 2>;
+    return rows == 8 ? (const void *)thin2d_queue_pass<8, 1, 0> : (const void *)thin2d_queue_pass<16, 1, 0>;
 }
 
 int launch_queue_pass2d(const Config2D &c2, int V, int grid, const Item2D *items,
                         const Tile2D *tiles, const Queue &q, const Close2D &c, const int *pbase,
-                        int p, hipStream_t s)
+                        int p, hipStream_t s, const ItemIO2D *io, bool first)
 {
     const dim3 g(grid + 1), blk(c2.waves * 64);
     int K = c2.iters_per_pass;
     Queue qq = q;
     Close2D cc = c;
-    void *args[] = {(void *)&items, (void *)&tiles, (void *)&qq, (void *)&cc,
-                    (void *)&K,     (void *)&pbase, (void *)&p};
-    BGT_HIP(hipLaunchKernel(queue_pass_fn(c2.rows, V), g, blk, args, pass_lds_bytes2d(c2, V), s));
+    void *args[] = {(void *)&items, (void *)&tiles, (void *)&qq,    (void *)&cc,
+                    (void *)&K,     (void *)&pbase, (void *)&p,     (void *)&io};
+    if (first && (!io || V != 1))
+        return fail(BGT_ERR_INVALID, "the packing 2D pass needs caller buffers and one word per lane");
+    BGT_HIP(hipLaunchKernel(queue_pass_fn(c2.rows, V, first), g, blk, args, pass_lds_bytes2d(c2, V), s));
     return BGT_OK;
 }
 
@@ -762,7 +801,10 @@ static int launch_pass2d(Batch &b, hipStream_t s, const int *pbase, int p)
 {
     size_t mk = 0;
     BGT_CHECK(timed_mark(b, s, 0, true, &mk));
-    BGT_CHECK(launch_queue_pass2d(b.c2, b.V2, b.grid2, b.d_items2, b.d_tiles2, b.q2, b.close2, pbase, p, s));
+    // fused pack: pass 0 (always a direct launch) reads the caller's bytes
+    const bool first = b.fpack2 && !pbase && p == 0;
+    BGT_CHECK(launch_queue_pass2d(b.c2, b.V2, b.grid2, b.d_items2, b.d_tiles2, b.q2, b.close2, pbase, p, s,
+                                  first ? b.d_fio2 : nullptr, first));
     BGT_CHECK(timed_mark(b, s, 0, false, &mk));
     return BGT_OK;
 }
@@ -831,6 +873,14 @@ static int launch_chunk(Batch &b, bool three_d, bool use_graph, hipStream_t s)
     };
     if (!three_d) b.pass2 += kChunk;
     if (!use_graph) return body(s);
+    if (!three_d && p0 == 0 && b.fpack2) {
+        // the chunk holding pass 0 (its own kernel) runs as direct launches;
+        // replays of the captured chunks then number their passes from kChunk
+        for (int i = 0; i < kChunk; ++i) BGT_CHECK(launch_pass2d(b, s, nullptr, i));
+        hipLaunchKernelGGL(batch_advance, dim3(1), dim3(64), 0, s, b.d_gpass2, kChunk);
+        BGT_HIP(hipGetLastError());
+        return BGT_OK;
+    }
     if (!three_d) {
         if (!b.graph2 || b.graph2_stream != s) {
             if (b.graph2) (void)hipGraphExecDestroy(b.graph2);
@@ -899,12 +949,13 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         BGT_HIP(hipEventRecord(b.ev_fork, s));
         BGT_HIP(hipStreamWaitEvent(b.side, b.ev_fork, 0));
     }
-    if (b.nblk2)
+    const bool fpack2 = b.fpack2;
+    if (b.nblk2 && !fpack2)
         hipLaunchKernelGGL(batch_pack2d, dim3(b.nblk2), dim3(256), 0, s, b.d_items2, b.d_io2,
                            b.d_blk2, b.d_occ);
     if (n2)
-        hipLaunchKernelGGL(batch_build_list, dim3((unsigned)((b.ntiles2 + 255) / 256)), dim3(256),
-                           0, s, b.q2);
+        hipLaunchKernelGGL(fpack2 ? batch_list_all : batch_build_list,
+                           dim3((unsigned)((b.ntiles2 + 255) / 256)), dim3(256), 0, s, b.q2);
     BGT_HIP(hipGetLastError());
     b.on_tiles3 = n3 > 0 && b.c3.engine == 2;
     if (b.on_tiles3) {
@@ -1194,6 +1245,7 @@ int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D 
     auto trial = [&](const Config2D &t2, const Config3D &t3) {
         if (n && t2.waves == c2.waves && t2.rows == c2.rows && t2.iters_per_pass == c2.iters_per_pass &&
             t2.blocks_per_cu == c2.blocks_per_cu && t2.lane_words == c2.lane_words &&
+            t2.fused_io == c2.fused_io &&
             t3.blocks_per_cu == c3.blocks_per_cu)
             return;  // the current shape, already timed
         if (batch_set_config(b, t2, t3) != BGT_OK) return;

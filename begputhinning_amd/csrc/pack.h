@@ -12,14 +12,19 @@
 
 namespace bgt {
 
-// 4 packed bytes -> 4 bits (bit k set iff byte k is nonzero)
-__device__ __forceinline__ uint32_t nz4(uint32_t x)
+// 16 packed bytes -> 16 bits (bit k set iff byte k is nonzero).  Per dword,
+// bit 7 of each byte becomes "byte nonzero" ((b & 0x7f) + 0x7f carries into
+// bit 7 unless the low seven bits are zero; or-ing b adds its own bit 7), and
+// v_dot4_u32_u8 sums the four flag bytes (128 each) weighted 1, 2, 4, 8 (and
+// 16 .. 128 for the next dword): 128 x the flags of eight bytes per chain.
+__device__ __forceinline__ uint32_t nz16(uint4 v)
 {
-    uint32_t t = x | (x >> 4);
-    t |= t >> 2;
-    t |= t >> 1;
-    t &= 0x01010101u;
-    return (t | (t >> 7) | (t >> 14) | (t >> 21)) & 0xfu;
+    auto f = [](uint32_t x) { return (((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u; };
+    const uint32_t lo = __builtin_amdgcn_udot4(f(v.y), 0x80402010u,
+                                               __builtin_amdgcn_udot4(f(v.x), 0x08040201u, 0u, false), false);
+    const uint32_t hi = __builtin_amdgcn_udot4(f(v.w), 0x80402010u,
+                                               __builtin_amdgcn_udot4(f(v.z), 0x08040201u, 0u, false), false);
+    return (lo >> 7) | (hi << 1);
 }
 
 // 4 bits -> 4 bytes of 0/1
@@ -44,7 +49,7 @@ __device__ __forceinline__ uint64_t pack_run_u8(const uint8_t *p, int n)
     for (int k = 0; k < 5; ++k) {
         if (k < nq) {
             const uint4 v = q[k];
-            const uint32_t bits = nz4(v.x) | nz4(v.y) << 4 | nz4(v.z) << 8 | nz4(v.w) << 12;
+            const uint32_t bits = nz16(v);
             m[k >> 1] |= bits << ((k & 1) * 16);
         }
     }
@@ -89,7 +94,7 @@ __device__ __forceinline__ uint64_t pack_runs_wave_u8(const uint8_t *p, int n, b
         uint32_t m = 0;
         if (k < nb) {
             const uint4 v = q[k];
-            m = nz4(v.x) | nz4(v.y) << 4 | nz4(v.z) << 8 | nz4(v.w) << 12;
+            m = nz16(v);
         }
         lds[k] = m;
     }

@@ -45,8 +45,11 @@ struct Close2D {
 // One pass P = (pbase ? *pbase : 0) + p of the band queue (thin2d_queue_pass
 // in batch.hip, every item's bands planned with V words per lane) on `grid`
 // band workgroups plus the bookkeeping one.
+// `first`: pass 0 of a batch whose items all start there, packing the items
+// from the caller's input `io` (thin2d_queue_pass, one word per lane).
 int launch_queue_pass2d(const Config2D &c2, int V, int grid, const Item2D *items, const Tile2D *tiles,
-                        const Queue &q, const Close2D &c, const int *pbase, int p, hipStream_t s);
+                        const Queue &q, const Close2D &c, const int *pbase, int p, hipStream_t s,
+                        const ItemIO2D *io = nullptr, bool first = false);
 // Band workgroups of a pass: as many as fit (at most per_cu_cap per CU when
 // > 0) and no more than ntiles.
 int queue_pass_grid2d(const Config2D &c2, int V, int ntiles, int per_cu_cap);

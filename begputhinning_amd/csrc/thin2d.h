@@ -23,6 +23,10 @@ struct Config2D {
     // 64-bit words per lane (1, 2 or 4), 0 = automatic: the fewest that hold
     // a row in one wave64.  Rows wider than the choice take more.
     int lane_words = 0;
+    // batches (one word per lane): 1 = pass 0 packs the images from the
+    // caller's bytes itself, 0 = a separate pack launch (faster: the pack
+    // runs at full occupancy, pass 0's band loads at the pass's)
+    int fused_io = 0;
 };
 
 // Widest 2D image: a row in one wave64 of lanes of four 64-bit words.

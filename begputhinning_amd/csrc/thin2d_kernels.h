@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "pack.h"
+
 namespace bgt {
 
 // Per-item state words.  ST_MASK collects, over the tiles of a pass, bit j =
@@ -51,6 +53,15 @@ __host__ __device__ constexpr int lanes_for(int L, int V) { return (L + V - 1) /
 struct Tile2D {
     int item;
     int band;
+};
+
+// Caller buffers of an item (w uint8 pixels per row, rows at any alignment
+// and pitch) whose first pass packs its band windows from `in` itself.
+struct ItemIO2D {
+    const uint8_t *in;
+    size_t in_pitch;
+    uint8_t *out;
+    size_t out_pitch;
 };
 
 // ---- lane exchange -------------------------------------------------------
@@ -232,9 +243,15 @@ struct TileInfo {
 // land in `info` (bit j of the mask: iteration j deleted an interior pixel);
 // returns the number of iterations that deleted something in the whole case
 // (else 0).
-template <int R, int V>
-__device__ int band_work(const Item2D &it, int row0, int nrows, bool whole, int K, int par,
-                         uint64_t *edges, TileInfo *info, int whole_max = 0x7fffffff)
+//
+// IO = 2 (with `io`, the item's caller buffers; the item's first pass): the
+// window comes from io->in, the interior rows go to both buffers, as a pack
+// would leave them, and an interior found empty skips the iterations
+// (nothing can change in it).  Otherwise the packed state alone.
+template <int R, int V, int IO = 0>
+__device__ __forceinline__ int band_work(const Item2D &it, int row0, int nrows, bool whole, int K,
+                                         int par, uint64_t *edges, TileInfo *info,
+                                         int whole_max = 0x7fffffff, const ItemIO2D *io = nullptr)
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int NW = blockDim.x >> 6;
@@ -257,21 +274,41 @@ __device__ int band_work(const Item2D &it, int row0, int nrows, bool whole, int 
     const int ra = max(0, H - q * R), rb = min(R, H + nrows - q * R);
     const uint32_t interior =
         (active && rb > ra) ? (uint32_t)(((1ull << rb) - 1ull) & ~((1ull << ra) - 1ull)) : 0u;
-    const uint64_t *src = par ? it.buf[1] : it.buf[0];
+    uint64_t *src = par ? it.buf[1] : it.buf[0];
     uint64_t *dst = par ? it.buf[0] : it.buf[1];
     const int w0 = c * V;  // first word of this lane
+    constexpr bool bytes_in = IO == 2;
 
     Row<V> w[R];
+    if (bytes_in) {
+        const uint8_t *in = io->in;
+        const size_t in_pitch = io->in_pitch;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int y = y_reg0 + r;
-        const bool row_ok = active && y >= 0 && y < it.h;
+        for (int r = 0; r < R; ++r) {
+            const int y = y_reg0 + r;
+            const bool row_ok = active && y >= 0 && y < it.h;
 #pragma unroll
-        for (int j = 0; j < V; ++j) {
-            uint64_t v = 0;
-            if (row_ok && w0 + j < L) v = src[(size_t)y * L + w0 + j];
-            w[r].h[2 * j] = (uint32_t)v;
-            w[r].h[2 * j + 1] = (uint32_t)(v >> 32);
+            for (int j = 0; j < V; ++j) {
+                uint64_t v = 0;
+                if (row_ok && w0 + j < L) {
+                    const int x0 = 64 * (w0 + j);
+                    v = pack_run_u8(in + (size_t)y * in_pitch + x0, min(64, it.w - x0));
+                }
+                w[r].h[2 * j] = (uint32_t)v;
+                w[r].h[2 * j + 1] = (uint32_t)(v >> 32);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int y = y_reg0 + r;
+            const bool row_ok = active && y >= 0 && y < it.h;
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const uint64_t v = row_ok && w0 + j < L ? src[(size_t)y * L + w0 + j] : 0ull;
+                w[r].h[2 * j] = (uint32_t)v;
+                w[r].h[2 * j + 1] = (uint32_t)(v >> 32);
+            }
         }
     }
     // edges written after subiteration s live in buffer (s + 1) & 1; buffer 0
@@ -313,7 +350,9 @@ __device__ int band_work(const Item2D &it, int row0, int nrows, bool whole, int 
     unsigned bits = 0;
     int s = 0, changed_iters = 0;
     uint32_t count = count_rows(w, interior);
-    const int max_iters = whole ? whole_max : K;
+    // an empty interior stays empty (subiterations only delete)
+    const bool empty = IO == 2 && !__syncthreads_or(count != 0u);
+    const int max_iters = empty ? 0 : whole ? whole_max : K;
 #pragma unroll 1
     for (int itn = 0; itn < max_iters; ++itn) {
 #define BGT_SUB(D)                                                                  \
@@ -353,7 +392,9 @@ __device__ int band_work(const Item2D &it, int row0, int nrows, bool whole, int 
 #pragma unroll
             for (int j = 0; j < V; ++j)
                 if (w0 + j < L) {
-                    dst[(size_t)y * L + w0 + j] = ((uint64_t)w[r].h[2 * j + 1] << 32) | w[r].h[2 * j];
+                    const uint64_t v = ((uint64_t)w[r].h[2 * j + 1] << 32) | w[r].h[2 * j];
+                    dst[(size_t)y * L + w0 + j] = v;
+                    if (bytes_in) src[(size_t)y * L + w0 + j] = v;
                     occ |= w[r].h[2 * j] | w[r].h[2 * j + 1];
                 }
         }

@@ -201,6 +201,13 @@ def set_lane_words2d(words=0):
     check(lib().bgt_set_lane_words2d(words))
 
 
+def set_fused_io2d(on=False):
+    """Batches of 2D images up to 4096 wide: the first band pass reads the
+    input bytes itself (True) or a separate pack launch does (False, the
+    default).  Results do not depend on it."""
+    check(lib().bgt_set_fused_io2d(int(bool(on))))
+
+
 def extract_segments_device(skel, padded=None, stream=None):
     """ExtractSegmentsV over a contiguous uint8 torch tensor on the GPU (h x w
     or d x h x w), the top-left corner of a zero-padded frame of shape

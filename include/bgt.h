@@ -66,6 +66,14 @@ int bgt_set_engine3d(int engine);
  * 8192, 4 up to 16384, the widest 2D image).  Images wider than the choice
  * take more; a batch's images all take those of its widest. */
 int bgt_set_lane_words2d(int words);
+/* Batches of 2D images up to 4096 wide (no reference counterpart; results do
+ * not depend on it): 1 = the first band pass reads the caller's bytes
+ * itself, the conversion of EncodeBitsKernel
+ * (2D/CUDA_DeviceHelperFunctions.cuh:85-130) fused into the first thinning
+ * pass; 0 (default, 3% faster on the mixed batch) = a separate pack launch.
+ * Batches take it when created or when bgt_batch_use_current_config is
+ * called. */
+int bgt_set_fused_io2d(int on);
 
 /* ------------------------------------------------------------------------- */
 /* Drop-in thinning on host buffers.                                          */
