@@ -73,6 +73,7 @@ SIGNATURES = {
     "bgt_set_engine3d": (_i, [_i]),
     "bgt_set_lane_words2d": (_i, [_i]),
     "bgt_set_fused_io2d": (_i, [_i]),
+    "bgt_set_persistent2d": (_i, [_i]),
     "bgt_batch_create": (_i, [ctypes.POINTER(ctypes.c_void_p)]),
     "bgt_batch_use_current_config": (_i, [_vp]),
     "bgt_batch_add2d": (_i, [_vp, _vp, _sz, _i, _i, _vp, _sz]),

@@ -166,31 +166,25 @@ __device__ __forceinline__ bool converged_before(const Close2D &c, const unsigne
 // tail passes: the most workgroups a band is split over
 constexpr int kParts = 16;
 
-// One pass P = (*pbase if pbase else 0) + p of the 2D batch over the queued
-// bands.  Workgroups 0 .. gridDim.x - 2 take bands; the last workgroup does
-// the pass's bookkeeping (closes the items of pass P - 2, clears the idle
-// queue counters and history slot) and takes no band.
-
-// output in the pass that leaves them final (band_work), or, when their item
-// converged before the pass, from the packed state here: no pack or unpack
-// launch runs.
-// IO = 2 (V = 1 only): the kernel of pass 0 of a batch whose items all start
-// there, which packs the images from the caller's bytes (`io`) itself and
-// queues every neighbour of a changed band (band occupancy is only being
-// found in that pass): no pack launch runs.  A kernel of its own, so that
-// the other passes carry none of that code.  IO = 0: `io` is null.
+// One pass P of the 2D batch over the queued bands (the body of a pass
+// launch, or of one pass of the persistent chain).  Workgroups
+// 0 .. nwg - 1 take bands; workgroup nwg does the pass's bookkeeping (closes
+// the items of pass P - 2, clears the idle queue counters and history slot)
+// and takes no band.
+//
+// IO = 2 (V = 1 only): pass 0 of a batch whose items all start there, which
+// packs the images from the caller's bytes (`io`) itself and queues every
+// neighbour of a changed band (band occupancy is only being found in that
+// pass): no pack launch runs.  A kernel of its own, so that the other passes
+// carry none of that code.  IO = 0: `io` is null.
 template <int R, int V, int IO>
-__global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restrict__ items,
-                                                          const Tile2D *__restrict__ tiles,
-                                                          Queue q, Close2D c, int K,
-                                                          const int *pbase, int p,
-                                                          const ItemIO2D *__restrict__ io)
+__device__ __forceinline__ void queue_pass_body(const Item2D *__restrict__ items,
+                                                const Tile2D *__restrict__ tiles, const Queue &q,
+                                                const Close2D &c, int K, int P, int nwg,
+                                                const ItemIO2D *__restrict__ io, uint64_t *edges,
+                                                TileInfo &info)
 {
-    extern __shared__ uint64_t edges[];
-    __shared__ TileInfo info;
     const int tid = threadIdx.x;
-    const int P = (pbase ? *pbase : 0) + p;
-    const int nwg = (int)gridDim.x - 1;
     if ((int)blockIdx.x == nwg) {
         if (P >= 2)
             for (int i = tid; i < c.nitems; i += blockDim.x) close_item(c, i, K, P - 2);
@@ -303,11 +297,89 @@ __global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restric
     }
 }
 
-// after the last pass Pend - 1: close the items of passes Pend - 2 and Pend - 1
-__global__ void batch_close2d(Close2D c, int K, int pend)
+// One pass P = (*pbase if pbase else 0) + p: gridDim.x - 1 band workgroups
+// and the bookkeeping one.
+template <int R, int V, int IO>
+__global__ __launch_bounds__(512) void thin2d_queue_pass(const Item2D *__restrict__ items,
+                                                          const Tile2D *__restrict__ tiles,
+                                                          Queue q, Close2D c, int K,
+                                                          const int *pbase, int p,
+                                                          const ItemIO2D *__restrict__ io)
 {
+    extern __shared__ uint64_t edges[];
+    __shared__ TileInfo info;
+    queue_pass_body<R, V, IO>(items, tiles, q, c, K, (pbase ? *pbase : 0) + p, (int)gridDim.x - 1,
+                              io, edges, info);
+}
+
+// after the last pass Pend - 1: close the items of passes Pend - 2 and Pend - 1
+// (Pend from *dpend when given: the persistent chain counts its passes)
+__global__ void batch_close2d(Close2D c, int K, int pend, const int *dpend)
+{
+    if (dpend) pend = *dpend;
     for (int P = max(0, pend - 2); P < pend; ++P)
         for (int i = threadIdx.x; i < c.nitems; i += blockDim.x) close_item(c, i, K, P);
+}
+
+// Device-wide barrier of the persistent 2D chain: each workgroup arrives once
+// per pass on a monotonic counter and waits until it reaches `target`, with
+// agent-scope release and acquire fences around it, so that the bands written
+// in a pass are visible to every workgroup (and XCD) in the next.  A wait
+// longer than kBarrierTicks of the constant-rate clock (or another
+// workgroup's timeout) sets *err and returns false: every workgroup then
+// leaves, and the run reports the error.
+constexpr uint64_t kBarrierTicks = 100000000ull;  // about 1 s at the 100 MHz clock
+
+__device__ __forceinline__ bool chain_barrier(unsigned *bar, unsigned target, int *err, int *s_ok)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(bar, 1u);
+        const uint64_t t0 = wall_clock64();
+        int ok = 1;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                wall_clock64() - t0 > kBarrierTicks) {
+                atomicExch(err, 1);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();
+        *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok != 0;
+}
+
+// The whole 2D chain in one launch: every pass of the band queue, each
+// followed by the device-wide barrier, until a pass leaves the next queue
+// empty (or after max_passes).  ctl[0] = barrier counter, ctl[1] = passes
+// run, ctl[2] = barrier timeout.  The grid must be co-resident (one
+// workgroup per slot the pass kernel's occupancy allows), plus the
+// bookkeeping workgroup.
+template <int R, int V>
+__global__ __launch_bounds__(512) void thin2d_chain(const Item2D *__restrict__ items,
+                                                     const Tile2D *__restrict__ tiles, Queue q,
+                                                     Close2D c, int K, int max_passes, int *ctl)
+{
+    extern __shared__ uint64_t edges[];
+    __shared__ TileInfo info;
+    __shared__ int s_ok;
+    const int nwg = (int)gridDim.x - 1;
+    int P = 0;
+    for (; P < max_passes; ++P) {
+        queue_pass_body<R, V, 0>(items, tiles, q, c, K, P, nwg, nullptr, edges, info);
+        if (!chain_barrier((unsigned *)&ctl[0], gridDim.x * (unsigned)(P + 1), &ctl[2], &s_ok)) return;
+        // the queue of pass P + 1 is complete and stays so until pass P + 2
+        if (__hip_atomic_load(&q.cnt[(P + 1) % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            ++P;
+            break;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl[1] = P;
 }
 
 // graph replays: the next chunk's passes start `n` later
@@ -413,7 +485,7 @@ struct Batch {
     int *d_stamp = nullptr;
     long ntiles2 = 0;
 
-    int *h_live = nullptr;  // pinned, two slots of 32: [8..] 3D control words
+    int *h_live = nullptr;  // pinned, two slots of 32: [8..] 3D control words; [64..66] 2D chain words
     unsigned long long *h_rep = nullptr;  // pinned, device-mapped pass reports (Close2D)
     char *h_stage = nullptr;   // pinned staging of the tables uploaded by prepare
     size_t h_stage_bytes = 0;
@@ -712,7 +784,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     if (n2)
         b.grid2 = queue_pass_grid2d(b.c2, b.V2, (int)std::min<long>(b.ntiles2 * kParts, 1L << 30),
                                     blocks2(b));
-    if (!b.h_live && hipHostMalloc((void **)&b.h_live, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    if (!b.h_live && hipHostMalloc((void **)&b.h_live, 96 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(BGT_ERR_NOMEM, "pinned allocation failed");
     if (!b.h_rep) {
         if (hipHostMalloc((void **)&b.h_rep, 16 * sizeof(unsigned long long),
@@ -772,6 +844,34 @@ static const void *queue_pass_fn(int rows, int V, bool first = false)
     if (first)
         return rows == 8 ? (const void *)thin2d_queue_pass<8, 1, 2> : (const void *)thin2d_queue_pass<16, 1, 2>;
     return rows == 8 ? (const void *)thin2d_queue_pass<8, 1, 0> : (const void *)thin2d_queue_pass<16, 1, 0>;
+}
+
+static const void *chain_fn(int rows, int V)
+{
+    if (V == 4) return rows == 8 ? (const void *)thin2d_chain<8, 4> : (const void *)thin2d_chain<16, 4>;
+    if (V == 2) return rows == 8 ? (const void *)thin2d_chain<8, 2> : (const void *)thin2d_chain<16, 2>;
+    return rows == 8 ? (const void *)thin2d_chain<8, 1> : (const void *)thin2d_chain<16, 1>;
+}
+
+// the persistent 2D chain (Config2D::persistent): one launch of thin2d_chain
+// on a co-resident grid (its band workgroups and the bookkeeping one)
+static int launch_chain2d(Batch &b, hipStream_t s, long max_passes)
+{
+    const int grid = std::max(1, occupancy_grid(chain_fn(b.c2.rows, b.V2), b.c2.waves * 64,
+                                                 pass_lds_bytes2d(b.c2, b.V2),
+                                                 (int)std::min<long>(b.ntiles2 * kParts, 1L << 30),
+                                                 blocks2(b)) - 1);
+    int K = b.c2.iters_per_pass;
+    int mp = (int)std::min<long>(max_passes, 0x7fffffffL);
+    int *ctl = b.d_ctl + 8;
+    void *args[] = {(void *)&b.d_items2, (void *)&b.d_tiles2, (void *)&b.q2, (void *)&b.close2,
+                    (void *)&K, (void *)&mp, (void *)&ctl};
+    size_t mk = 0;
+    BGT_CHECK(timed_mark(b, s, 0, true, &mk));
+    BGT_HIP(hipLaunchKernel(chain_fn(b.c2.rows, b.V2), dim3(grid + 1), dim3(b.c2.waves * 64), args,
+                            pass_lds_bytes2d(b.c2, b.V2), s));
+    BGT_CHECK(timed_mark(b, s, 0, false, &mk));
+    return BGT_OK;
 }
 
 int launch_queue_pass2d(const Config2D &c2, int V, int grid, const Item2D *items,
@@ -1102,9 +1202,15 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     // so the 2D unpack overlaps the end of the 3D chain
     auto run2 = [&]() -> int {
         if (!n2) return BGT_OK;
-        BGT_CHECK(drive2());
+        const int *dpend = nullptr;
+        if (b.c2.persistent) {
+            BGT_CHECK(launch_chain2d(b, s, max_vox + 3));
+            dpend = b.d_ctl + 9;
+        } else {
+            BGT_CHECK(drive2());
+        }
         hipLaunchKernelGGL(batch_close2d, dim3(1), dim3(256), 0, s, b.close2,
-                           b.c2.iters_per_pass, b.pass2);
+                           b.c2.iters_per_pass, b.pass2, dpend);
         if (b.nublk2)
             hipLaunchKernelGGL(batch_unpack2d, dim3(b.nublk2), dim3(256), 0, s, b.d_items2,
                                b.d_io2, b.d_ublk2);
@@ -1156,7 +1262,11 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     if (n3)
         BGT_HIP(hipMemcpyAsync(b.h_states + (size_t)n2 * ST_INTS, b.on_tiles3 ? b.t3.states : b.e3.states,
                                (size_t)n3 * ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (n2 && b.c2.persistent)
+        BGT_HIP(hipMemcpyAsync(b.h_live + 64, b.d_ctl + 8, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
+    if (n2 && b.c2.persistent && b.h_live[66])
+        return fail(BGT_ERR_HIP, "persistent 2D chain: a device-wide barrier waited past its limit");
     b.ran = true;
     if (b.timing) BGT_CHECK(collect_timing(b));
     return BGT_OK;
@@ -1245,7 +1355,7 @@ int batch_autotune(Batch &b, hipStream_t s, int reps, Config2D *best2, Config3D 
     auto trial = [&](const Config2D &t2, const Config3D &t3) {
         if (n && t2.waves == c2.waves && t2.rows == c2.rows && t2.iters_per_pass == c2.iters_per_pass &&
             t2.blocks_per_cu == c2.blocks_per_cu && t2.lane_words == c2.lane_words &&
-            t2.fused_io == c2.fused_io &&
+            t2.fused_io == c2.fused_io && t2.persistent == c2.persistent &&
             t3.blocks_per_cu == c3.blocks_per_cu)
             return;  // the current shape, already timed
         if (batch_set_config(b, t2, t3) != BGT_OK) return;

@@ -113,6 +113,7 @@ static Config2D &config2d_storage()
         if (const char *e = std::getenv("BGT_2D_BLOCKS")) v.blocks_per_cu = std::atoi(e);
         if (const char *e = std::getenv("BGT_2D_LANE_WORDS")) v.lane_words = std::atoi(e);
         if (const char *e = std::getenv("BGT_2D_FUSED_IO")) v.fused_io = std::atoi(e);
+        if (const char *e = std::getenv("BGT_2D_PERSISTENT")) v.persistent = std::atoi(e);
         if (check_config2d(v) != BGT_OK) {
             std::fprintf(stderr, "libbgt: ignoring BGT_2D_* environment: %s\n", g_error.c_str());
             v = Config2D{};
@@ -178,6 +179,16 @@ int bgt_set_lane_words2d(int words)
 {
     Config2D c = default_config2d();
     c.lane_words = words;
+    BGT_CHECK(check_config2d(c));
+    std::lock_guard<std::mutex> lk(g_config_mu);
+    config2d_storage() = c;
+    return BGT_OK;
+}
+
+int bgt_set_persistent2d(int on)
+{
+    Config2D c = default_config2d();
+    c.persistent = on;
     BGT_CHECK(check_config2d(c));
     std::lock_guard<std::mutex> lk(g_config_mu);
     config2d_storage() = c;

@@ -27,6 +27,9 @@ struct Config2D {
     // caller's bytes itself, 0 = a separate pack launch (faster: the pack
     // runs at full occupancy, pass 0's band loads at the pass's)
     int fused_io = 0;
+    // batches: 1 = the whole 2D chain in one persistent launch (passes behind
+    // a device-wide barrier), 0 = a launch per pass, driven by the host
+    int persistent = 0;
 };
 
 // Widest 2D image: a row in one wave64 of lanes of four 64-bit words.

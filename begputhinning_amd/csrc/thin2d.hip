@@ -237,7 +237,7 @@ int check_config2d(const Config2D &cfg)
         cfg.iters_per_pass < 1 || cfg.iters_per_pass > 16 || cfg.blocks_per_cu < -1 ||
         cfg.blocks_per_cu > 32 ||
         (cfg.lane_words != 0 && cfg.lane_words != 1 && cfg.lane_words != 2 && cfg.lane_words != 4) ||
-        cfg.fused_io < 0 || cfg.fused_io > 1)
+        cfg.fused_io < 0 || cfg.fused_io > 1 || cfg.persistent < 0 || cfg.persistent > 1)
         return fail(BGT_ERR_INVALID,
                     "bad 2D config (waves=%d rows=%d k=%d blocks/CU=%d lane words=%d fused io=%d)",
                     cfg.waves, cfg.rows, cfg.iters_per_pass, cfg.blocks_per_cu, cfg.lane_words,

@@ -208,6 +208,13 @@ def set_fused_io2d(on=False):
     check(lib().bgt_set_fused_io2d(int(bool(on))))
 
 
+def set_persistent2d(on=False):
+    """Batches of 2D images: the whole band-pass chain in one persistent
+    launch (True) or a launch per pass driven by the host (False, the
+    default).  Results do not depend on it."""
+    check(lib().bgt_set_persistent2d(int(bool(on))))
+
+
 def extract_segments_device(skel, padded=None, stream=None):
     """ExtractSegmentsV over a contiguous uint8 torch tensor on the GPU (h x w
     or d x h x w), the top-left corner of a zero-padded frame of shape

@@ -74,6 +74,15 @@ int bgt_set_lane_words2d(int words);
  * Batches take it when created or when bgt_batch_use_current_config is
  * called. */
 int bgt_set_fused_io2d(int on);
+/* Batches of 2D images (no reference counterpart; results do not depend on
+ * it): 1 = the whole band-pass chain in one persistent launch, its passes
+ * separated by a device-wide barrier (bounded: a wait past about a second
+ * fails the run); 0 (default) = a launch per pass, followed by the host
+ * through per-pass reports in host-mapped memory.  Replaces the reference's
+ * launch + blocking copy + host sum per pass (2D/LibThinning2D.cu:102-112).
+ * Batches take it when created or when bgt_batch_use_current_config is
+ * called. */
+int bgt_set_persistent2d(int on);
 
 /* ------------------------------------------------------------------------- */
 /* Drop-in thinning on host buffers.                                          */
