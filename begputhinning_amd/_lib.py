@@ -7,7 +7,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libbgt.so")
+# BGT_LIBRARY names another build of the same library (a sanitizer build, say)
+LIB_PATH = os.environ.get("BGT_LIBRARY") or os.path.join(HERE, "libbgt.so")
 
 BGT_OK = 0
 BGT_ERR_INVALID = 1
@@ -74,6 +75,7 @@ SIGNATURES = {
     "bgt_set_lane_words2d": (_i, [_i]),
     "bgt_set_fused_io2d": (_i, [_i]),
     "bgt_set_persistent2d": (_i, [_i]),
+    "bgt_debug_fail_device_allocs": (_i, [_i]),
     "bgt_batch_create": (_i, [ctypes.POINTER(ctypes.c_void_p)]),
     "bgt_batch_use_current_config": (_i, [_vp]),
     "bgt_batch_add2d": (_i, [_vp, _vp, _sz, _i, _i, _vp, _sz]),

@@ -31,42 +31,48 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 
 
-def _compile(src):
+def _compile(src, obj_dir=OBJ, extra=()):
     path = os.path.join(CSRC, src)
-    obj = os.path.join(OBJ, src + ".o")
+    obj = os.path.join(obj_dir, src + ".o")
     if src.endswith(".hip"):
-        cmd = [HIPCC, "-c", path, "-o", obj] + COMMON + DEVICE
+        cmd = [HIPCC, "-c", path, "-o", obj] + COMMON + DEVICE + list(extra)
     else:
         # host-only translation units: hipcc as the host compiler, no device code
         cmd = [HIPCC, "-c", "-x", "c++", path, "-o", obj, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-               "-ffp-contract=off"] + COMMON
+               "-ffp-contract=off"] + COMMON + list(extra)
     r = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
                     if "packed-fp32-ops" not in l)
     return src, r.returncode, out, cmd
 
 
-def build_library(verbose=False):
-    os.makedirs(OBJ, exist_ok=True)
+def build_library(verbose=False, out_dir=None, host_flags=()):
+    """Builds libbgt.so in-tree, or into out_dir with host_flags added to
+    every compile and to the link (each flag is passed to the host
+    compilation only: a sanitizer build of the host code, say)."""
+    obj_dir = os.path.join(out_dir, "_build") if out_dir else OBJ
+    lib_path = os.path.join(out_dir, "libbgt.so") if out_dir else LIB
+    extra = [a for f in host_flags for a in ("-Xarch_host", f)]
+    os.makedirs(obj_dir, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")))
     failed = []
     with cf.ThreadPoolExecutor(jobs) as ex:
-        for src, rc, out, cmd in ex.map(_compile, srcs):
+        for src, rc, out, cmd in ex.map(lambda x: _compile(x, obj_dir, extra), srcs):
             if out.strip() and (verbose or rc):
                 print(f"[{src}]\n{out}", file=sys.stderr)
             if rc:
                 failed.append(src)
     if failed:
         raise RuntimeError("hipcc failed for: " + ", ".join(failed))
-    objs = [os.path.join(OBJ, s + ".o") for s in srcs]
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-fPIC"]
+    objs = [os.path.join(obj_dir, s + ".o") for s in srcs]
+    tmp = lib_path + ".tmp"
+    cmd = [HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-fPIC"] + extra
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib_path)
+    return lib_path
 
 
 def build_oracle():

@@ -741,7 +741,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
                 b.owns_mem = false;
                 continue;
             }
-            hipError_t e = hipMalloc(&b.mem, b.bytes);
+            hipError_t e = device_malloc(&b.mem, b.bytes);
             if (e != hipSuccess) {
                 (void)hipGetLastError();
                 b.mem = nullptr;

@@ -2,6 +2,7 @@
 // reporting and the per-thread device workspace.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -59,7 +60,7 @@ int workspace_reserve(Workspace &ws, size_t bytes)
     }
     size_t want = bytes + bytes / 4 + 4096;
     void *p = nullptr;
-    hipError_t e = hipMalloc(&p, want);
+    hipError_t e = device_malloc(&p, want);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return fail(BGT_ERR_NOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
@@ -68,6 +69,19 @@ int workspace_reserve(Workspace &ws, size_t bytes)
     ws.bytes = want;
     ws.device = dev;
     return BGT_OK;
+}
+
+// device allocations left to fail (bgt_debug_fail_device_allocs)
+static std::atomic<int> g_fail_allocs{0};
+
+hipError_t device_malloc(void **p, size_t bytes)
+{
+    for (int n = g_fail_allocs.load(); n > 0; n = g_fail_allocs.load())
+        if (g_fail_allocs.compare_exchange_weak(n, n - 1)) {
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+    return hipMalloc(p, bytes);
 }
 
 bool on_main_thread() { return (long)syscall(SYS_gettid) == (long)getpid(); }
@@ -182,6 +196,13 @@ int bgt_set_lane_words2d(int words)
     BGT_CHECK(check_config2d(c));
     std::lock_guard<std::mutex> lk(g_config_mu);
     config2d_storage() = c;
+    return BGT_OK;
+}
+
+int bgt_debug_fail_device_allocs(int n)
+{
+    if (n < 0) return fail(BGT_ERR_INVALID, "negative allocation count %d", n);
+    g_fail_allocs.store(n);
     return BGT_OK;
 }
 

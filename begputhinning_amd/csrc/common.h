@@ -31,6 +31,10 @@ int fail(int code, const char *fmt, ...);
     } while (0)
 
 // ---- device memory -------------------------------------------------------
+// Every device allocation of the library goes through here, so that tests can
+// make the next ones fail (bgt_debug_fail_device_allocs).
+hipError_t device_malloc(void **p, size_t bytes);
+
 // Grow-only per-thread workspace used by the host-pointer drop-in entry points
 // so that repeated calls do not hipMalloc/hipFree every time.
 // A thread's workspaces and pinned poll buffers are released when the thread

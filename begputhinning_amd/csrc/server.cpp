@@ -164,7 +164,7 @@ void Server::worker(int cls)
             arena = nullptr;
             arena_bytes = 0;
             const size_t grow = need + need / 4;
-            if (hipMalloc(&arena, grow) == hipSuccess) {
+            if (device_malloc(&arena, grow) == hipSuccess) {
                 arena_bytes = grow;
             } else {
                 (void)hipGetLastError();

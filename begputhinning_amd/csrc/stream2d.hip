@@ -322,7 +322,7 @@ int stream2d_create(const Stream2DConfig &cfg, const Config2D &c2, Stream2DDone 
                  o_hist = place(sizeof(unsigned) * 4 * (size_t)ns), o_chg = place(2),
                  o_bufs = place(sizeof(uint64_t) * 2 * e->slot_words * (size_t)ns);
     e->bytes = off;
-    if (hipMalloc(&e->mem, e->bytes) != hipSuccess) {
+    if (device_malloc(&e->mem, e->bytes) != hipSuccess) {
         (void)hipGetLastError();
         e->mem = nullptr;
         return fail(BGT_ERR_NOMEM, "serving queue of %zu bytes failed", e->bytes);

@@ -215,6 +215,12 @@ def set_persistent2d(on=False):
     check(lib().bgt_set_persistent2d(int(bool(on))))
 
 
+def debug_fail_device_allocs(n):
+    """Test hook: the library's next n device allocations fail (its calls
+    then raise BGTError with BGT_ERR_NOMEM); 0 clears it."""
+    check(lib().bgt_debug_fail_device_allocs(int(n)))
+
+
 def extract_segments_device(skel, padded=None, stream=None):
     """ExtractSegmentsV over a contiguous uint8 torch tensor on the GPU (h x w
     or d x h x w), the top-left corner of a zero-padded frame of shape

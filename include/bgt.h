@@ -83,6 +83,11 @@ int bgt_set_fused_io2d(int on);
  * Batches take it when created or when bgt_batch_use_current_config is
  * called. */
 int bgt_set_persistent2d(int on);
+/* Test hook (no reference counterpart; the reference aborts on CUDA errors,
+ * 3D/LibThinning3D.cu:21-29): the next n device allocations the library makes
+ * fail as if the device were out of memory, and the calls that needed them
+ * return BGT_ERR_NOMEM.  0 clears it. */
+int bgt_debug_fail_device_allocs(int n);
 
 /* ------------------------------------------------------------------------- */
 /* Drop-in thinning on host buffers.                                          */
