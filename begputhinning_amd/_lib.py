@@ -93,6 +93,9 @@ SIGNATURES = {
     "bgt_server_submit2d": (_i, [_vp, _vp, _sz, _i, _i, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)]),
     "bgt_server_submit3d": (_i, [_vp, _vp, _i, _i, _i, _vp, ctypes.POINTER(ctypes.c_uint64)]),
     "bgt_server_wait": (_i, [_vp, ctypes.c_uint64, _ip, ctypes.POINTER(ctypes.c_double)]),
+    "bgt_server_enable_segments": (_i, [_vp, _i]),
+    "bgt_server_wait_segments": (_i, [_vp, ctypes.c_uint64, _ip, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(Segments)]),
     "bgt_server_poll": (_i, [_vp, ctypes.c_uint64, _ip]),
     "bgt_server_stats": (_i, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                               ctypes.POINTER(ctypes.c_size_t)]),

@@ -479,6 +479,19 @@ int bgt_server_wait(bgt_server sv, uint64_t ticket, int *iterations, double *lat
     return server_wait(sv->s, ticket, iterations, latency_ms);
 }
 
+int bgt_server_enable_segments(bgt_server sv, int threads)
+{
+    if (!sv) return fail(BGT_ERR_INVALID, "null server");
+    return server_enable_segments(sv->s, threads);
+}
+
+int bgt_server_wait_segments(bgt_server sv, uint64_t ticket, int *iterations, double *latency_ms,
+                             double *segments_ms, bgt_segments *out)
+{
+    if (!sv) return fail(BGT_ERR_INVALID, "null server");
+    return server_wait_segments(sv->s, ticket, iterations, latency_ms, segments_ms, out);
+}
+
 int bgt_server_poll(bgt_server sv, uint64_t ticket, int *done)
 {
     if (!sv || !done) return fail(BGT_ERR_INVALID, "null pointer");

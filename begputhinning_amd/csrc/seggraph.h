@@ -8,16 +8,19 @@
 
 namespace bgt {
 
-// Skeleton points of a padded frame in raster order: raster index, the
-// tracer's code (-1 isolated, 2 path, -n end or junction with n neighbours),
-// and for each of the nd neighbours in the reference's order the rank of that
-// neighbour if it is a point itself, else -1.
+// Skeleton points of a padded frame in raster order: coordinates packed as
+// x | y << 16 | z << 32 (sides are at most 32767), the tracer's code (-1
+// isolated, 2 path, -n end or junction with n neighbours), and for each of
+// the nd neighbours in the reference's order the rank of that neighbour if it
+// is a point itself, else -1.
+// The arrays are views of a per-thread host buffer, valid until the thread's
+// next segment_graph_device call.
 struct SegGraph {
     int npts = 0;
     int nd = 8;
-    std::vector<int64_t> idx;
-    std::vector<int8_t> code;
-    std::vector<int> nbr;  // npts x nd
+    const int64_t *idx = nullptr;
+    const int8_t *code = nullptr;
+    const int *nbr = nullptr;  // npts x nd
 };
 
 // d_skel: w x h x d uint8 skeleton in device memory (nonzero = set), the top

@@ -20,7 +20,7 @@
 //   seg_count  -- the chunk's point bits, their count, and each workgroup's sum;
 //   seg_scan   -- one workgroup: exclusive sums of the workgroup sums, total;
 //   seg_base   -- each chunk's first rank;
-//   seg_emit   -- per point: raster index, code, neighbour ranks.
+//   seg_emit   -- per point: packed coordinates, code, neighbour ranks.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void seg_emit(Frame f, long nchunks, const uin
         const int k = __ffsll((long long)m) - 1;
         m &= m - 1;
         const int x = x0 + k;
-        idx[r] = ((int64_t)z * f.ph + y) * f.pw + x;
+        idx[r] = (int64_t)x | (int64_t)y << 16 | (int64_t)z << 32;  // packed coordinates
         int nn = 0, j = 0;
         // the reference's neighbour order: raster order of the offsets
         for (int dz = f.three_d ? -1 : 0; dz <= (f.three_d ? 1 : 0); ++dz)
@@ -181,6 +181,39 @@ __global__ __launch_bounds__(256) void seg_emit(Frame f, long nchunks, const uin
     }
 }
 
+// Grow-only pinned host staging of one thread's pre-pass results (device to
+// host copies into pageable memory go through the runtime's own staging),
+// released at thread exit.
+struct PinnedStage {
+    char *ptr = nullptr;
+    size_t bytes = 0;
+    ~PinnedStage()
+    {
+        if (ptr && !on_main_thread()) (void)hipHostFree(ptr);
+    }
+    int reserve(size_t n)
+    {
+        if (ptr && bytes >= n) return BGT_OK;
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        const size_t want = n + n / 4 + 4096;
+        if (hipHostMalloc((void **)&ptr, want, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            ptr = nullptr;
+            return fail(BGT_ERR_NOMEM, "pinned staging of %zu bytes failed", want);
+        }
+        bytes = want;
+        return BGT_OK;
+    }
+};
+
+PinnedStage &thread_stage()
+{
+    thread_local PinnedStage st;
+    return st;
+}
+
 }  // namespace
 
 int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int ph, int pd,
@@ -190,6 +223,8 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     g.nd = three_d ? 26 : 8;
     if (w <= 0 || h <= 0 || d <= 0) return BGT_OK;
     if (pw < w || ph < h || pd < d) return fail(BGT_ERR_INVALID, "padded frame smaller than skeleton");
+    if (pw > 32767 || ph > 32767 || pd > 32767)
+        return fail(BGT_ERR_UNSUPPORTED, "coordinates are int16 (sides <= 32767)");
     Frame f{d_skel, w, h, d, pw, ph, pd, (w + 63) / 64, three_d ? 1 : 0};
     const long nchunks = (long)d * h * f.cpr;
     const long nwg = (nchunks + 255) / 256;
@@ -213,9 +248,12 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     hipLaunchKernelGGL(seg_scan, dim3(1), dim3(1024), 0, s, wsum, (int)nwg, d_total);
     hipLaunchKernelGGL(seg_base, dim3((unsigned)nwg), dim3(256), 0, s, nchunks, cnt, wsum, base);
     BGT_HIP(hipGetLastError());
-    long total = 0;
-    BGT_HIP(hipMemcpyAsync(&total, d_total, sizeof(long), hipMemcpyDeviceToHost, s));
+    PinnedStage &st = thread_stage();
+    BGT_CHECK(st.reserve(4096));
+    BGT_HIP(hipMemcpyAsync(st.ptr, d_total, sizeof(long), hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
+    long total = 0;
+    std::memcpy(&total, st.ptr, sizeof(long));
     if (total > 0x7fffffffL / g.nd) return fail(BGT_ERR_UNSUPPORTED, "too many skeleton points");
     g.npts = (int)total;
     if (!g.npts) return BGT_OK;
@@ -232,13 +270,19 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     hipLaunchKernelGGL(seg_emit, dim3((unsigned)nwg), dim3(256), 0, s, f, nchunks, bits, base,
                        d_idx, d_code, d_nbr);
     BGT_HIP(hipGetLastError());
-    g.idx.resize(total);
-    g.code.resize(total);
-    g.nbr.resize((size_t)total * g.nd);
-    BGT_HIP(hipMemcpyAsync(g.idx.data(), d_idx, sizeof(int64_t) * total, hipMemcpyDeviceToHost, s));
-    BGT_HIP(hipMemcpyAsync(g.code.data(), d_code, (size_t)total, hipMemcpyDeviceToHost, s));
-    BGT_HIP(hipMemcpyAsync(g.nbr.data(), d_nbr, sizeof(int) * total * g.nd, hipMemcpyDeviceToHost, s));
+    // the three arrays lie one after another in the workspace: one copy
+    BGT_CHECK(st.reserve(out));
+    BGT_HIP(hipMemcpyAsync(st.ptr, wo.ptr, out, hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
+    // the host trace reads the arrays at random: from ordinary pageable
+    // memory (this thread's copy, kept between calls) it runs faster
+    thread_local std::vector<char> hv;
+    if (hv.size() < out) hv.resize(out);
+    std::memcpy(hv.data(), st.ptr, out);
+    const char *hq = hv.data();
+    g.idx = (const int64_t *)hq;
+    g.code = (const int8_t *)(hq + ((const char *)d_code - (const char *)d_idx));
+    g.nbr = (const int *)(hq + ((const char *)d_nbr - (const char *)d_idx));
     return BGT_OK;
 }
 

@@ -126,16 +126,22 @@ void trace(const uint8_t *mask, int w, int h, int d, bool three_d, Flat &out)
 // neighbour rank of -1 (not a point) in place of a zero code.
 void trace_graph(const SegGraph &g, int pw, int ph, bool three_d, Flat &out)
 {
+    (void)pw;
+    (void)ph;
     const int nd = g.nd;
-    const long sl = (long)pw * ph;
-    std::vector<int8_t> code(g.code), orig(g.code);
+    // this thread's working copy of the codes, kept between calls
+    thread_local std::vector<int8_t> code;
+    code.assign(g.code, g.code + g.npts);
+    const int8_t *orig = g.code;
+    // a segment list holds each point about once: reserve for that
+    out.x.reserve((size_t)g.npts + 16);
+    out.y.reserve((size_t)g.npts + 16);
+    if (three_d) out.z.reserve((size_t)g.npts + 16);
     auto emit = [&](int r) {
-        const long q = (long)g.idx[r];
-        const int zz = (int)(q / sl);
-        const long rr = q - (long)zz * sl;
-        out.x.push_back((int16_t)(rr % pw));
-        out.y.push_back((int16_t)(rr / pw));
-        if (three_d) out.z.push_back((int16_t)zz);
+        const uint64_t q = (uint64_t)g.idx[r];  // x | y << 16 | z << 32
+        out.x.push_back((int16_t)(q & 0xffffu));
+        out.y.push_back((int16_t)((q >> 16) & 0xffffu));
+        if (three_d) out.z.push_back((int16_t)(q >> 32));
     };
     auto close = [&](bool end) {
         out.is_end.push_back(end ? 1 : 0);

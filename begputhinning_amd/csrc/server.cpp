@@ -15,9 +15,14 @@
 // No counterpart in the reference (it thins one image per blocking call,
 // 2D/LibThinning2D.cu:39-130).
 #include <hip/hip_runtime.h>
+#include <sys/resource.h>
+#include <sys/syscall.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
+#include <cstring>
 #include <condition_variable>
 #include <deque>
 #include <memory>
@@ -29,6 +34,8 @@
 
 #include "batch.h"
 #include "common.h"
+#include "seggraph.h"
+#include "segments.h"
 #include "server.h"
 #include "stream2d.h"
 #include "thin2d.h"
@@ -58,7 +65,20 @@ struct Request {
     int status = BGT_OK;
     std::string error;  // message of a failed batch (set on the worker thread)
     bool done = false;
+    // segment tracing of served skeletons (server_enable_segments)
+    bool traced = false;      // segments ready (or tracing failed or not wanted)
+    int trace_status = BGT_OK;
+    std::string trace_error;
+    bgt_segments segs{};
+    double t_segs = 0;
 };
+
+void free_segments(bgt_segments &g)
+{
+    std::free(g.x); std::free(g.y); std::free(g.z);
+    std::free(g.seg_start); std::free(g.is_end);
+    std::memset(&g, 0, sizeof(g));
+}
 
 }  // namespace
 
@@ -81,10 +101,73 @@ struct Server {
     // statistics
     uint64_t batches = 0, completed = 0;
     size_t peak_in_use = 0;
+    // tracer threads: each finished request's skeleton to its segment list
+    // (the device pre-pass on the thread's stream, then the host trace)
+    int tracing = 0;
+    bool tstop = false;  // set once no request can finish any more
+    std::deque<Request *> tq;
+    std::condition_variable cv_trace;
+    std::vector<std::thread> tracers;
 
     void worker(int cls);
+    void tracer();
     void finish(const std::vector<Request *> &take, const std::vector<int> &iters, int rc);
+    void completed_locked(Request *r);
 };
+
+// With tracing on, a request whose skeleton is done (successfully) goes to
+// the tracers; otherwise it has nothing more to wait for.  Holds m.
+void Server::completed_locked(Request *r)
+{
+    if (tracing && r->status == BGT_OK) {
+        tq.push_back(r);
+        cv_trace.notify_one();
+    } else {
+        r->traced = true;
+    }
+}
+
+void Server::tracer()
+{
+    // below the engine's host threads, which must react to every pass: a
+    // trace waits rather than a pass
+    (void)setpriority(PRIO_PROCESS, (id_t)syscall(SYS_gettid), 10);
+    (void)hipSetDevice(device);
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    for (;;) {
+        Request *r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv_trace.wait(lk, [&] { return tstop || !tq.empty(); });
+            if (tq.empty()) break;  // stopping, nothing left
+            r = tq.front();
+            tq.pop_front();
+        }
+        bgt_segments g{};
+        int rc;
+        if (!s) {
+            rc = fail(BGT_ERR_HIP, "tracer stream creation failed");
+        } else if (r->out_pitch != (size_t)r->w && r->cls == 0) {
+            rc = fail(BGT_ERR_UNSUPPORTED, "tracing a served image needs contiguous output rows");
+        } else {
+            SegGraph gr;
+            rc = segment_graph_device(r->out, r->w, r->h, r->d, r->w, r->h, r->d, r->cls == 1, s, gr);
+            if (rc == BGT_OK) rc = extract_segments_graph(gr, r->w, r->h, r->cls == 1, &g);
+        }
+        const std::string err = rc == BGT_OK ? std::string() : last_error_message();
+        {
+            std::lock_guard<std::mutex> lk(m);
+            r->trace_status = rc;
+            r->trace_error = err;
+            r->segs = g;
+            r->t_segs = now_ms();
+            r->traced = true;
+        }
+        cv_done.notify_all();
+    }
+    if (s) (void)hipStreamDestroy(s);
+}
 
 // Marks a batch's requests done (with the status and, on failure, the
 // worker thread's error message) and releases their memory.
@@ -102,6 +185,7 @@ void Server::finish(const std::vector<Request *> &take, const std::vector<int> &
             r->t_done = t;
             r->done = true;
             in_use -= r->bytes;
+            completed_locked(r);
         }
         completed += take.size();
         ++batches;
@@ -220,6 +304,7 @@ int server_create(const ServerConfig &cfg, const Config2D &c2, const Config3D &c
                                               r->t_done = now_ms();
                                               r->done = true;
                                               ++raw->completed;
+                                              raw->completed_locked(r);
                                           }
                                           raw->cv_done.notify_all();
                                       },
@@ -246,6 +331,7 @@ void server_destroy(Server *sv)
                 r->status = BGT_ERR_INVALID;
                 r->error = "server destroyed before the request ran";
                 r->done = true;
+                r->traced = true;
             }
         sv->queue[0].clear();
         sv->queue[1].clear();
@@ -254,7 +340,53 @@ void server_destroy(Server *sv)
     sv->cv_done.notify_all();
     stream2d_destroy(sv->q2d);  // its running frames finish, waiting ones fail
     for (auto &t : sv->threads) t.join();
+    {
+        std::lock_guard<std::mutex> lk(sv->m);
+        sv->tstop = true;
+    }
+    sv->cv_trace.notify_all();  // the tracers finish the queued skeletons, then leave
+    for (auto &t : sv->tracers) t.join();
+    for (auto &kv : sv->reqs) free_segments(kv.second->segs);
     delete sv;
+}
+
+int server_enable_segments(Server *sv, int threads)
+{
+    if (!sv || threads < 1 || threads > 64) return fail(BGT_ERR_INVALID, "bad tracer count");
+    std::lock_guard<std::mutex> lk(sv->m);
+    if (sv->tracing) return fail(BGT_ERR_INVALID, "segment tracing is already on");
+    if (!sv->reqs.empty()) return fail(BGT_ERR_INVALID, "turn segment tracing on before submitting");
+    sv->tracing = threads;
+    for (int k = 0; k < threads; ++k) sv->tracers.emplace_back(&Server::tracer, sv);
+    return BGT_OK;
+}
+
+int server_wait_segments(Server *sv, uint64_t ticket, int *iterations, double *latency_ms,
+                         double *segments_ms, bgt_segments *out)
+{
+    if (out) std::memset(out, 0, sizeof(*out));
+    std::unique_lock<std::mutex> lk(sv->m);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return fail(BGT_ERR_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
+    Request *r = it->second.get();
+    sv->cv_done.wait(lk, [&] { return r->done && r->traced; });
+    int rc = r->status;
+    std::string err = r->error;
+    if (rc == BGT_OK && r->trace_status != BGT_OK) {
+        rc = r->trace_status;
+        err = r->trace_error;
+    }
+    if (iterations) *iterations = r->iterations;
+    if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+    if (segments_ms) *segments_ms = r->t_segs - r->t_submit;
+    if (out && rc == BGT_OK) {
+        *out = r->segs;
+        std::memset(&r->segs, 0, sizeof(r->segs));
+    }
+    free_segments(r->segs);
+    sv->reqs.erase(it);
+    if (rc != BGT_OK) set_error(rc, "request failed: " + err);
+    return rc;
 }
 
 static int item_bytes(const Request &r, const Config2D &c2, const Config3D &c3, size_t *bytes)
@@ -321,11 +453,13 @@ int server_wait(Server *sv, uint64_t ticket, int *iterations, double *latency_ms
     auto it = sv->reqs.find(ticket);
     if (it == sv->reqs.end()) return fail(BGT_ERR_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
     Request *r = it->second.get();
-    sv->cv_done.wait(lk, [&] { return r->done; });
+    // with tracing on, the record lives until its tracer is done with it
+    sv->cv_done.wait(lk, [&] { return r->done && r->traced; });
     const int rc = r->status;
     const std::string err = r->error;
     if (iterations) *iterations = r->iterations;
     if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+    free_segments(r->segs);
     sv->reqs.erase(it);
     if (rc != BGT_OK) set_error(rc, "request failed: " + err);
     return rc;

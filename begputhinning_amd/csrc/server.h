@@ -4,6 +4,7 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "../../include/bgt.h"
 #include "thin2d.h"
 #include "thin3d.h"
 
@@ -28,5 +29,10 @@ int server_submit(Server *sv, int cls, const uint8_t *in, size_t in_pitch, uint8
 int server_wait(Server *sv, uint64_t ticket, int *iterations, double *latency_ms);
 int server_poll(Server *sv, uint64_t ticket, int *done);
 void server_stats(Server *sv, uint64_t *batches, uint64_t *completed, size_t *peak_bytes);
+// `threads` tracer threads trace every served skeleton into its segment list
+// (before any submit)
+int server_enable_segments(Server *sv, int threads);
+int server_wait_segments(Server *sv, uint64_t ticket, int *iterations, double *latency_ms,
+                         double *segments_ms, bgt_segments *out);
 
 }  // namespace bgt

@@ -52,6 +52,32 @@ class ThinningServer:
             self._keep.pop(ticket, None)
         return it.value, ms.value
 
+    def enable_segments(self, threads=4):
+        """Have the server trace every served skeleton into its segment list
+        (`threads` tracer threads; before the first submit).  wait() then
+        also waits for the trace."""
+        check(lib().bgt_server_enable_segments(self._h, int(threads)))
+
+    def wait_segments(self, ticket):
+        """Block until the request's segment list is ready: (iterations,
+        submit-to-skeleton ms, submit-to-segments ms, segments) with the
+        segments as extract_segments_flat gives them."""
+        from ._lib import Segments
+        from .thinning import _take_segments
+        it = ctypes.c_int(0)
+        ms = ctypes.c_double(0.0)
+        sms = ctypes.c_double(0.0)
+        seg = Segments()
+        inp = self._keep.get(ticket, (None,))[0]
+        try:
+            check(lib().bgt_server_wait_segments(self._h, ctypes.c_uint64(ticket), ctypes.byref(it),
+                                                 ctypes.byref(ms), ctypes.byref(sms),
+                                                 ctypes.byref(seg)))
+        finally:
+            self._keep.pop(ticket, None)
+        three_d = inp is not None and inp.dim() == 3
+        return it.value, ms.value, sms.value, _take_segments(seg, three_d)
+
     def done(self, ticket):
         d = ctypes.c_int(0)
         check(lib().bgt_server_poll(self._h, ctypes.c_uint64(ticket), ctypes.byref(d)))

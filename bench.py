@@ -101,12 +101,13 @@ def serve_pool(synth):
 def serve_leg(args, torch, B, synth, dev):
     """Poisson arrivals of 80% frames / 20% volumes drawn from the pool,
     submitted open-loop to the server, at each rate of --serve-qps.  Frames
-    join the server's running 2D queue at a pass boundary.  Skeleton-ready
-    latency is submit-to-done on the server's clock; segments-ready adds the
-    segment list of the served skeleton (the device pre-pass and the host
-    trace, bgt_extract_segments2d/3d_device), taken by a pool of waiter
-    threads, on the host clock from submission.  Every served output is
-    compared with the per-image device result of its input."""
+    join the server's running 2D queue at a pass boundary.  Each rate runs
+    twice: for skeleton-ready latency (submit-to-done on the server's clock),
+    and with the server also tracing every served skeleton into its segment
+    list (its tracer threads: the device pre-pass, then the host trace) for
+    segments-ready latency (submit-to-segments), which takes host cores the
+    engine's threads would otherwise have.  Every served output is compared
+    with the per-image device result of its input."""
     import threading
     from begputhinning_amd.serve import ThinningServer
     pool = serve_pool(synth)
@@ -119,7 +120,10 @@ def serve_leg(args, torch, B, synth, dev):
     vols = [k for k, a in enumerate(pool) if a.ndim == 3]
     n = args.serve_requests
     levels = []
-    for qps in args.serve_qps:
+
+    def run_level(qps, traced):
+        """One open-loop run at `qps`: (skeleton ms, segments ms or None,
+        outputs equal, Mvoxel/s, worker batches, picks)."""
         gaps, mix = synth.arrivals(n, qps, 0xBE6E0005)
         rng = np.random.default_rng(0xBE6E0006)
         pick = [int(frames[rng.integers(len(frames))]) if m < 0.8 else int(vols[rng.integers(len(vols))])
@@ -127,18 +131,18 @@ def serve_leg(args, torch, B, synth, dev):
         outs = [torch.empty_like(ins[k]) for k in pick]
         torch.cuda.synchronize()
         srv = ThinningServer(budget_bytes=32 << 30, workers2d=2, workers3d=2, max_items=64)
+        if traced:
+            srv.enable_segments(args.serve_tracers)
         tickets = [None] * n
         t_sub = [0.0] * n
         lat = np.zeros(n)
         seg_lat = np.zeros(n)
-        t_done = [0.0] * n
         submitted = threading.Semaphore(0)
         nxt = [0]
         nxt_mu = threading.Lock()
         errors = []
 
         def waiter():
-            st_w = torch.cuda.Stream()  # the segment pre-pass of each waiter on its own stream
             try:
                 while True:
                     with nxt_mu:
@@ -149,10 +153,10 @@ def serve_leg(args, torch, B, synth, dev):
                     submitted.acquire()  # one release per submitted request
                     while tickets[i] is None:  # submitted in order; i may be a little ahead
                         time.sleep(1e-5)
-                    _, lat[i] = srv.wait(tickets[i])
-                    t_done[i] = time.perf_counter()
-                    B.extract_segments_device(outs[i], stream=st_w)
-                    seg_lat[i] = (time.perf_counter() - t_sub[i]) * 1e3
+                    if traced:
+                        _, lat[i], seg_lat[i], _ = srv.wait_segments(tickets[i])
+                    else:
+                        _, lat[i] = srv.wait(tickets[i])
             except Exception as exc:  # reported after the run
                 errors.append(exc)
 
@@ -184,6 +188,11 @@ def serve_leg(args, torch, B, synth, dev):
             raise errors[0]
         same = all(torch.equal(o, single[k]) for o, k in zip(outs, pick))
         vox = sum(pool[k].size for k in pick)
+        return lat, (seg_lat if traced else None), same, vox / span / 1e6, st["batches"], pick
+
+    for qps in args.serve_qps:
+        lat, _, same, mvox, nb, pick = run_level(qps, False)
+        tlat, seg_lat, same_t, _, _, _ = run_level(qps, True)
         is2 = np.array([pool[k].ndim == 2 for k in pick])
         big = np.array([pool[k].ndim == 2 and pool[k].shape == (1024, 1024) for k in pick])
 
@@ -196,19 +205,25 @@ def serve_leg(args, torch, B, synth, dev):
               "skeleton_ms": {"2d": pct(lat, is2), "1024x1024": pct(lat, big), "3d": pct(lat, ~is2)},
               "segments_ms": {"2d": pct(seg_lat, is2), "1024x1024": pct(seg_lat, big),
                               "3d": pct(seg_lat, ~is2)},
-              "mvoxel_per_s": round(vox / span / 1e6, 1), "served_equals_single_item": same,
-              "worker_batches": st["batches"]}
+              "skeleton_ms_while_tracing": {"1024x1024": pct(tlat, big), "3d": pct(tlat, ~is2)},
+              "mvoxel_per_s": round(mvox, 1), "served_equals_single_item": same and same_t,
+              "worker_batches": nb}
         log(f"serving {qps:.0f} QPS: skeleton 2D {lv['skeleton_ms']['2d']}, 1024^2 "
-            f"{lv['skeleton_ms']['1024x1024']}, 3D {lv['skeleton_ms']['3d']}; segments 1024^2 "
-            f"{lv['segments_ms']['1024x1024']}, 3D {lv['segments_ms']['3d']}; "
-            f"{lv['mvoxel_per_s']} Mvoxel/s, equal {same}")
+            f"{lv['skeleton_ms']['1024x1024']}, 3D {lv['skeleton_ms']['3d']}; with tracing: segments "
+            f"1024^2 {lv['segments_ms']['1024x1024']}, 3D {lv['segments_ms']['3d']}, skeleton 1024^2 "
+            f"{lv['skeleton_ms_while_tracing']['1024x1024']}; {lv['mvoxel_per_s']} Mvoxel/s, "
+            f"equal {lv['served_equals_single_item']}")
         levels.append(lv)
     ok = [lv["qps"] for lv in levels if lv["skeleton_ms"]["1024x1024"]
           and lv["skeleton_ms"]["1024x1024"]["p99"] < 5.0]
+    ok_seg = [lv["qps"] for lv in levels if lv["segments_ms"]["1024x1024"]
+              and lv["segments_ms"]["1024x1024"]["p99"] < 5.0]
     return {"mix": "80% 2D U{256..1024} (a sixth of the pool 1024x1024), 20% 3D U{64..512}",
-            "workers": "frames: the running 2D queue (64 slots); volumes: 2 workers",
+            "workers": f"frames: the running 2D queue (64 slots); volumes: 2 workers; "
+                       f"{args.serve_tracers} tracer threads",
             "budget_bytes": 32 << 30, "levels": levels,
-            "max_qps_1024x1024_p99_under_5ms": max(ok) if ok else None}
+            "max_qps_1024x1024_p99_under_5ms": max(ok) if ok else None,
+            "max_qps_1024x1024_segments_p99_under_5ms": max(ok_seg) if ok_seg else None}
 
 
 def frame_leg(args, torch, B, synth, dev):
@@ -313,6 +328,10 @@ def main():
                     default=[1000.0, 2000.0, 4000.0, 6000.0, 8000.0, 12000.0],
                     help="Poisson arrival rates of the serving leg (none or 0 = skip it)")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-tracers", type=int, default=8,
+                    help="the server's segment tracer threads")
+    ap.add_argument("--only-serving", action="store_true",
+                    help="run only the serving leg and print its figures")
     ap.add_argument("--only-frames", action="store_true",
                     help="run only the 1024x1024 per-image latency leg (for counter runs of the "
                          "per-image path) and print its figures")
@@ -339,6 +358,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.only_frames:
         print(json.dumps({"latency_1024x1024_ms": frame_leg(args, torch, B, synth, dev)}))
+        return
+    if args.only_serving:
+        print(json.dumps({"serving": serve_leg(args, torch, B, synth, dev)}))
         return
     t0 = time.time()
     items, n2 = gen_items()

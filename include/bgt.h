@@ -270,6 +270,21 @@ int bgt_server_submit3d(bgt_server s, const uint8_t *d_in, int width, int height
  * submit-to-done on the host's steady clock.  Either pointer may be NULL. */
 int bgt_server_wait(bgt_server s, uint64_t ticket, int *iterations, double *latency_ms);
 int bgt_server_poll(bgt_server s, uint64_t ticket, int *done);
+/* Segment lists of served skeletons (no reference counterpart; the
+ * reference traces host arrays after a blocking call, 2D/CenterlineExtraction.cpp:39-42):
+ * `threads` tracer threads of the server take each request as its skeleton
+ * is done and trace it -- the device pre-pass on the thread's own stream,
+ * then the host trace -- the result that of bgt_extract_segments2d/3d_device
+ * on the output (a frame of the output's size; 2D outputs need contiguous
+ * rows).  Call before the first submit.  bgt_server_wait then also waits for
+ * the trace (and drops the list). */
+int bgt_server_enable_segments(bgt_server s, int threads);
+/* Blocks until the request's segment list is ready; hands it over in *out
+ * (release with bgt_segments_free).  latency_ms: submit-to-skeleton,
+ * segments_ms: submit-to-segments, on the host's steady clock.  Any pointer
+ * but `out` may be NULL. */
+int bgt_server_wait_segments(bgt_server s, uint64_t ticket, int *iterations, double *latency_ms,
+                             double *segments_ms, bgt_segments *out);
 int bgt_server_stats(bgt_server s, uint64_t *batches, uint64_t *completed, size_t *peak_bytes);
 /* Requests not yet started fail; batches in flight finish first.  Call after
  * every wait has returned. */
