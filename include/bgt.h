@@ -49,9 +49,10 @@ int bgt_version(void);
  * volumes, whose kernels take the other slots, else the occupancy limit).  3D: workgroups per CU of the subiteration kernel (0-32,
  * 0 = the occupancy limit), the most iterations queued between two host
  * convergence polls (1-64), and the average active words per volume at or
- * below which the remaining iterations run in one launch, one workgroup per
- * volume (0 = never).  Applies to later calls of the calling process and to batches
- * created afterwards. */
+ * below which the word-list engine hands the remaining iterations over: to
+ * the tile engine (engine 1, the default; 16384) or to one launch, one
+ * workgroup per volume (engine 0); 0 = never.  Applies to later calls of the
+ * calling process and to batches created afterwards. */
 int bgt_set_config2d(int waves, int rows, int iters_per_pass, int blocks_per_cu);
 int bgt_set_config3d(int blocks_per_cu, int max_chunk, int tail_words);
 /* 3D engine (no reference counterpart; results do not depend on it): 0 = the
