@@ -80,6 +80,7 @@ __global__ __launch_bounds__(256) void batch_pack2d(const Item2D *__restrict__ i
     else
         v = valid ? pack_run_u8(p, n) : 0ull;
     if (!valid) return;
+    v = lane_order(v);
     it.buf[0][t] = v;
     it.buf[1][t] = v;
     if (v) occ[it.tile0 + y / it.band_rows] = 1;

@@ -132,6 +132,55 @@ __device__ __forceinline__ void unpack_run_u8(uint64_t v, uint8_t *p, int n)
     }
 }
 
+// ---- lane order of the 2D state words -------------------------------------
+//
+// The 2D band kernel keeps pixel k (0..63) of a state word at bit
+// lane_bit(k) = (k & 1) * 32 + (k >> 1): the even pixels in the low 32-bit
+// half, the odd pixels in the high half.  Then the West neighbour of every odd
+// pixel and the East neighbour of every even pixel sit at the same bit of the
+// other half, and only the other neighbour of each half needs a shifted copy.
+// lane_order turns a word in pixel order (bit k = pixel k, as the packers
+// produce it) into lane order; pixel_order turns it back.
+
+__host__ __device__ constexpr int lane_bit(int k) { return (k & 1) * 32 + (k >> 1); }
+
+// even bits of x -> bits 0..15, odd bits -> bits 16..31 (four delta swaps)
+__host__ __device__ inline uint32_t unshuffle32(uint32_t x)
+{
+    uint32_t t;
+    t = (x ^ (x >> 1)) & 0x22222222u; x ^= t ^ (t << 1);
+    t = (x ^ (x >> 2)) & 0x0c0c0c0cu; x ^= t ^ (t << 2);
+    t = (x ^ (x >> 4)) & 0x00f000f0u; x ^= t ^ (t << 4);
+    t = (x ^ (x >> 8)) & 0x0000ff00u; x ^= t ^ (t << 8);
+    return x;
+}
+
+// the inverse: bits 0..15 -> even bits, bits 16..31 -> odd bits
+__host__ __device__ inline uint32_t shuffle32(uint32_t x)
+{
+    uint32_t t;
+    t = (x ^ (x >> 8)) & 0x0000ff00u; x ^= t ^ (t << 8);
+    t = (x ^ (x >> 4)) & 0x00f000f0u; x ^= t ^ (t << 4);
+    t = (x ^ (x >> 2)) & 0x0c0c0c0cu; x ^= t ^ (t << 2);
+    t = (x ^ (x >> 1)) & 0x22222222u; x ^= t ^ (t << 1);
+    return x;
+}
+
+__host__ __device__ inline uint64_t lane_order(uint64_t v)
+{
+    const uint32_t lo = unshuffle32((uint32_t)v), hi = unshuffle32((uint32_t)(v >> 32));
+    const uint32_t even = (lo & 0xffffu) | (hi << 16), odd = (lo >> 16) | (hi & 0xffff0000u);
+    return (uint64_t)even | ((uint64_t)odd << 32);
+}
+
+__host__ __device__ inline uint64_t pixel_order(uint64_t v)
+{
+    const uint32_t even = (uint32_t)v, odd = (uint32_t)(v >> 32);
+    const uint32_t lo = shuffle32((even & 0xffffu) | (odd << 16));
+    const uint32_t hi = shuffle32((even >> 16) | (odd & 0xffff0000u));
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
 }  // namespace bgt
 
 namespace bgt {
@@ -293,7 +342,8 @@ __device__ __forceinline__ void unpack_block64_wave_u8(uint8_t *base, uint32_t n
 // waves of 64 blocks).
 __host__ __device__ inline uint32_t unpack_wave_blocks(uint32_t n) { return (n / 64 + 2 + 63) & ~63u; }
 
-// Pixels of an image's rows in its packed state (64-bit words, L per row).
+// Pixels of an image's rows in its packed state (64-bit words in lane order,
+// L per row).
 struct RowBits {
     const uint64_t *row0;
     uint32_t L;
@@ -301,21 +351,21 @@ struct RowBits {
     {
         const uint64_t *row = row0 + (size_t)y * L;
         const uint32_t i = x >> 6, s = x & 63;
-        uint64_t v = row[i] >> s;
-        if (s > 48) v |= row[i + 1] << (64 - s);  // x + 16 <= width: word i+1 is in the row
+        uint64_t v = pixel_order(row[i]) >> s;
+        if (s > 48) v |= pixel_order(row[i + 1]) << (64 - s);  // x + 16 <= width: word i+1 is in the row
         return (uint32_t)v & 0xffffu;
     }
     __device__ uint32_t bit(uint32_t y, uint32_t x) const
     {
-        return (uint32_t)(row0[(size_t)y * L + (x >> 6)] >> (x & 63)) & 1u;
+        return (uint32_t)(row0[(size_t)y * L + (x >> 6)] >> lane_bit((int)(x & 63))) & 1u;
     }
     // 64 pixels from x of row y; pixels past the row's last word read as 0
     __device__ uint64_t bits64(uint32_t y, uint32_t x) const
     {
         const uint64_t *row = row0 + (size_t)y * L;
         const uint32_t i = x >> 6, s = x & 63;
-        uint64_t v = row[i] >> s;
-        if (s && i + 1 < L) v |= row[i + 1] << (64 - s);
+        uint64_t v = pixel_order(row[i]) >> s;
+        if (s && i + 1 < L) v |= pixel_order(row[i + 1]) << (64 - s);
         return v;
     }
 };

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void stream2d_pack(const Item2D *__restrict__ 
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)it.L * it.h) return;
     const int y = (int)(t / it.L), x0 = (int)(t - (long)y * it.L) * 64;
-    const uint64_t v = pack_run_u8(in + (size_t)y * pitch + x0, min(64, it.w - x0));
+    const uint64_t v = lane_order(pack_run_u8(in + (size_t)y * pitch + x0, min(64, it.w - x0)));
     it.buf[0][t] = v;
     it.buf[1][t] = v;
     if (v) occ[it.tile0 + y / it.band_rows] = 1;

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void pack2d_u8(const uint8_t *__restrict__ in,
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)L * h) return;
     const int y = (int)(idx / L), x0 = (int)(idx - (long)y * L) * 64;  // x0 < w
-    out[idx] = pack_run_u8(in + (size_t)y * pitch + x0, min(64, w - x0));
+    out[idx] = lane_order(pack_run_u8(in + (size_t)y * pitch + x0, min(64, w - x0)));
 }
 
 template <typename T>
@@ -77,7 +77,7 @@ __global__ void pack2d_t(const T *__restrict__ in, int w, int h, int L, uint64_t
     uint64_t v = 0;
     for (int k = 0; k < 64; ++k) {
         int x = c * 64 + k;
-        if (x < w && fg2d<T>(row[x])) v |= 1ull << k;
+        if (x < w && fg2d<T>(row[x])) v |= 1ull << lane_bit(k);
     }
     out[idx] = v;
 }
@@ -117,7 +117,7 @@ __global__ void unpack2d_t(const Item2D *__restrict__ item, T *__restrict__ out)
     T *row = out + (size_t)y * it.w;
     for (int k = 0; k < 64; ++k) {
         int x = c * 64 + k;
-        if (x < it.w) row[x] = ((v >> k) & 1) ? one_value<T>() : (T)0;
+        if (x < it.w) row[x] = ((v >> lane_bit(k)) & 1) ? one_value<T>() : (T)0;
     }
 }
 

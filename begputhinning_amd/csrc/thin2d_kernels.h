@@ -1,10 +1,13 @@
 // thin2d_kernels.h -- gfx950 device code of the 2D thinning path.
 //
-// State layout in HBM: bit-packed rows, bit k of 64-bit word c of row y is
-// pixel x = 64*c + k (the reference packs 32-pixel words the same way,
-// 2D/CUDA_DeviceHelperFunctions.cuh:101).  A row has L = ceil(width/64)
-// words; bits past the width are zero and stay zero because a subiteration
-// only clears bits.
+// State layout in HBM: bit-packed rows, 64-bit word c of row y holding pixels
+// x = 64*c + k, k = 0..63 (the reference packs 32-pixel words of consecutive
+// pixels, 2D/CUDA_DeviceHelperFunctions.cuh:101), in lane order: pixel k at
+// bit lane_bit(k) = (k & 1) * 32 + (k >> 1), even pixels in the low half, odd
+// in the high half (pack.h), so that each half has one of its West/East
+// neighbours at the same bit of the other half.  A row has L =
+// ceil(width/64) words; bits of pixels past the width are zero and stay zero
+// because a subiteration only clears bits.
 //
 // Work unit: a "band" of consecutive rows of one image, processed by one
 // workgroup of NW wave64s.  Lane (g, c) = (lane / L, lane % L) of wave v owns
@@ -78,36 +81,46 @@ __device__ __forceinline__ uint32_t from_upper_lane(uint32_t v)
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, true);
 }
 
-// The 2V 32-bit halves of the V consecutive 64-bit words a lane holds of one
-// row, low pixels first: half i holds pixels 32 i .. 32 i + 31 of the lane.
+// The V consecutive 64-bit words a lane holds of one row, in lane order
+// (pack.h): h[2j] holds the even pixels of word j (bit b = pixel 2b of the
+// word), h[2j + 1] its odd pixels (bit b = pixel 2b + 1).
 template <int V>
 struct Row {
     uint32_t h[2 * V];
 };
 
-// bit k <- pixel x-1 (the reference's ShiftLeft): the halves shifted up by one
-// bit, the lowest taking the top bit of the lane below (masked at segment
-// starts)
+// bit <- pixel x-1 (the reference's ShiftLeft).  An odd pixel's West
+// neighbour is the even pixel at the same bit; an even pixel 2b's is odd
+// pixel 2b - 1, one bit down in the odd half, bit 0 taking bit 31 of the
+// previous word's odd half (of the lane below for the first word, masked at
+// segment starts).
 template <int V>
 __device__ __forceinline__ Row<V> shift_w(const Row<V> &v, uint32_t carry_mask)
 {
     Row<V> o;
     const uint32_t c = from_lower_lane(v.h[2 * V - 1]) & carry_mask;
-    o.h[0] = __builtin_amdgcn_alignbit(v.h[0], c, 31);
 #pragma unroll
-    for (int i = 1; i < 2 * V; ++i) o.h[i] = __builtin_amdgcn_alignbit(v.h[i], v.h[i - 1], 31);
+    for (int j = 0; j < V; ++j) {
+        o.h[2 * j] = __builtin_amdgcn_alignbit(v.h[2 * j + 1], j ? v.h[2 * j - 1] : c, 31);
+        o.h[2 * j + 1] = v.h[2 * j];
+    }
     return o;
 }
-// bit k <- pixel x+1 (the reference's ShiftRight): shifted down by one bit, the
-// highest taking the bottom bit of the lane above (masked at segment ends)
+// bit <- pixel x+1 (the reference's ShiftRight).  An even pixel's East
+// neighbour is the odd pixel at the same bit; an odd pixel 2b + 1's is even
+// pixel 2b + 2, one bit up in the even half, bit 31 taking bit 0 of the next
+// word's even half (of the lane above for the last word, masked at segment
+// ends).
 template <int V>
 __device__ __forceinline__ Row<V> shift_e(const Row<V> &v, uint32_t carry_mask)
 {
     Row<V> o;
     const uint32_t c = from_upper_lane(v.h[0]) & carry_mask;
 #pragma unroll
-    for (int i = 0; i + 1 < 2 * V; ++i) o.h[i] = __builtin_amdgcn_alignbit(v.h[i + 1], v.h[i], 1);
-    o.h[2 * V - 1] = __builtin_amdgcn_alignbit(c, v.h[2 * V - 1], 1);
+    for (int j = 0; j < V; ++j) {
+        o.h[2 * j] = v.h[2 * j + 1];
+        o.h[2 * j + 1] = __builtin_amdgcn_alignbit(j + 1 < V ? v.h[2 * j + 2] : c, v.h[2 * j], 1);
+    }
     return o;
 }
 
@@ -292,7 +305,7 @@ __device__ __forceinline__ int band_work(const Item2D &it, int row0, int nrows, 
                 uint64_t v = 0;
                 if (row_ok && w0 + j < L) {
                     const int x0 = 64 * (w0 + j);
-                    v = pack_run_u8(in + (size_t)y * in_pitch + x0, min(64, it.w - x0));
+                    v = lane_order(pack_run_u8(in + (size_t)y * in_pitch + x0, min(64, it.w - x0)));
                 }
                 w[r].h[2 * j] = (uint32_t)v;
                 w[r].h[2 * j + 1] = (uint32_t)(v >> 32);
