@@ -20,7 +20,14 @@
 //   seg_count  -- the chunk's point bits, their count, and each workgroup's sum;
 //   seg_scan   -- one workgroup: exclusive sums of the workgroup sums, total;
 //   seg_base   -- each chunk's first rank;
-//   seg_emit   -- per point: packed coordinates, code, neighbour ranks.
+//   seg_emit   -- per point: packed coordinates, code, neighbour ranks;
+// then one thread per point (or half-edge) for the chains of path points:
+//   seg_link, seg_jump  -- list ranking of the chains by pointer jumping;
+//   seg_chain_flags, seg_scan, seg_chain_base -- chain ids and records in
+//                          the order of each chain's first end;
+//   seg_chain_scatter   -- every chain's points in order.
+// The host's second pass then follows a chain as one run instead of point by
+// point.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -181,6 +188,207 @@ __global__ __launch_bounds__(256) void seg_emit(Frame f, long nchunks, const uin
     }
 }
 
+// ---- chains of path points -------------------------------------------------
+//
+// A path point (code 2) has at most two neighbours that are points; those that
+// are path points too link it into a chain.  Half-edge 2p + s of path point p
+// means "leave p through its slot s" (slot s = its s-th point neighbour in the
+// reference's order).  It is terminal when that slot holds no path point: the
+// walk ends at p, with the slot's point (or -1) as the neighbour off the
+// chain.  Otherwise it continues as the half-edge of the next point that does
+// not lead back to p.  Pointer jumping (Wyllie's list ranking) gives every
+// half-edge its end point, its distance to it and the neighbour off the chain
+// there; runs that close on themselves never end.
+
+struct Jump {
+    int *nxt;    // next half-edge, -1 terminal, -2 none (not a path point)
+    int *dist;   // steps to the end
+    int *end;    // end point (valid once terminal)
+    int *xo;     // neighbour off the chain at the end
+};
+
+// the first two neighbours of p that are points
+__device__ __forceinline__ void point_slots(const int *nbr, int nd, int p, int &a, int &b)
+{
+    a = b = -1;
+    const int *row = nbr + (size_t)p * nd;
+    for (int j = 0; j < nd; ++j) {
+        const int r = row[j];
+        if (r < 0) continue;
+        if (a < 0) a = r;
+        else { b = r; break; }
+    }
+}
+
+__global__ __launch_bounds__(256) void seg_link(int npts, int nd, const int8_t *code, const int *nbr,
+                                                Jump j)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= npts) return;
+    if (code[p] != 2) {
+        j.nxt[2 * p] = j.nxt[2 * p + 1] = -2;
+        return;
+    }
+    int sl[2];
+    point_slots(nbr, nd, p, sl[0], sl[1]);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int q = sl[s], h = 2 * p + s;
+        if (q >= 0 && code[q] == 2) {
+            int qa, qb;
+            point_slots(nbr, nd, q, qa, qb);
+            j.nxt[h] = 2 * q + (qa == p ? 1 : 0);  // q's slot that does not lead back to p
+            j.dist[h] = 1;
+            j.end[h] = -1;
+            j.xo[h] = -1;
+        } else {
+            j.nxt[h] = -1;
+            j.dist[h] = 0;
+            j.end[h] = p;
+            j.xo[h] = q;
+        }
+    }
+}
+
+// One launch follows kHops + 1 pointers of the previous launch's state, so
+// every half-edge's reach grows (kHops + 1)-fold (twice in the textbook
+// round): ceil(log5 n) launches for chains of up to n points.
+constexpr int kHops = 4;
+
+__global__ __launch_bounds__(256) void seg_jump(int nh, Jump a, Jump b)
+{
+    const int h = blockIdx.x * 256 + threadIdx.x;
+    if (h >= nh) return;
+    int n = a.nxt[h], d = a.dist[h], last = h;
+    for (int k = 0; k < kHops && n >= 0; ++k) {
+        d += a.dist[n];
+        last = n;
+        n = a.nxt[n];
+    }
+    b.nxt[h] = n;
+    b.dist[h] = d;
+    b.end[h] = a.end[last];
+    b.xo[h] = a.xo[last];
+}
+
+// exclusive prefix of v over the 256 threads of a workgroup (s_w: 4 ints)
+__device__ __forceinline__ int wg_exclusive(int v, int *s_w)
+{
+    int s = v;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(s, o);
+        if (lane >= o) s += t;
+    }
+    if (lane == 63) s_w[wv] = s;
+    __syncthreads();
+    int before = 0;
+    for (int k = 0; k < wv; ++k) before += s_w[k];
+    __syncthreads();
+    return before + s - v;
+}
+
+struct ChainTmp {
+    int *start;  // [npts] the chain's first end (e0) of a chain point, else -1
+    int *pos;    // [npts] position from e0
+    int *len;    // [npts] chain length, at e0 only (else 0)
+    int *wch, *wlen;  // [nwg] workgroup sums of chain starts and lengths
+    int *cid, *coff;  // [npts] chain id and first position, at e0 only
+};
+
+// per path point: its chain's e0 and its position; e0 also its length
+__global__ __launch_bounds__(256) void seg_chain_flags(int npts, const int8_t *code, Jump j,
+                                                       ChainTmp c)
+{
+    __shared__ int s_a[4], s_b[4];
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    int st = 0, ln = 0;
+    if (p < npts) {
+        int s0 = -1, ps = 0;
+        if (code[p] == 2 && j.nxt[2 * p] == -1 && j.nxt[2 * p + 1] == -1) {
+            const int a = j.end[2 * p], b = j.end[2 * p + 1];
+            s0 = a < b ? a : b;
+            ps = s0 == a ? j.dist[2 * p] : j.dist[2 * p + 1];
+            if (p == s0) {
+                st = 1;
+                ln = j.dist[2 * p] + j.dist[2 * p + 1] + 1;
+            }
+        }
+        c.start[p] = s0;
+        c.pos[p] = ps;
+        c.len[p] = ln;
+    }
+    int x = st, y = ln;
+    for (int o = 32; o > 0; o >>= 1) {
+        x += __shfl_xor(x, o);
+        y += __shfl_xor(y, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_a[threadIdx.x >> 6] = x;
+        s_b[threadIdx.x >> 6] = y;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c.wch[blockIdx.x] = s_a[0] + s_a[1] + s_a[2] + s_a[3];
+        c.wlen[blockIdx.x] = s_b[0] + s_b[1] + s_b[2] + s_b[3];
+    }
+}
+
+// chain ids and first positions in the order of e0, and the chain records
+__global__ __launch_bounds__(256) void seg_chain_base(int npts, Jump j, ChainTmp c, SegChain *rec)
+{
+    __shared__ int s_w[4];
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    const bool st = p < npts && c.start[p] == p;
+    const int ln = st ? c.len[p] : 0;
+    const int id = c.wch[blockIdx.x] + wg_exclusive(st ? 1 : 0, s_w);
+    const int off = c.wlen[blockIdx.x] + wg_exclusive(ln, s_w);
+    if (!st) return;
+    c.cid[p] = id;
+    c.coff[p] = off;
+    // e0 = p: one half-edge ends at p itself (its neighbour off the chain at
+    // e0), the other at e1 (or at p too, for a single point)
+    const int h0 = 2 * p, h1 = 2 * p + 1;
+    const bool first_here = j.end[h0] == p && (j.dist[h0] == 0);
+    SegChain r;
+    r.off = off;
+    r.len = ln;
+    r.e0 = p;
+    if (ln == 1) {
+        r.e1 = p;
+        r.x0 = j.xo[h0];
+        r.x1 = j.xo[h1];
+    } else if (first_here) {
+        r.e1 = j.end[h1];
+        r.x0 = j.xo[h0];
+        r.x1 = j.xo[h1];
+    } else {
+        r.e1 = j.end[h0];
+        r.x0 = j.xo[h1];
+        r.x1 = j.xo[h0];
+    }
+    rec[id] = r;
+}
+
+// every chain point: its chain id, and its place in the chain's run
+__global__ __launch_bounds__(256) void seg_chain_scatter(int npts, const int64_t *idx, ChainTmp c,
+                                                         int *chain, SegChainPt *pts)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= npts) return;
+    const int s0 = c.start[p];
+    if (s0 < 0) {
+        chain[p] = -1;
+        return;
+    }
+    chain[p] = c.cid[s0];
+    SegChainPt q;
+    q.xyz = idx[p];
+    q.rank = p;
+    q.pad = 0;
+    pts[c.coff[s0] + c.pos[p]] = q;
+}
+
 // Grow-only pinned host staging of one thread's pre-pass results (device to
 // host copies into pageable memory go through the runtime's own staging),
 // released at thread exit.
@@ -257,32 +465,95 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     if (total > 0x7fffffffL / g.nd) return fail(BGT_ERR_UNSUPPORTED, "too many skeleton points");
     g.npts = (int)total;
     if (!g.npts) return BGT_OK;
-    const size_t out = round_up(sizeof(int64_t) * total, 256) + round_up((size_t)total, 256) +
-                       round_up(sizeof(int) * total * g.nd, 256);
+    const int n = g.npts;
+    // device results, in the order of the host copy: the per-point arrays, then
+    // the chain records and the chains' points (each sized for the most there
+    // can be; only the used part is copied)
+    const size_t fixed = round_up(sizeof(int64_t) * n, 256) + round_up((size_t)n, 256) +
+                         round_up(sizeof(int) * (size_t)n * g.nd, 256) + round_up(sizeof(int) * (size_t)n, 256);
+    const size_t rec_cap = round_up(sizeof(SegChain) * (size_t)n, 256);
+    const size_t pts_cap = round_up(sizeof(SegChainPt) * (size_t)n, 256);
     Workspace &wo = thread_workspace(6);
-    BGT_CHECK(workspace_reserve(wo, out));
+    BGT_CHECK(workspace_reserve(wo, fixed + rec_cap + pts_cap));
     char *q = (char *)wo.ptr;
     int64_t *d_idx = (int64_t *)q;
-    q += round_up(sizeof(int64_t) * total, 256);
+    q += round_up(sizeof(int64_t) * n, 256);
     int8_t *d_code = (int8_t *)q;
-    q += round_up((size_t)total, 256);
+    q += round_up((size_t)n, 256);
     int *d_nbr = (int *)q;
+    q += round_up(sizeof(int) * (size_t)n * g.nd, 256);
+    int *d_chain = (int *)q;
+    q += round_up(sizeof(int) * (size_t)n, 256);
+    SegChain *d_rec = (SegChain *)q;
+    SegChainPt *d_pts = (SegChainPt *)(q + rec_cap);
     hipLaunchKernelGGL(seg_emit, dim3((unsigned)nwg), dim3(256), 0, s, f, nchunks, bits, base,
                        d_idx, d_code, d_nbr);
     BGT_HIP(hipGetLastError());
-    // the three arrays lie one after another in the workspace: one copy
-    BGT_CHECK(st.reserve(out));
-    BGT_HIP(hipMemcpyAsync(st.ptr, wo.ptr, out, hipMemcpyDeviceToHost, s));
+
+    // chains: link, ceil(log5 n) jump launches, chain starts and lengths, their
+    // prefix sums, records and runs
+    const int pwg = (n + 255) / 256, hwg = (2 * n + 255) / 256;
+    const size_t half = round_up(sizeof(int) * 2 * (size_t)n, 256), one = round_up(sizeof(int) * (size_t)n, 256),
+                 wsz = round_up(sizeof(int) * (size_t)pwg, 256);
+    Workspace &wt = thread_workspace(7);
+    BGT_CHECK(workspace_reserve(wt, 8 * half + 5 * one + 2 * wsz + 256));
+    char *t = (char *)wt.ptr;
+    Jump jb[2];
+    for (Jump &j : jb) {
+        j.nxt = (int *)t; t += half;
+        j.dist = (int *)t; t += half;
+        j.end = (int *)t; t += half;
+        j.xo = (int *)t; t += half;
+    }
+    ChainTmp ct;
+    ct.start = (int *)t; t += one;
+    ct.pos = (int *)t; t += one;
+    ct.len = (int *)t; t += one;
+    ct.cid = (int *)t; t += one;
+    ct.coff = (int *)t; t += one;
+    ct.wch = (int *)t; t += wsz;
+    ct.wlen = (int *)t; t += wsz;
+    long *d_tot = (long *)t;  // [0] chains, [1] chain points
+    hipLaunchKernelGGL(seg_link, dim3((unsigned)pwg), dim3(256), 0, s, n, g.nd, d_code, d_nbr, jb[0]);
+    int cur = 0;
+    for (long reach = 1; reach < n; reach *= kHops + 1, cur ^= 1)
+        hipLaunchKernelGGL(seg_jump, dim3((unsigned)hwg), dim3(256), 0, s, 2 * n, jb[cur], jb[cur ^ 1]);
+    hipLaunchKernelGGL(seg_chain_flags, dim3((unsigned)pwg), dim3(256), 0, s, n, d_code, jb[cur], ct);
+    hipLaunchKernelGGL(seg_scan, dim3(1), dim3(1024), 0, s, ct.wch, pwg, d_tot);
+    hipLaunchKernelGGL(seg_scan, dim3(1), dim3(1024), 0, s, ct.wlen, pwg, d_tot + 1);
+    hipLaunchKernelGGL(seg_chain_base, dim3((unsigned)pwg), dim3(256), 0, s, n, jb[cur], ct, d_rec);
+    hipLaunchKernelGGL(seg_chain_scatter, dim3((unsigned)pwg), dim3(256), 0, s, n, d_idx, ct, d_chain,
+                       d_pts);
+    BGT_HIP(hipGetLastError());
+    BGT_HIP(hipMemcpyAsync(st.ptr, d_tot, 2 * sizeof(long), hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
-    // the host trace reads the arrays at random: from ordinary pageable
-    // memory (this thread's copy, kept between calls) it runs faster
+    long tot[2];
+    std::memcpy(tot, st.ptr, sizeof(tot));
+    if (tot[0] < 0 || tot[0] > n || tot[1] < tot[0] || tot[1] > n)
+        return fail(BGT_ERR_HIP, "segment pre-pass: %ld chains of %ld points among %d points", tot[0], tot[1], n);
+    const size_t rec_bytes = sizeof(SegChain) * (size_t)tot[0], pts_bytes = sizeof(SegChainPt) * (size_t)tot[1];
+    // one pinned buffer: the per-point arrays, the records, the runs
+    BGT_CHECK(st.reserve(fixed + rec_bytes + pts_bytes));
+    BGT_HIP(hipMemcpyAsync(st.ptr, wo.ptr, fixed, hipMemcpyDeviceToHost, s));
+    if (rec_bytes) BGT_HIP(hipMemcpyAsync(st.ptr + fixed, d_rec, rec_bytes, hipMemcpyDeviceToHost, s));
+    if (pts_bytes)
+        BGT_HIP(hipMemcpyAsync(st.ptr + fixed + rec_bytes, d_pts, pts_bytes, hipMemcpyDeviceToHost, s));
+    BGT_HIP(hipStreamSynchronize(s));
+    // the host trace reads the point arrays at random: from ordinary pageable
+    // memory (this thread's copy, kept between calls) it runs faster than
+    // from the pinned buffer
+    const size_t bytes = fixed + rec_bytes + pts_bytes;
     thread_local std::vector<char> hv;
-    if (hv.size() < out) hv.resize(out);
-    std::memcpy(hv.data(), st.ptr, out);
+    if (hv.size() < bytes) hv.resize(bytes);
+    std::memcpy(hv.data(), st.ptr, bytes);
     const char *hq = hv.data();
     g.idx = (const int64_t *)hq;
     g.code = (const int8_t *)(hq + ((const char *)d_code - (const char *)d_idx));
     g.nbr = (const int *)(hq + ((const char *)d_nbr - (const char *)d_idx));
+    g.chain = (const int *)(hq + ((const char *)d_chain - (const char *)d_idx));
+    g.nchains = (int)tot[0];
+    g.chains = (const SegChain *)(hq + fixed);
+    g.chain_pts = (const SegChainPt *)(hq + fixed + rec_bytes);
     return BGT_OK;
 }
 

@@ -124,6 +124,18 @@ void trace(const uint8_t *mask, int w, int h, int d, bool three_d, Flat &out)
 // (seggraph.hip): the same visiting order, neighbour order and code
 // bookkeeping, with a point's rank in place of its raster index and a
 // neighbour rank of -1 (not a point) in place of a zero code.
+//
+// Chains.  A walk enters a path point only from a neighbour that is not a
+// path point, i.e. at an end of its chain, and the chain is then untouched:
+// its points change code only when a walk passes through them, and a walk
+// that enters a chain passes through all of it (from each point the only
+// live neighbour is the next one, the previous being cleared).  So the walk
+// emits the whole chain from the entered end, clears its codes, and goes on
+// from the far end e: to e's neighbour off the chain if that is live, else it
+// stops at e, as the step-by-step walk does.  For a single point, the
+// neighbour off the chain is the one of its two it was not entered from.  A
+// path point outside any chain (on a closed run, which nothing can enter)
+// keeps the step-by-step walk.
 void trace_graph(const SegGraph &g, int pw, int ph, bool three_d, Flat &out)
 {
     (void)pw;
@@ -137,12 +149,13 @@ void trace_graph(const SegGraph &g, int pw, int ph, bool three_d, Flat &out)
     out.x.reserve((size_t)g.npts + 16);
     out.y.reserve((size_t)g.npts + 16);
     if (three_d) out.z.reserve((size_t)g.npts + 16);
-    auto emit = [&](int r) {
-        const uint64_t q = (uint64_t)g.idx[r];  // x | y << 16 | z << 32
+    auto emit_xyz = [&](int64_t v) {
+        const uint64_t q = (uint64_t)v;  // x | y << 16 | z << 32
         out.x.push_back((int16_t)(q & 0xffffu));
         out.y.push_back((int16_t)((q >> 16) & 0xffffu));
         if (three_d) out.z.push_back((int16_t)(q >> 32));
     };
+    auto emit = [&](int r) { emit_xyz(g.idx[r]); };
     auto close = [&](bool end) {
         out.is_end.push_back(end ? 1 : 0);
         out.start.push_back((uint32_t)out.x.size());
@@ -160,13 +173,44 @@ void trace_graph(const SegGraph &g, int pw, int ph, bool three_d, Flat &out)
             ++ni;
             while (ni < nd && !live(nb[ni])) ++ni;
             if (ni >= nd) { close(end); continue; }
-            int t = nb[ni];
+            int t = nb[ni], prev = idx;
             while (code[t] == 2) {
+                const int ch = g.chain ? g.chain[t] : -1;
+                const SegChain c = ch >= 0 ? g.chains[ch] : SegChain{0, 0, -1, -1, -1, -1};
+                if (ch >= 0 && (t == c.e0 || t == c.e1)) {
+                    const SegChainPt *cp = g.chain_pts + c.off;
+                    int e, x;
+                    if (c.len == 1) {
+                        emit_xyz(cp[0].xyz);
+                        code[t] = 0;
+                        e = t;
+                        x = prev == c.x0 ? c.x1 : c.x0;
+                    } else if (t == c.e0) {
+                        for (int k = 0; k < c.len; ++k) {
+                            emit_xyz(cp[k].xyz);
+                            code[cp[k].rank] = 0;
+                        }
+                        e = c.e1;
+                        x = c.x1;
+                    } else {
+                        for (int k = c.len - 1; k >= 0; --k) {
+                            emit_xyz(cp[k].xyz);
+                            code[cp[k].rank] = 0;
+                        }
+                        e = c.e0;
+                        x = c.x0;
+                    }
+                    prev = e;
+                    t = live(x) ? x : e;
+                    continue;
+                }
                 code[t] = 0;
                 emit(t);
+                const int from = t;
                 const int *nt = &g.nbr[(size_t)t * nd];
                 for (int k = 0; k < nd; ++k)
                     if (live(nt[k])) { t = nt[k]; break; }
+                prev = from;
             }
             emit(t);
             if (std::abs((int)orig[t]) < 2) end = true;

@@ -11,7 +11,11 @@
 //   - CenterlineExtraction's prune/smooth/flatten of the padded frame's
 //     segments, float bits included, for several (max_segment_length,
 //     smoothing_span), against or2d/or3d_centerline;
-//   - the MEX [y x (z)] columns, against a direct column-major copy.
+//   - the MEX [y x (z)] columns, against a direct column-major copy;
+//   - the tracer's second pass over the device pre-pass's graph (points,
+//     codes, neighbour ranks and chains of path points, built here on the
+//     host), jumping whole chains and stepping point by point, on the same
+//     masks and on dense noise.
 // Any sanitizer report aborts the program (-fno-sanitize-recover).
 #include <cstdint>
 #include <cstdio>
@@ -21,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "../../begputhinning_amd/csrc/seggraph.h"
 #include "../../begputhinning_amd/csrc/segments.h"
 #include "../../oracle/oracle.h"
 
@@ -107,6 +112,140 @@ static void check_segments(const std::vector<uint8_t> &m, int w, int h, int d, b
     std::free(s.x); std::free(s.y); std::free(s.z); std::free(s.seg_start); std::free(s.is_end);
 }
 
+// The device pre-pass's graph (begputhinning_amd/csrc/seggraph.hip) built on
+// the host for a mask whose frame is the mask itself: the set voxels off the
+// frame's border in raster order, their packed coordinates, codes (neighbours
+// counted over the whole mask) and neighbour ranks; and the chains of path
+// points, each walked here from its end of lower rank (the device ranks them
+// by pointer jumping instead).
+struct HostGraph {
+    std::vector<int64_t> idx;
+    std::vector<int8_t> code;
+    std::vector<int> nbr, chain;
+    std::vector<bgt::SegChain> chains;
+    std::vector<bgt::SegChainPt> pts;
+    bgt::SegGraph g;
+};
+
+static void host_graph(const std::vector<uint8_t> &m, int w, int h, int d, bool three_d, HostGraph &G)
+{
+    const int nd = three_d ? 26 : 8;
+    std::vector<int> rank(m.size(), -1);
+    int n = 0;
+    auto inner = [&](int x, int y, int z) {
+        return x >= 1 && x < w - 1 && y >= 1 && y < h - 1 && (!three_d || (z >= 1 && z < d - 1));
+    };
+    for (int z = 0; z < d; ++z)
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x)
+                if (m[((size_t)z * h + y) * w + x] && inner(x, y, z)) rank[((size_t)z * h + y) * w + x] = n++;
+    G.idx.assign(n, 0);
+    G.code.assign(n, 0);
+    G.nbr.assign((size_t)n * nd, -1);
+    for (int z = 0; z < d; ++z)
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) {
+                const int r = rank[((size_t)z * h + y) * w + x];
+                if (r < 0) continue;
+                G.idx[r] = (int64_t)x | (int64_t)y << 16 | (int64_t)z << 32;
+                int nn = 0, j = 0;
+                for (int dz = three_d ? -1 : 0; dz <= (three_d ? 1 : 0); ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            if (!dz && !dy && !dx) continue;
+                            const size_t q = ((size_t)(z + dz) * h + (y + dy)) * w + (x + dx);
+                            nn += m[q] != 0;
+                            G.nbr[(size_t)r * nd + j++] = rank[q];
+                        }
+                if (nn == 0) nn = 1;
+                G.code[r] = (int8_t)(nn == 2 ? 2 : -nn);
+            }
+    // the (at most two) point neighbours of a path point, in neighbour order
+    auto slots = [&](int p, int *s) {
+        int k = 0;
+        s[0] = s[1] = -1;
+        for (int j = 0; j < nd && k < 2; ++j)
+            if (G.nbr[(size_t)p * nd + j] >= 0) s[k++] = G.nbr[(size_t)p * nd + j];
+    };
+    G.chain.assign(n, -1);
+    G.chains.clear();
+    G.pts.clear();
+    for (int p = 0; p < n; ++p) {
+        if (G.code[p] != 2 || G.chain[p] >= 0) continue;
+        int s[2];
+        slots(p, s);
+        int on_path = 0;
+        for (int v : s) on_path += v >= 0 && G.code[v] == 2;
+        if (on_path == 2) continue;  // not an end: reached from one, or on a closed run
+        bgt::SegChain c{};
+        c.off = (int)G.pts.size();
+        c.e0 = p;
+        const int id = (int)G.chains.size();
+        int off_chain[2] = {-1, -1}, k = 0;
+        for (int v : s)
+            if (v >= 0 && G.code[v] != 2) off_chain[k++] = v;
+        int prev = -1, cur = p;
+        for (;;) {
+            G.chain[cur] = id;
+            G.pts.push_back(bgt::SegChainPt{G.idx[cur], cur, 0});
+            int t[2], next = -1;
+            slots(cur, t);
+            for (int v : t)
+                if (v >= 0 && v != prev && G.code[v] == 2 && G.chain[v] < 0) { next = v; break; }
+            if (next < 0) break;
+            prev = cur;
+            cur = next;
+        }
+        c.e1 = cur;
+        c.len = (int)G.pts.size() - c.off;
+        c.x0 = off_chain[0];
+        if (c.len == 1) {
+            c.x1 = off_chain[1];
+        } else {
+            int t[2];
+            slots(cur, t);
+            c.x1 = -1;
+            for (int v : t)
+                if (v >= 0 && G.code[v] != 2) { c.x1 = v; break; }
+        }
+        G.chains.push_back(c);
+    }
+    G.g = bgt::SegGraph{};
+    G.g.npts = n;
+    G.g.nd = nd;
+    G.g.idx = G.idx.data();
+    G.g.code = G.code.data();
+    G.g.nbr = G.nbr.data();
+    G.g.chain = G.chain.data();
+    G.g.nchains = (int)G.chains.size();
+    G.g.chains = G.chains.data();
+    G.g.chain_pts = G.pts.data();
+}
+
+// the host's second pass over the pre-pass graph, jumping chains and stepping
+// point by point, against the oracle's tracer
+static void check_graph(const std::vector<uint8_t> &m, int w, int h, int d, bool three_d, const char *what)
+{
+    if (w < 1 || h < 1 || d < 1) return;
+    HostGraph G;
+    host_graph(m, w, h, d, three_d, G);
+    or_segments o{};
+    if (three_d)
+        or3d_extract(m.data(), w, h, d, &o);
+    else
+        or2d_extract(m.data(), w, h, &o);
+    for (int pass = 0; pass < 2; ++pass) {
+        bgt::SegGraph g = G.g;
+        if (pass) g.chain = nullptr;  // step by step
+        bgt_segments s{};
+        EXPECT(bgt::extract_segments_graph(g, w, h, three_d, &s) == BGT_OK, "%s: graph trace", what);
+        EXPECT(same_segments(s, o, three_d), "%s %dx%dx%d: graph trace (%s) differs", what, w, h, d,
+               pass ? "step by step" : "chains");
+        std::free(s.x); std::free(s.y); std::free(s.z); std::free(s.seg_start); std::free(s.is_end);
+    }
+    or_segments_free(&o);
+}
+
 static void check_centerline(const std::vector<uint8_t> &img, int w, int h, int d, bool three_d,
                              int maxlen, int span)
 {
@@ -179,20 +318,39 @@ int main(int argc, char **argv)
         const int w = side2(g), h = side2(g);
         auto raw = random_mask(g, (size_t)w * h, 0.3);
         check_segments(raw, w, h, 1, false, "2D raw");
+        check_graph(raw, w, h, 1, false, "2D raw");
         auto img = blobs(g, w, h, 1, 1 + r % 9);
         auto skel = img;
         or2d_thin(skel.data(), w, h, 0);
         check_segments(skel, w, h, 1, false, "2D skeleton");
+        check_graph(skel, w, h, 1, false, "2D skeleton");
         check_centerline(img, w, h, 1, false, params[r % 5][0], params[r % 5][1]);
         // 3D
         const int w3 = side3(g), h3 = side3(g), d3 = side3(g);
         auto raw3 = random_mask(g, (size_t)w3 * h3 * d3, 0.25);
         check_segments(raw3, w3, h3, d3, true, "3D raw");
+        check_graph(raw3, w3, h3, d3, true, "3D raw");
         auto vol = blobs(g, w3, h3, d3, 1 + r % 5);
         auto skel3 = vol;
         or3d_thin(skel3.data(), w3, h3, d3, 0);
         check_segments(skel3, w3, h3, d3, true, "3D skeleton");
+        check_graph(skel3, w3, h3, d3, true, "3D skeleton");
         check_centerline(vol, w3, h3, d3, true, params[(r + 2) % 5][0], params[(r + 2) % 5][1]);
+    }
+    // dense noise at many densities: junctions of every degree, nodes next to
+    // nodes, chains closing on themselves or on one node, points next to the
+    // border
+    std::uniform_real_distribution<double> dens(0.05, 0.7);
+    std::uniform_int_distribution<int> small(3, 40), small3(3, 12);
+    for (int r = 0; r < rounds * 40; ++r) {
+        const int w = small(g), h = small(g);
+        auto m = random_mask(g, (size_t)w * h, dens(g));
+        if (r & 1) or2d_thin(m.data(), w, h, 0);
+        check_graph(m, w, h, 1, false, "2D noise");
+        const int w3 = small3(g), h3 = small3(g), d3 = small3(g);
+        auto m3 = random_mask(g, (size_t)w3 * h3 * d3, dens(g) * 0.6);
+        if (r & 1) or3d_thin(m3.data(), w3, h3, d3, 0);
+        check_graph(m3, w3, h3, d3, true, "3D noise");
     }
     // empty and one-pixel frames
     std::vector<uint8_t> one(1, 1), none(1, 0);
