@@ -1117,12 +1117,14 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         bool live = false;
         bool final = false;          // its last launch is queued (the 3D tail)
         int head = 0, inflight = 0;  // oldest pending poll slot, pending polls
+        bool tiles[2] = {false, false};  // a slot's poll holds the tile engine's words
         long chunks = 0;
     } c3;
     c3.live = n3 > 0;
     auto enqueue3 = [&](Chain &c) -> int {
         const int slot = (c.head + c.inflight) & 1;
         int *hs = b.h_live + slot * 32;
+        c.tiles[slot] = b.on_tiles3;
         if (b.on_tiles3) {
             // a chunk of iterations, one launch each; iteration t closes the
             // items whose iteration t - 1 changed nothing
@@ -1158,8 +1160,15 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     // reads the oldest pending poll (its event has completed)
     auto consume3 = [&](Chain &c) {
         const int *hp = b.h_live + c.head * 32;
-        if (b.on_tiles3) {
+        if (c.tiles[c.head]) {
             if (hp[8 + TC_LIVE] <= 0) c.live = false;
+            c.head ^= 1;
+            --c.inflight;
+            return;
+        }
+        if (b.on_tiles3) {
+            // a word-list chunk queued before the tile engine took over: the
+            // tile engine's polls follow it
             c.head ^= 1;
             --c.inflight;
             return;
@@ -1178,14 +1187,18 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     // both.
     auto drive3 = [&](Chain &c) -> int {
         while (c.live) {
-            // engine 1: once few words are active, the tile engine takes over
-            // after the chunks in flight (the host then knows the iteration)
+            // engine 1: once few words are active, the tile engine takes over,
+            // queued behind the word-list chunks in flight: each runs kChunk
+            // iterations (iterations past convergence find empty lists), so
+            // the takeover iteration is known, and the list count last read
+            // bounds the later ones
             const bool to_tiles = b.c3.engine == 1 && !b.on_tiles3 && b.c3.tail_words > 0 &&
                                   b.iter3 >= 1 && (long)b.abound3 <= (long)b.c3.tail_words * n3;
-            if (to_tiles && c.inflight == 0) {
-                BGT_CHECK(tile3d_from_words(b.t3, b.e3, b.iter3, b.bound3, s3));
+            if (to_tiles) {
+                const int k = b.iter3 + c.inflight * kChunk;
+                BGT_CHECK(tile3d_from_words(b.t3, b.e3, k, b.bound3, s3));
                 b.on_tiles3 = true;
-                b.titer3 = b.iter3;
+                b.titer3 = k;
                 continue;
             }
             if (c.inflight < 2 && !c.final && !to_tiles) {
