@@ -10,36 +10,39 @@ namespace bgt {
 
 // A chain: a maximal run of path points (code 2) linked through each other,
 // from end e0 to end e1 (e0 == e1 for a single point), e0 the end of lower
-// rank.  x0 / x1 are the points next to e0 / e1 off the chain (-1: none);
-// for a single point, its two neighbours that are points (in the reference's
-// order).  Runs that close on themselves form no chain.
+// rank.  x0 / x1 are the points next to e0 / e1 off the chain (-1: none),
+// with their packed coordinates; for a single point, its two neighbours that
+// are points (in the reference's order).  Runs that close on themselves form
+// no chain.
 struct SegChain {
-    int off, len;  // the chain's points: chain_pts[off .. off + len), from e0 to e1
+    int off, len;  // the chain's points: chain_xyz[off .. off + len), from e0 to e1
     int e0, e1, x0, x1;
-};
-struct SegChainPt {
-    int64_t xyz;   // packed coordinates
-    int rank, pad;
+    int64_t x0xyz, x1xyz;
 };
 
-// Skeleton points of a padded frame in raster order: coordinates packed as
-// x | y << 16 | z << 32 (sides are at most 32767), the tracer's code (-1
-// isolated, 2 path, -n end or junction with n neighbours), and for each of
-// the nd neighbours in the reference's order the rank of that neighbour if it
-// is a point itself, else -1; and the chains of path points, each point's
-// chain (-1: none) and the chains' points in order.
+// The skeleton points of a padded frame as the host's second pass needs them,
+// points ranked in raster order and coordinates packed as x | y << 16 |
+// z << 32 (sides are at most 32767):
+//   - per point, the tracer's code (-1 isolated, 2 path, -n end or junction
+//     with n neighbours) and its chain (-1: none);
+//   - the nodes (points of negative code) in rank order: rank, coordinates,
+//     and for each of the nd neighbours in the reference's order the rank of
+//     that neighbour if it is a point itself, else -1;
+//   - the chains, and their points' coordinates in order.
 // The arrays are views of a per-thread host buffer, valid until the thread's
 // next segment_graph_device call.
 struct SegGraph {
     int npts = 0;
     int nd = 8;
-    const int64_t *idx = nullptr;
     const int8_t *code = nullptr;
-    const int *nbr = nullptr;  // npts x nd
-    const int *chain = nullptr;  // npts (null: no chains, the tracer steps point by point)
+    const int *chain = nullptr;
+    int nnodes = 0;
+    const int *node_rank = nullptr;
+    const int64_t *node_xyz = nullptr;
+    const int *node_nbr = nullptr;  // nnodes x nd
     int nchains = 0;
     const SegChain *chains = nullptr;
-    const SegChainPt *chain_pts = nullptr;
+    const int64_t *chain_xyz = nullptr;
 };
 
 // d_skel: w x h x d uint8 skeleton in device memory (nonzero = set), the top

@@ -292,20 +292,32 @@ struct ChainTmp {
     int *start;  // [npts] the chain's first end (e0) of a chain point, else -1
     int *pos;    // [npts] position from e0
     int *len;    // [npts] chain length, at e0 only (else 0)
-    int *wch, *wlen;  // [nwg] workgroup sums of chain starts and lengths
+    int *wch, *wlen, *wnode;  // [nwg] workgroup sums of chain starts, lengths, nodes
     int *cid, *coff;  // [npts] chain id and first position, at e0 only
 };
 
-// per path point: its chain's e0 and its position; e0 also its length
+// What the host's second pass reads (SegGraph), each array sized for npts.
+struct GraphOut {
+    int *chain;
+    int *node_rank;
+    int64_t *node_xyz;
+    int *node_nbr;
+    SegChain *rec;
+    int64_t *chain_xyz;
+};
+
+// per path point: its chain's e0 and its position; e0 also its length;
+// workgroup sums of chain starts, chain lengths and nodes
 __global__ __launch_bounds__(256) void seg_chain_flags(int npts, const int8_t *code, Jump j,
                                                        ChainTmp c)
 {
-    __shared__ int s_a[4], s_b[4];
+    __shared__ int s_a[4], s_b[4], s_c[4];
     const int p = blockIdx.x * 256 + threadIdx.x;
-    int st = 0, ln = 0;
+    int st = 0, ln = 0, nodes = 0;
     if (p < npts) {
         int s0 = -1, ps = 0;
-        if (code[p] == 2 && j.nxt[2 * p] == -1 && j.nxt[2 * p + 1] == -1) {
+        const int8_t cd = code[p];
+        if (cd == 2 && j.nxt[2 * p] == -1 && j.nxt[2 * p + 1] == -1) {
             const int a = j.end[2 * p], b = j.end[2 * p + 1];
             s0 = a < b ? a : b;
             ps = s0 == a ? j.dist[2 * p] : j.dist[2 * p + 1];
@@ -314,42 +326,56 @@ __global__ __launch_bounds__(256) void seg_chain_flags(int npts, const int8_t *c
                 ln = j.dist[2 * p] + j.dist[2 * p + 1] + 1;
             }
         }
+        nodes = cd < 0;
         c.start[p] = s0;
         c.pos[p] = ps;
         c.len[p] = ln;
     }
-    int x = st, y = ln;
+    int x = st, y = ln, z = nodes;
     for (int o = 32; o > 0; o >>= 1) {
         x += __shfl_xor(x, o);
         y += __shfl_xor(y, o);
+        z += __shfl_xor(z, o);
     }
     if ((threadIdx.x & 63) == 0) {
         s_a[threadIdx.x >> 6] = x;
         s_b[threadIdx.x >> 6] = y;
+        s_c[threadIdx.x >> 6] = z;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         c.wch[blockIdx.x] = s_a[0] + s_a[1] + s_a[2] + s_a[3];
         c.wlen[blockIdx.x] = s_b[0] + s_b[1] + s_b[2] + s_b[3];
+        c.wnode[blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
     }
 }
 
-// chain ids and first positions in the order of e0, and the chain records
-__global__ __launch_bounds__(256) void seg_chain_base(int npts, Jump j, ChainTmp c, SegChain *rec)
+// chain ids and first positions in the order of e0 and the chain records;
+// the nodes in rank order with their coordinates and neighbour ranks
+__global__ __launch_bounds__(256) void seg_chain_base(int npts, int nd, const int64_t *idx,
+                                                      const int8_t *code, const int *nbr, Jump j,
+                                                      ChainTmp c, GraphOut o)
 {
     __shared__ int s_w[4];
     const int p = blockIdx.x * 256 + threadIdx.x;
     const bool st = p < npts && c.start[p] == p;
+    const bool node = p < npts && code[p] < 0;
     const int ln = st ? c.len[p] : 0;
     const int id = c.wch[blockIdx.x] + wg_exclusive(st ? 1 : 0, s_w);
     const int off = c.wlen[blockIdx.x] + wg_exclusive(ln, s_w);
+    const int nid = c.wnode[blockIdx.x] + wg_exclusive(node ? 1 : 0, s_w);
+    if (node) {
+        o.node_rank[nid] = p;
+        o.node_xyz[nid] = idx[p];
+        for (int k = 0; k < nd; ++k) o.node_nbr[(size_t)nid * nd + k] = nbr[(size_t)p * nd + k];
+    }
     if (!st) return;
     c.cid[p] = id;
     c.coff[p] = off;
     // e0 = p: one half-edge ends at p itself (its neighbour off the chain at
     // e0), the other at e1 (or at p too, for a single point)
     const int h0 = 2 * p, h1 = 2 * p + 1;
-    const bool first_here = j.end[h0] == p && (j.dist[h0] == 0);
+    const bool first_here = j.end[h0] == p && j.dist[h0] == 0;
     SegChain r;
     r.off = off;
     r.len = ln;
@@ -367,26 +393,25 @@ __global__ __launch_bounds__(256) void seg_chain_base(int npts, Jump j, ChainTmp
         r.x0 = j.xo[h1];
         r.x1 = j.xo[h0];
     }
-    rec[id] = r;
+    r.x0xyz = r.x0 >= 0 ? idx[r.x0] : 0;
+    r.x1xyz = r.x1 >= 0 ? idx[r.x1] : 0;
+    o.rec[id] = r;
 }
 
-// every chain point: its chain id, and its place in the chain's run
+// every chain point: its chain id, and its coordinates at its place in the
+// chain's run
 __global__ __launch_bounds__(256) void seg_chain_scatter(int npts, const int64_t *idx, ChainTmp c,
-                                                         int *chain, SegChainPt *pts)
+                                                         GraphOut o)
 {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= npts) return;
     const int s0 = c.start[p];
     if (s0 < 0) {
-        chain[p] = -1;
+        o.chain[p] = -1;
         return;
     }
-    chain[p] = c.cid[s0];
-    SegChainPt q;
-    q.xyz = idx[p];
-    q.rank = p;
-    q.pad = 0;
-    pts[c.coff[s0] + c.pos[p]] = q;
+    o.chain[p] = c.cid[s0];
+    o.chain_xyz[c.coff[s0] + c.pos[p]] = idx[p];
 }
 
 // Grow-only pinned host staging of one thread's pre-pass results (device to
@@ -466,37 +491,44 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     g.npts = (int)total;
     if (!g.npts) return BGT_OK;
     const int n = g.npts;
-    // device results, in the order of the host copy: the per-point arrays, then
-    // the chain records and the chains' points (each sized for the most there
-    // can be; only the used part is copied)
-    const size_t fixed = round_up(sizeof(int64_t) * n, 256) + round_up((size_t)n, 256) +
-                         round_up(sizeof(int) * (size_t)n * g.nd, 256) + round_up(sizeof(int) * (size_t)n, 256);
-    const size_t rec_cap = round_up(sizeof(SegChain) * (size_t)n, 256);
-    const size_t pts_cap = round_up(sizeof(SegChainPt) * (size_t)n, 256);
+    // the pre-pass's per-point arrays (device only)
+    const size_t a_idx = round_up(sizeof(int64_t) * n, 256), a_code = round_up((size_t)n, 256),
+                 a_nbr = round_up(sizeof(int) * (size_t)n * g.nd, 256), a_int = round_up(sizeof(int) * (size_t)n, 256);
+    // what the host reads, each region sized for n points (only the used part
+    // of each is copied)
+    const size_t a_nxyz = round_up(sizeof(int64_t) * n, 256), a_rec = round_up(sizeof(SegChain) * (size_t)n, 256);
     Workspace &wo = thread_workspace(6);
-    BGT_CHECK(workspace_reserve(wo, fixed + rec_cap + pts_cap));
+    BGT_CHECK(workspace_reserve(wo, a_idx + a_code + a_nbr + 3 * a_int + 2 * a_nxyz + a_nbr + a_rec));
     char *q = (char *)wo.ptr;
     int64_t *d_idx = (int64_t *)q;
-    q += round_up(sizeof(int64_t) * n, 256);
-    int8_t *d_code = (int8_t *)q;
-    q += round_up((size_t)n, 256);
+    q += a_idx;
     int *d_nbr = (int *)q;
-    q += round_up(sizeof(int) * (size_t)n * g.nd, 256);
-    int *d_chain = (int *)q;
-    q += round_up(sizeof(int) * (size_t)n, 256);
-    SegChain *d_rec = (SegChain *)q;
-    SegChainPt *d_pts = (SegChainPt *)(q + rec_cap);
+    q += a_nbr;
+    int8_t *d_code = (int8_t *)q;  // the host copy starts here: code, chain
+    q += a_code;
+    GraphOut go;
+    go.chain = (int *)q;
+    q += a_int;
+    go.node_rank = (int *)q;
+    q += a_int;
+    go.node_xyz = (int64_t *)q;
+    q += a_nxyz;
+    go.node_nbr = (int *)q;
+    q += a_nbr;
+    go.rec = (SegChain *)q;
+    q += a_rec;
+    go.chain_xyz = (int64_t *)q;
     hipLaunchKernelGGL(seg_emit, dim3((unsigned)nwg), dim3(256), 0, s, f, nchunks, bits, base,
                        d_idx, d_code, d_nbr);
     BGT_HIP(hipGetLastError());
 
-    // chains: link, ceil(log5 n) jump launches, chain starts and lengths, their
-    // prefix sums, records and runs
+    // chains: link, ceil(log5 n) jump launches, chain starts and lengths and
+    // nodes, their prefix sums, records, nodes and runs
     const int pwg = (n + 255) / 256, hwg = (2 * n + 255) / 256;
     const size_t half = round_up(sizeof(int) * 2 * (size_t)n, 256), one = round_up(sizeof(int) * (size_t)n, 256),
                  wsz = round_up(sizeof(int) * (size_t)pwg, 256);
     Workspace &wt = thread_workspace(7);
-    BGT_CHECK(workspace_reserve(wt, 8 * half + 5 * one + 2 * wsz + 256));
+    BGT_CHECK(workspace_reserve(wt, 8 * half + 5 * one + 3 * wsz + 256));
     char *t = (char *)wt.ptr;
     Jump jb[2];
     for (Jump &j : jb) {
@@ -513,7 +545,8 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     ct.coff = (int *)t; t += one;
     ct.wch = (int *)t; t += wsz;
     ct.wlen = (int *)t; t += wsz;
-    long *d_tot = (long *)t;  // [0] chains, [1] chain points
+    ct.wnode = (int *)t; t += wsz;
+    long *d_tot = (long *)t;  // [0] chains, [1] chain points, [2] nodes
     hipLaunchKernelGGL(seg_link, dim3((unsigned)pwg), dim3(256), 0, s, n, g.nd, d_code, d_nbr, jb[0]);
     int cur = 0;
     for (long reach = 1; reach < n; reach *= kHops + 1, cur ^= 1)
@@ -521,39 +554,56 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     hipLaunchKernelGGL(seg_chain_flags, dim3((unsigned)pwg), dim3(256), 0, s, n, d_code, jb[cur], ct);
     hipLaunchKernelGGL(seg_scan, dim3(1), dim3(1024), 0, s, ct.wch, pwg, d_tot);
     hipLaunchKernelGGL(seg_scan, dim3(1), dim3(1024), 0, s, ct.wlen, pwg, d_tot + 1);
-    hipLaunchKernelGGL(seg_chain_base, dim3((unsigned)pwg), dim3(256), 0, s, n, jb[cur], ct, d_rec);
-    hipLaunchKernelGGL(seg_chain_scatter, dim3((unsigned)pwg), dim3(256), 0, s, n, d_idx, ct, d_chain,
-                       d_pts);
+    hipLaunchKernelGGL(seg_scan, dim3(1), dim3(1024), 0, s, ct.wnode, pwg, d_tot + 2);
+    hipLaunchKernelGGL(seg_chain_base, dim3((unsigned)pwg), dim3(256), 0, s, n, g.nd, d_idx, d_code, d_nbr,
+                       jb[cur], ct, go);
+    hipLaunchKernelGGL(seg_chain_scatter, dim3((unsigned)pwg), dim3(256), 0, s, n, d_idx, ct, go);
     BGT_HIP(hipGetLastError());
-    BGT_HIP(hipMemcpyAsync(st.ptr, d_tot, 2 * sizeof(long), hipMemcpyDeviceToHost, s));
+    BGT_HIP(hipMemcpyAsync(st.ptr, d_tot, 3 * sizeof(long), hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
-    long tot[2];
+    long tot[3];
     std::memcpy(tot, st.ptr, sizeof(tot));
-    if (tot[0] < 0 || tot[0] > n || tot[1] < tot[0] || tot[1] > n)
-        return fail(BGT_ERR_HIP, "segment pre-pass: %ld chains of %ld points among %d points", tot[0], tot[1], n);
-    const size_t rec_bytes = sizeof(SegChain) * (size_t)tot[0], pts_bytes = sizeof(SegChainPt) * (size_t)tot[1];
-    // one pinned buffer: the per-point arrays, the records, the runs
-    BGT_CHECK(st.reserve(fixed + rec_bytes + pts_bytes));
-    BGT_HIP(hipMemcpyAsync(st.ptr, wo.ptr, fixed, hipMemcpyDeviceToHost, s));
-    if (rec_bytes) BGT_HIP(hipMemcpyAsync(st.ptr + fixed, d_rec, rec_bytes, hipMemcpyDeviceToHost, s));
-    if (pts_bytes)
-        BGT_HIP(hipMemcpyAsync(st.ptr + fixed + rec_bytes, d_pts, pts_bytes, hipMemcpyDeviceToHost, s));
+    if (tot[0] < 0 || tot[0] > n || tot[1] < tot[0] || tot[1] > n || tot[2] < 0 || tot[2] > n)
+        return fail(BGT_ERR_HIP, "segment pre-pass: %ld chains of %ld points, %ld nodes among %d points",
+                    tot[0], tot[1], tot[2], n);
+    // one pinned buffer: code and chain, the nodes' ranks, coordinates and
+    // neighbours, the chain records, the chains' coordinates
+    const size_t nn = (size_t)tot[2];
+    struct Part {
+        const void *src;
+        size_t bytes;
+    } parts[] = {{d_code, a_code + a_int},  // code and chain lie one after the other
+                 {go.node_rank, sizeof(int) * nn},
+                 {go.node_xyz, sizeof(int64_t) * nn},
+                 {go.node_nbr, sizeof(int) * nn * g.nd},
+                 {go.rec, sizeof(SegChain) * (size_t)tot[0]},
+                 {go.chain_xyz, sizeof(int64_t) * (size_t)tot[1]}};
+    size_t at[6], bytes = 0;
+    for (int k = 0; k < 6; ++k) {
+        at[k] = bytes;
+        bytes += round_up(parts[k].bytes, 16);
+    }
+    BGT_CHECK(st.reserve(bytes));
+    for (int k = 0; k < 6; ++k)
+        if (parts[k].bytes)
+            BGT_HIP(hipMemcpyAsync(st.ptr + at[k], parts[k].src, parts[k].bytes, hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
-    // the host trace reads the point arrays at random: from ordinary pageable
-    // memory (this thread's copy, kept between calls) it runs faster than
-    // from the pinned buffer
-    const size_t bytes = fixed + rec_bytes + pts_bytes;
+    // the host trace reads these at random: from ordinary pageable memory
+    // (this thread's copy, kept between calls) it runs faster than from the
+    // pinned buffer
     thread_local std::vector<char> hv;
     if (hv.size() < bytes) hv.resize(bytes);
     std::memcpy(hv.data(), st.ptr, bytes);
     const char *hq = hv.data();
-    g.idx = (const int64_t *)hq;
-    g.code = (const int8_t *)(hq + ((const char *)d_code - (const char *)d_idx));
-    g.nbr = (const int *)(hq + ((const char *)d_nbr - (const char *)d_idx));
-    g.chain = (const int *)(hq + ((const char *)d_chain - (const char *)d_idx));
+    g.code = (const int8_t *)(hq + at[0]);
+    g.chain = (const int *)(hq + at[0] + a_code);
+    g.nnodes = (int)nn;
+    g.node_rank = (const int *)(hq + at[1]);
+    g.node_xyz = (const int64_t *)(hq + at[2]);
+    g.node_nbr = (const int *)(hq + at[3]);
     g.nchains = (int)tot[0];
-    g.chains = (const SegChain *)(hq + fixed);
-    g.chain_pts = (const SegChainPt *)(hq + fixed + rec_bytes);
+    g.chains = (const SegChain *)(hq + at[4]);
+    g.chain_xyz = (const int64_t *)(hq + at[5]);
     return BGT_OK;
 }
 

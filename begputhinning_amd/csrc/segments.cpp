@@ -125,22 +125,34 @@ void trace(const uint8_t *mask, int w, int h, int d, bool three_d, Flat &out)
 // bookkeeping, with a point's rank in place of its raster index and a
 // neighbour rank of -1 (not a point) in place of a zero code.
 //
-// Chains.  A walk enters a path point only from a neighbour that is not a
-// path point, i.e. at an end of its chain, and the chain is then untouched:
-// its points change code only when a walk passes through them, and a walk
-// that enters a chain passes through all of it (from each point the only
-// live neighbour is the next one, the previous being cleared).  So the walk
-// emits the whole chain from the entered end, clears its codes, and goes on
-// from the far end e: to e's neighbour off the chain if that is live, else it
-// stops at e, as the step-by-step walk does.  For a single point, the
-// neighbour off the chain is the one of its two it was not entered from.  A
-// path point outside any chain (on a closed run, which nothing can enter)
-// keeps the step-by-step walk.
-void trace_graph(const SegGraph &g, int pw, int ph, bool three_d, Flat &out)
+// Only nodes start segments, so the pass visits the node list.  A walk enters
+// a path point only from a neighbour that is not a path point, i.e. at an end
+// of its chain, and the chain is then untouched: its points change code only
+// when a walk passes through them, and a walk that enters a chain passes
+// through all of it (from each point the only live neighbour is the next one,
+// the previous being cleared).  So the walk emits the whole chain from the
+// entered end and goes on from the far end e: to e's neighbour off the chain
+// if that is live (a node: nodes never reach code 2), else it stops at e, as
+// the step-by-step walk does.  For a single point, the neighbour off the chain
+// is the one of its two the walk did not come from.  Of a chain's points only
+// the ends neighbour points off it, so only their codes are cleared; no other
+// point of it is looked at again.  A path point that is not such an end can
+// only be reached by stepping through a chain, which never happens, and is
+// reported as an inconsistent graph.
+int trace_graph(const SegGraph &g, bool three_d, Flat &out)
 {
-    (void)pw;
-    (void)ph;
     const int nd = g.nd;
+    // offsets of the nd neighbours in the packed coordinates, in the
+    // reference's order (a point's neighbours lie inside the frame, so no
+    // field borrows from another)
+    int64_t delta[26];
+    {
+        int j = 0;
+        for (int dz = three_d ? -1 : 0; dz <= (three_d ? 1 : 0); ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx)
+                    if (dz || dy || dx) delta[j++] = (int64_t)dx + (int64_t)dy * 65536 + (int64_t)dz * 4294967296LL;
+    }
     // this thread's working copy of the codes, kept between calls
     thread_local std::vector<int8_t> code;
     code.assign(g.code, g.code + g.npts);
@@ -149,75 +161,81 @@ void trace_graph(const SegGraph &g, int pw, int ph, bool three_d, Flat &out)
     out.x.reserve((size_t)g.npts + 16);
     out.y.reserve((size_t)g.npts + 16);
     if (three_d) out.z.reserve((size_t)g.npts + 16);
-    auto emit_xyz = [&](int64_t v) {
+    auto emit = [&](int64_t v) {
         const uint64_t q = (uint64_t)v;  // x | y << 16 | z << 32
         out.x.push_back((int16_t)(q & 0xffffu));
         out.y.push_back((int16_t)((q >> 16) & 0xffffu));
         if (three_d) out.z.push_back((int16_t)(q >> 32));
     };
-    auto emit = [&](int r) { emit_xyz(g.idx[r]); };
     auto close = [&](bool end) {
         out.is_end.push_back(end ? 1 : 0);
         out.start.push_back((uint32_t)out.x.size());
     };
     auto live = [&](int r) { return r >= 0 && code[r] != 0; };
-    for (int idx = 0; idx < g.npts; ++idx) {
+    for (int k = 0; k < g.nnodes; ++k) {
+        const int idx = g.node_rank[k];
         const int8_t nodes = code[idx];
         if (nodes >= 0) continue;
         code[idx] = 0;
-        const int *nb = &g.nbr[(size_t)idx * nd];
+        const int *nb = &g.node_nbr[(size_t)k * nd];
+        const int64_t here = g.node_xyz[k];
         int ni = -1;
         for (int8_t si = 0; si > nodes; --si) {
             bool end = std::abs((int)orig[idx]) < 2;
-            emit(idx);
+            emit(here);
             ++ni;
             while (ni < nd && !live(nb[ni])) ++ni;
             if (ni >= nd) { close(end); continue; }
-            int t = nb[ni], prev = idx;
-            while (code[t] == 2) {
-                const int ch = g.chain ? g.chain[t] : -1;
-                const SegChain c = ch >= 0 ? g.chains[ch] : SegChain{0, 0, -1, -1, -1, -1};
-                if (ch >= 0 && (t == c.e0 || t == c.e1)) {
-                    const SegChainPt *cp = g.chain_pts + c.off;
-                    int e, x;
-                    if (c.len == 1) {
-                        emit_xyz(cp[0].xyz);
-                        code[t] = 0;
-                        e = t;
-                        x = prev == c.x0 ? c.x1 : c.x0;
-                    } else if (t == c.e0) {
-                        for (int k = 0; k < c.len; ++k) {
-                            emit_xyz(cp[k].xyz);
-                            code[cp[k].rank] = 0;
-                        }
-                        e = c.e1;
-                        x = c.x1;
-                    } else {
-                        for (int k = c.len - 1; k >= 0; --k) {
-                            emit_xyz(cp[k].xyz);
-                            code[cp[k].rank] = 0;
-                        }
-                        e = c.e0;
-                        x = c.x0;
-                    }
-                    prev = e;
-                    t = live(x) ? x : e;
-                    continue;
+            int t = nb[ni];
+            int64_t txyz = here + delta[ni];
+            if (code[t] == 2) {
+                const int ch = g.chain[t];
+                if (ch < 0 || ch >= g.nchains)
+                    return fail(BGT_ERR_HIP, "segment graph: path point %d on no chain", t);
+                const SegChain c = g.chains[ch];
+                if (t != c.e0 && t != c.e1)
+                    return fail(BGT_ERR_HIP, "segment graph: chain %d entered inside", ch);
+                const int64_t *cp = g.chain_xyz + c.off;
+                int e, x;
+                int64_t exyz, xxyz;
+                if (c.len == 1) {
+                    emit(cp[0]);
+                    e = t;
+                    exyz = cp[0];
+                    const bool from0 = idx == c.x0;
+                    x = from0 ? c.x1 : c.x0;
+                    xxyz = from0 ? c.x1xyz : c.x0xyz;
+                } else if (t == c.e0) {
+                    for (int q = 0; q < c.len; ++q) emit(cp[q]);
+                    e = c.e1;
+                    exyz = cp[c.len - 1];
+                    x = c.x1;
+                    xxyz = c.x1xyz;
+                } else {
+                    for (int q = c.len - 1; q >= 0; --q) emit(cp[q]);
+                    e = c.e0;
+                    exyz = cp[0];
+                    x = c.x0;
+                    xxyz = c.x0xyz;
                 }
-                code[t] = 0;
-                emit(t);
-                const int from = t;
-                const int *nt = &g.nbr[(size_t)t * nd];
-                for (int k = 0; k < nd; ++k)
-                    if (live(nt[k])) { t = nt[k]; break; }
-                prev = from;
+                code[c.e0] = 0;
+                code[c.e1] = 0;
+                if (live(x)) {
+                    t = x;
+                    txyz = xxyz;
+                } else {
+                    t = e;
+                    txyz = exyz;
+                }
+                if (code[t] == 2) return fail(BGT_ERR_HIP, "segment graph: chain %d leaves to a path point", ch);
             }
-            emit(t);
+            emit(txyz);
             if (std::abs((int)orig[t]) < 2) end = true;
             code[t] = (int8_t)(code[t] + 1);
             close(end);
         }
     }
+    return BGT_OK;
 }
 
 template <typename T>
@@ -279,8 +297,10 @@ int extract_segments_mask(const uint8_t *mask, int w, int h, int d, bool three_d
 int extract_segments_graph(const SegGraph &g, int pw, int ph, bool three_d, bgt_segments *out)
 {
     std::memset(out, 0, sizeof(*out));
+    (void)pw;
+    (void)ph;
     Flat f;
-    trace_graph(g, pw, ph, three_d, f);
+    BGT_CHECK(trace_graph(g, three_d, f));
     return export_segments(f, three_d, out);
 }
 

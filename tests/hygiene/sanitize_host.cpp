@@ -12,10 +12,9 @@
 //     segments, float bits included, for several (max_segment_length,
 //     smoothing_span), against or2d/or3d_centerline;
 //   - the MEX [y x (z)] columns, against a direct column-major copy;
-//   - the tracer's second pass over the device pre-pass's graph (points,
-//     codes, neighbour ranks and chains of path points, built here on the
-//     host), jumping whole chains and stepping point by point, on the same
-//     masks and on dense noise.
+//   - the tracer's second pass over the device pre-pass's graph (codes,
+//     nodes with their neighbour ranks, chains of path points, built here on
+//     the host), on the same masks and on dense noise.
 // Any sanitizer report aborts the program (-fno-sanitize-recover).
 #include <cstdint>
 #include <cstdio>
@@ -114,16 +113,15 @@ static void check_segments(const std::vector<uint8_t> &m, int w, int h, int d, b
 
 // The device pre-pass's graph (begputhinning_amd/csrc/seggraph.hip) built on
 // the host for a mask whose frame is the mask itself: the set voxels off the
-// frame's border in raster order, their packed coordinates, codes (neighbours
-// counted over the whole mask) and neighbour ranks; and the chains of path
-// points, each walked here from its end of lower rank (the device ranks them
-// by pointer jumping instead).
+// frame's border in raster order with their codes (neighbours counted over
+// the whole mask); the nodes (negative codes) with their packed coordinates
+// and neighbour ranks; and the chains of path points, each walked here from
+// its end of lower rank (the device ranks them by pointer jumping instead).
 struct HostGraph {
-    std::vector<int64_t> idx;
+    std::vector<int64_t> idx, node_xyz, chain_xyz;
     std::vector<int8_t> code;
-    std::vector<int> nbr, chain;
+    std::vector<int> nbr, chain, node_rank, node_nbr;
     std::vector<bgt::SegChain> chains;
-    std::vector<bgt::SegChainPt> pts;
     bgt::SegGraph g;
 };
 
@@ -160,6 +158,15 @@ static void host_graph(const std::vector<uint8_t> &m, int w, int h, int d, bool 
                 if (nn == 0) nn = 1;
                 G.code[r] = (int8_t)(nn == 2 ? 2 : -nn);
             }
+    G.node_rank.clear();
+    G.node_xyz.clear();
+    G.node_nbr.clear();
+    for (int r = 0; r < n; ++r)
+        if (G.code[r] < 0) {
+            G.node_rank.push_back(r);
+            G.node_xyz.push_back(G.idx[r]);
+            G.node_nbr.insert(G.node_nbr.end(), G.nbr.begin() + (size_t)r * nd, G.nbr.begin() + (size_t)(r + 1) * nd);
+        }
     // the (at most two) point neighbours of a path point, in neighbour order
     auto slots = [&](int p, int *s) {
         int k = 0;
@@ -169,7 +176,7 @@ static void host_graph(const std::vector<uint8_t> &m, int w, int h, int d, bool 
     };
     G.chain.assign(n, -1);
     G.chains.clear();
-    G.pts.clear();
+    G.chain_xyz.clear();
     for (int p = 0; p < n; ++p) {
         if (G.code[p] != 2 || G.chain[p] >= 0) continue;
         int s[2];
@@ -178,7 +185,7 @@ static void host_graph(const std::vector<uint8_t> &m, int w, int h, int d, bool 
         for (int v : s) on_path += v >= 0 && G.code[v] == 2;
         if (on_path == 2) continue;  // not an end: reached from one, or on a closed run
         bgt::SegChain c{};
-        c.off = (int)G.pts.size();
+        c.off = (int)G.chain_xyz.size();
         c.e0 = p;
         const int id = (int)G.chains.size();
         int off_chain[2] = {-1, -1}, k = 0;
@@ -187,7 +194,7 @@ static void host_graph(const std::vector<uint8_t> &m, int w, int h, int d, bool 
         int prev = -1, cur = p;
         for (;;) {
             G.chain[cur] = id;
-            G.pts.push_back(bgt::SegChainPt{G.idx[cur], cur, 0});
+            G.chain_xyz.push_back(G.idx[cur]);
             int t[2], next = -1;
             slots(cur, t);
             for (int v : t)
@@ -197,7 +204,7 @@ static void host_graph(const std::vector<uint8_t> &m, int w, int h, int d, bool 
             cur = next;
         }
         c.e1 = cur;
-        c.len = (int)G.pts.size() - c.off;
+        c.len = (int)G.chain_xyz.size() - c.off;
         c.x0 = off_chain[0];
         if (c.len == 1) {
             c.x1 = off_chain[1];
@@ -208,22 +215,25 @@ static void host_graph(const std::vector<uint8_t> &m, int w, int h, int d, bool 
             for (int v : t)
                 if (v >= 0 && G.code[v] != 2) { c.x1 = v; break; }
         }
+        c.x0xyz = c.x0 >= 0 ? G.idx[c.x0] : 0;
+        c.x1xyz = c.x1 >= 0 ? G.idx[c.x1] : 0;
         G.chains.push_back(c);
     }
     G.g = bgt::SegGraph{};
     G.g.npts = n;
     G.g.nd = nd;
-    G.g.idx = G.idx.data();
     G.g.code = G.code.data();
-    G.g.nbr = G.nbr.data();
     G.g.chain = G.chain.data();
+    G.g.nnodes = (int)G.node_rank.size();
+    G.g.node_rank = G.node_rank.data();
+    G.g.node_xyz = G.node_xyz.data();
+    G.g.node_nbr = G.node_nbr.data();
     G.g.nchains = (int)G.chains.size();
     G.g.chains = G.chains.data();
-    G.g.chain_pts = G.pts.data();
+    G.g.chain_xyz = G.chain_xyz.data();
 }
 
-// the host's second pass over the pre-pass graph, jumping chains and stepping
-// point by point, against the oracle's tracer
+// the host's second pass over the pre-pass graph against the oracle's tracer
 static void check_graph(const std::vector<uint8_t> &m, int w, int h, int d, bool three_d, const char *what)
 {
     if (w < 1 || h < 1 || d < 1) return;
@@ -234,15 +244,10 @@ static void check_graph(const std::vector<uint8_t> &m, int w, int h, int d, bool
         or3d_extract(m.data(), w, h, d, &o);
     else
         or2d_extract(m.data(), w, h, &o);
-    for (int pass = 0; pass < 2; ++pass) {
-        bgt::SegGraph g = G.g;
-        if (pass) g.chain = nullptr;  // step by step
-        bgt_segments s{};
-        EXPECT(bgt::extract_segments_graph(g, w, h, three_d, &s) == BGT_OK, "%s: graph trace", what);
-        EXPECT(same_segments(s, o, three_d), "%s %dx%dx%d: graph trace (%s) differs", what, w, h, d,
-               pass ? "step by step" : "chains");
-        std::free(s.x); std::free(s.y); std::free(s.z); std::free(s.seg_start); std::free(s.is_end);
-    }
+    bgt_segments s{};
+    EXPECT(bgt::extract_segments_graph(G.g, w, h, three_d, &s) == BGT_OK, "%s: graph trace", what);
+    EXPECT(same_segments(s, o, three_d), "%s %dx%dx%d: graph trace differs", what, w, h, d);
+    std::free(s.x); std::free(s.y); std::free(s.z); std::free(s.seg_start); std::free(s.is_end);
     or_segments_free(&o);
 }
 
