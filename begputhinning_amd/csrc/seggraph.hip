@@ -20,7 +20,8 @@
 //   seg_count  -- the chunk's point bits, their count, and each workgroup's sum;
 //   seg_scan   -- one workgroup: exclusive sums of the workgroup sums, total;
 //   seg_base   -- each chunk's first rank;
-//   seg_emit   -- per point: packed coordinates, code, neighbour ranks;
+//   seg_place  -- the packed coordinates of the chunk's points;
+// then one thread per point: seg_emit -- its code and neighbour ranks;
 // then one thread per point (or half-edge) for the chains of path points:
 //   seg_link, seg_jump  -- list ranking of the chains by pointer jumping;
 //   seg_chain_flags, seg_scan, seg_chain_base -- chain ids and records in
@@ -34,6 +35,7 @@
 #include <vector>
 
 #include "common.h"
+#include "pack.h"
 #include "seggraph.h"
 
 namespace bgt {
@@ -65,21 +67,23 @@ __global__ __launch_bounds__(256) void seg_count(Frame f, long nchunks, uint64_t
                                                  int *cnt, int *wsum)
 {
     __shared__ int s_sum[4];
+    __shared__ uint32_t s_masks[4][320];
     const long c = (long)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = c < nchunks;
+    const long row = valid ? c / f.cpr : 0;
+    const int x0 = valid ? (int)(c - row * f.cpr) * 64 : 0;
+    const int nx = min(64, f.w - x0);
+    // the chunks of a contiguous skeleton follow each other in memory: the
+    // wave loads them together
+    uint64_t m = pack_runs_wave_u8(f.skel + (size_t)row * f.w + x0, nx, valid, s_masks[threadIdx.x >> 6]);
     int n = 0;
-    if (c < nchunks) {
-        const long row = c / f.cpr;
-        const int x0 = (int)(c - row * f.cpr) * 64;
+    if (valid) {
         const int y = (int)(row % f.h), z = (int)(row / f.h);
-        const uint8_t *p = f.skel + (size_t)row * f.w + x0;
-        const int nx = min(64, f.w - x0);
-        uint64_t m = 0;
         const bool row_ok = y >= 1 && y <= f.ph - 2 && (!f.three_d || (z >= 1 && z <= f.pd - 2));
-        if (row_ok)
-            for (int k = 0; k < nx; ++k) {
-                const int x = x0 + k;
-                if (p[k] && x >= 1 && x <= f.pw - 2) m |= 1ull << k;
-            }
+        if (!row_ok) m = 0;
+        if (x0 == 0) m &= ~1ull;  // x = 0 is on the border
+        const int xe = f.pw - 2 - x0;  // the last x off the border, relative to x0
+        if (xe < 63) m &= xe < 0 ? 0ull : (2ull << xe) - 1ull;
         bits[c] = m;
         n = __popcll(m);
         cnt[c] = n;
@@ -151,41 +155,49 @@ __device__ __forceinline__ int rank_of(const Frame &f, const uint64_t *bits, con
     return base[c] + __popcll(m & ((1ull << k) - 1ull));
 }
 
-// one thread per chunk: every point of the chunk
-__global__ __launch_bounds__(256) void seg_emit(Frame f, long nchunks, const uint64_t *bits,
-                                                const int *base, int64_t *idx, int8_t *code,
-                                                int *nbr)
+// one thread per chunk: the packed coordinates of every point of the chunk
+__global__ __launch_bounds__(256) void seg_place(Frame f, long nchunks, const uint64_t *bits,
+                                                 const int *base, int64_t *idx)
 {
     const long c = (long)blockIdx.x * 256 + threadIdx.x;
     if (c >= nchunks) return;
     uint64_t m = bits[c];
     if (!m) return;
     const long row = c / f.cpr;
-    const int x0 = (int)(c - row * f.cpr) * 64;
-    const int y = (int)(row % f.h), z = (int)(row / f.h);
-    const int nd = f.three_d ? 26 : 8;
+    const int64_t x0 = (c - row * f.cpr) * 64;
+    const int64_t yz = (int64_t)(row % f.h) << 16 | (int64_t)(row / f.h) << 32;
     int r = base[c];
     while (m) {
         const int k = __ffsll((long long)m) - 1;
         m &= m - 1;
-        const int x = x0 + k;
-        idx[r] = (int64_t)x | (int64_t)y << 16 | (int64_t)z << 32;  // packed coordinates
-        int nn = 0, j = 0;
-        // the reference's neighbour order: raster order of the offsets
-        for (int dz = f.three_d ? -1 : 0; dz <= (f.three_d ? 1 : 0); ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    if (!dz && !dy && !dx) continue;
-                    const int qx = x + dx, qy = y + dy, qz = z + dz;
-                    nn += set_at(f, qx, qy, qz);
-                    nbr[(size_t)r * nd + j] =
-                        off_border(f, qx, qy, qz) ? rank_of(f, bits, base, qx, qy, qz) : -1;
-                    ++j;
-                }
-        if (nn == 0) nn = 1;
-        code[r] = (int8_t)(nn == 2 ? 2 : -nn);
-        ++r;
+        idx[r++] = (x0 + k) | yz;  // x | y << 16 | z << 32
     }
+}
+
+// one thread per point: its code and the ranks of its neighbours
+__global__ __launch_bounds__(256) void seg_emit(Frame f, int npts, const uint64_t *bits,
+                                                const int *base, const int64_t *idx, int8_t *code,
+                                                int *nbr)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= npts) return;
+    const uint64_t q = (uint64_t)idx[r];
+    const int x = (int)(q & 0xffffu), y = (int)((q >> 16) & 0xffffu), z = (int)(q >> 32);
+    const int nd = f.three_d ? 26 : 8;
+    int nn = 0, j = 0;
+    // the reference's neighbour order: raster order of the offsets
+    for (int dz = f.three_d ? -1 : 0; dz <= (f.three_d ? 1 : 0); ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (!dz && !dy && !dx) continue;
+                const int qx = x + dx, qy = y + dy, qz = z + dz;
+                nn += set_at(f, qx, qy, qz);
+                nbr[(size_t)r * nd + j] =
+                    off_border(f, qx, qy, qz) ? rank_of(f, bits, base, qx, qy, qz) : -1;
+                ++j;
+            }
+    if (nn == 0) nn = 1;
+    code[r] = (int8_t)(nn == 2 ? 2 : -nn);
 }
 
 // ---- chains of path points -------------------------------------------------
@@ -518,7 +530,8 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     go.rec = (SegChain *)q;
     q += a_rec;
     go.chain_xyz = (int64_t *)q;
-    hipLaunchKernelGGL(seg_emit, dim3((unsigned)nwg), dim3(256), 0, s, f, nchunks, bits, base,
+    hipLaunchKernelGGL(seg_place, dim3((unsigned)nwg), dim3(256), 0, s, f, nchunks, bits, base, d_idx);
+    hipLaunchKernelGGL(seg_emit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, n, bits, base,
                        d_idx, d_code, d_nbr);
     BGT_HIP(hipGetLastError());
 
