@@ -254,6 +254,36 @@ def frame_leg(args, torch, B, synth, dev):
             "b_alg_bytes": int(b_alg)}
 
 
+def trace_leg(args, torch, B, synth, dev):
+    """Latency of the segment tracer over skeletons already in device memory
+    (bgt_extract_segments2d/3d_device: GPU pre-pass, copy of the point graph,
+    host walk): 1024x1024 polyline and tree skeletons, 256^3 and 512^3 vessel
+    trees, each the skeleton of a seeded input."""
+    cases = [("2d_1024x1024_polylines", [synth.polylines(1024, 1024, synth.item_seed(4, 10000 + k))
+                                         for k in range(4)], 40),
+             ("2d_1024x1024_tree", [synth.workload2(k) for k in range(4)], 40),
+             ("3d_256^3_tree", [synth.workload3(k) for k in range(2)], 12),
+             ("3d_512^3_tree", [synth.tree3d(512, 512, 512, synth.item_seed(5, 90000))], 6)]
+    st = torch.cuda.Stream()
+    out = {"path": "bgt_extract_segments2d/3d_device on the skeleton in device memory"}
+    for key, inputs, reps in cases:
+        thin = B.thin2d_device if inputs[0].ndim == 2 else B.thin3d_device
+        skels = [thin(torch.from_numpy(a).to(dev))[0] for a in inputs]
+        torch.cuda.synchronize()
+        for sk in skels:
+            r = B.extract_segments_device(sk, stream=st)
+        ts = []
+        for k in range(reps):
+            t0 = time.perf_counter()
+            r = B.extract_segments_device(skels[k % len(skels)], stream=st)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        out[key] = {"p50": round(float(np.percentile(ts, 50)), 3), "p99": round(float(np.percentile(ts, 99)), 3),
+                    "calls": reps, "points": int(len(r["x"])), "segments": int(len(r["start"]) - 1)}
+        del skels
+    log("segment tracing p50: " + ", ".join(f"{k} {v['p50']} ms" for k, v in out.items() if isinstance(v, dict)))
+    return out
+
+
 def host_cores():
     """Threads the host gives this process: the affinity mask, capped by
     OMP_NUM_THREADS where the environment sets the process's CPU share."""
@@ -335,6 +365,8 @@ def main():
     ap.add_argument("--only-frames", action="store_true",
                     help="run only the 1024x1024 per-image latency leg (for counter runs of the "
                          "per-image path) and print its figures")
+    ap.add_argument("--only-trace", action="store_true",
+                    help="run only the segment-tracing latency leg and print its figures")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, to scale hardware byte counters)")
@@ -361,6 +393,9 @@ def main():
         return
     if args.only_serving:
         print(json.dumps({"serving": serve_leg(args, torch, B, synth, dev)}))
+        return
+    if args.only_trace:
+        print(json.dumps({"trace_ms": trace_leg(args, torch, B, synth, dev)}))
         return
     t0 = time.time()
     items, n2 = gen_items()
@@ -436,7 +471,7 @@ def main():
     log(f"kernels: {DOMINANT} {ms2d:.2f} ms / {l2d} launches, 3D iterations {ms3d:.2f} ms / "
         f"{l3d}; timed step {timed_step_ms:.2f} ms")
 
-    per_image = latency = centerline = None
+    per_image = latency = centerline = tracing = None
     outs_pi = None
     if not args.only_batch:
         # ---- per-image drop-in path on the same items -----------------------
@@ -472,6 +507,7 @@ def main():
         log(f"CenterlineExtraction: 1024^2 p50 {cl['2d_1024x1024']['p50']} ms, "
             f"256^3 p50 {cl['3d_256^3']['p50']} ms")
         centerline = cl
+        tracing = trace_leg(args, torch, B, synth, dev)
 
     serving = None
     if args.serve_qps:
@@ -531,6 +567,7 @@ def main():
         "per_image": per_image,
         "latency_1024x1024_ms": latency,
         "centerline_ms": centerline,
+        "trace_ms": tracing,
         "serving": serving,
         "bit_exact": {"vs_oracle": exact, "vs_committed_hashes": gold_ok,
                       "batched_equals_per_image": same_pi},
