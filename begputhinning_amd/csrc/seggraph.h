@@ -49,5 +49,9 @@ struct SegGraph {
 // left corner of a zero-padded pw x ph x pd frame (d = pd = 1 in 2D).
 int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int ph, int pd,
                          bool three_d, hipStream_t s, SegGraph &g);
+// The same for a host array of any pixel type (nonzero = set) as a frame of
+// its own size: the array is copied to the device first.
+int segment_graph_host(int dtype, const void *data, int w, int h, int d, bool three_d, hipStream_t s,
+                       SegGraph &g);
 
 }  // namespace bgt

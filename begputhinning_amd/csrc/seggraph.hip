@@ -31,6 +31,7 @@
 // point.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -426,6 +427,14 @@ __global__ __launch_bounds__(256) void seg_chain_scatter(int npts, const int64_t
     o.chain_xyz[c.coff[s0] + c.pos[p]] = idx[p];
 }
 
+// the tracer's foreground of a pixel array: nonzero -> 1
+template <typename T>
+__global__ __launch_bounds__(256) void nonzero_u8(const T *__restrict__ in, uint8_t *__restrict__ m, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        m[i] = in[i] != (T)0;
+}
+
 // Grow-only pinned host staging of one thread's pre-pass results (device to
 // host copies into pageable memory go through the runtime's own staging),
 // released at thread exit.
@@ -460,6 +469,39 @@ PinnedStage &thread_stage()
 }
 
 }  // namespace
+
+int segment_graph_host(int dtype, const void *data, int w, int h, int d, bool three_d, hipStream_t s,
+                       SegGraph &g)
+{
+    g = SegGraph{};
+    g.nd = three_d ? 26 : 8;
+    const size_t es = dtype_size(dtype);
+    if (!data || !es) return fail(BGT_ERR_INVALID, "bad pixel array or type %d", dtype);
+    if (w <= 0 || h <= 0 || d <= 0) return fail(BGT_ERR_INVALID, "bad size");
+    const size_t n = (size_t)w * h * d;
+    const size_t raw = round_up(n * es, 256);
+    Workspace &wi = thread_workspace(4);
+    BGT_CHECK(workspace_reserve(wi, raw + (es > 1 ? round_up(n, 256) : 0)));
+    char *d_raw = (char *)wi.ptr;
+    BGT_HIP(hipMemcpyAsync(d_raw, data, n * es, hipMemcpyHostToDevice, s));
+    const uint8_t *mask = (const uint8_t *)d_raw;  // one-byte pixels: nonzero bytes are the foreground
+    if (es > 1) {
+        uint8_t *m = (uint8_t *)(d_raw + raw);
+        const unsigned grid = (unsigned)std::min<long>(((long)n + 255) / 256, 65536L);
+        switch (dtype) {
+        case BGT_U16: hipLaunchKernelGGL(nonzero_u8<uint16_t>, dim3(grid), dim3(256), 0, s, (const uint16_t *)d_raw, m, (long)n); break;
+        case BGT_I16: hipLaunchKernelGGL(nonzero_u8<int16_t>, dim3(grid), dim3(256), 0, s, (const int16_t *)d_raw, m, (long)n); break;
+        case BGT_U32: hipLaunchKernelGGL(nonzero_u8<uint32_t>, dim3(grid), dim3(256), 0, s, (const uint32_t *)d_raw, m, (long)n); break;
+        case BGT_I32: hipLaunchKernelGGL(nonzero_u8<int32_t>, dim3(grid), dim3(256), 0, s, (const int32_t *)d_raw, m, (long)n); break;
+        case BGT_F32: hipLaunchKernelGGL(nonzero_u8<float>, dim3(grid), dim3(256), 0, s, (const float *)d_raw, m, (long)n); break;
+        case BGT_F64: hipLaunchKernelGGL(nonzero_u8<double>, dim3(grid), dim3(256), 0, s, (const double *)d_raw, m, (long)n); break;
+        default: return fail(BGT_ERR_INVALID, "unknown pixel type %d", dtype);
+        }
+        BGT_HIP(hipGetLastError());
+        mask = m;
+    }
+    return segment_graph_device(mask, w, h, d, w, h, d, three_d, s, g);
+}
 
 int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int ph, int pd,
                          bool three_d, hipStream_t s, SegGraph &g)
