@@ -115,3 +115,15 @@ def test_large_host_arrays_traced_on_the_device(dtype):
                 assert got[k] is None
                 continue
             assert np.array_equal(ref[k], got[k]), f"{dtype} {m.shape}: {k} differs from the oracle"
+
+
+def test_padded_frame_beyond_int16_is_rejected():
+    """Device tracing refuses a padded frame with a side above 32767 (the
+    reference's short coordinates) and takes one of exactly 32767."""
+    import begputhinning_amd as B
+    t = torch.zeros((2, 8), dtype=torch.uint8, device="cuda")
+    t[1, 3] = 1
+    with pytest.raises(B.BGTError, match="int16"):
+        B.extract_segments_device(t, (2, 32768))
+    r = B.extract_segments_device(t, (2, 32767))
+    assert len(r["start"]) == 1  # the only point lies on the frame's border

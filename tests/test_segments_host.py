@@ -142,3 +142,20 @@ def test_padding_rule():
         img[h - 1, 10:20] = 1
         r = O.centerline2d(img, 0, 0)
         assert (len(r["idx"]) > 1) == traced, h
+
+
+def test_coordinates_beyond_int16_are_rejected():
+    """Segment points are int16 (the reference's point2D/point3D hold shorts,
+    2D/Structs.h:15-22): tracing a side longer than 32767 is refused with
+    BGT_ERR_UNSUPPORTED rather than wrapping, and 32767 itself is fine.
+    CenterlineExtraction refuses the same once its padded frame exceeds it."""
+    import begputhinning_amd as B
+    with pytest.raises(B.BGTError, match="int16"):
+        B.extract_segments_flat(np.zeros((1, 32768), np.uint8))
+    with pytest.raises(B.BGTError, match="int16"):
+        B.extract_segments_flat(np.zeros((32768, 1, 1), np.uint8))
+    r = B.extract_segments_flat(np.zeros((3, 32767), np.uint8))
+    assert len(r["start"]) == 1 and len(r["x"]) == 0
+    # 32740 pixels wide pads to 32768: past int16
+    with pytest.raises(B.BGTError, match="int16"):
+        B.CenterlineExtraction(np.zeros((4, 32740), np.uint8), 10, 3)
