@@ -24,10 +24,12 @@ struct SegChain {
 // points ranked in raster order and coordinates packed as x | y << 16 |
 // z << 32 (sides are at most 32767):
 //   - per point, the tracer's code (-1 isolated, 2 path, -n end or junction
-//     with n neighbours) and its chain (-1: none);
+//     with n neighbours);
 //   - the nodes (points of negative code) in rank order: rank, coordinates,
-//     and for each of the nd neighbours in the reference's order the rank of
-//     that neighbour if it is a point itself, else -1;
+//     and their neighbours that are points, in the reference's neighbour
+//     order, each as rank | slot << 32 | (chain + 1) << 37 (slot: its place
+//     among the nd neighbours; chain: the chain of a path point, else -1),
+//     node k's at node_ent[node_off[k] .. node_off[k + 1]);
 //   - the chains, and their points' coordinates in order.
 // The arrays are views of a per-thread host buffer, valid until the thread's
 // next segment_graph_device call.
@@ -35,11 +37,11 @@ struct SegGraph {
     int npts = 0;
     int nd = 8;
     const int8_t *code = nullptr;
-    const int *chain = nullptr;
     int nnodes = 0;
     const int *node_rank = nullptr;
     const int64_t *node_xyz = nullptr;
-    const int *node_nbr = nullptr;  // nnodes x nd
+    const int *node_off = nullptr;      // nnodes + 1
+    const int64_t *node_ent = nullptr;  // node_off[nnodes]
     int nchains = 0;
     const SegChain *chains = nullptr;
     const int64_t *chain_xyz = nullptr;

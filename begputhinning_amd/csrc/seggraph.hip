@@ -305,28 +305,32 @@ struct ChainTmp {
     int *start;  // [npts] the chain's first end (e0) of a chain point, else -1
     int *pos;    // [npts] position from e0
     int *len;    // [npts] chain length, at e0 only (else 0)
-    int *wch, *wlen, *wnode;  // [nwg] workgroup sums of chain starts, lengths, nodes
+    int *nent;   // [npts] a node's neighbours that are points (else 0)
+    int *wch, *wlen, *wnode, *went;  // [nwg] workgroup sums of chain starts, lengths, nodes, node entries
     int *cid, *coff;  // [npts] chain id and first position, at e0 only
+    int *eoff;   // [npts] a node's first entry
 };
 
-// What the host's second pass reads (SegGraph), each array sized for npts.
+// What the host's second pass reads (SegGraph), each array sized for npts
+// (node_ent for npts x nd).
 struct GraphOut {
-    int *chain;
     int *node_rank;
     int64_t *node_xyz;
-    int *node_nbr;
+    int *node_off;
+    int64_t *node_ent;
     SegChain *rec;
     int64_t *chain_xyz;
 };
 
-// per path point: its chain's e0 and its position; e0 also its length;
-// workgroup sums of chain starts, chain lengths and nodes
-__global__ __launch_bounds__(256) void seg_chain_flags(int npts, const int8_t *code, Jump j,
-                                                       ChainTmp c)
+// per path point: its chain's e0 and its position; e0 also its length; per
+// node: its neighbours that are points; workgroup sums of chain starts, chain
+// lengths, nodes and node entries
+__global__ __launch_bounds__(256) void seg_chain_flags(int npts, int nd, const int8_t *code, const int *nbr,
+                                                       Jump j, ChainTmp c)
 {
-    __shared__ int s_a[4], s_b[4], s_c[4];
+    __shared__ int s_a[4], s_b[4], s_c[4], s_d[4];
     const int p = blockIdx.x * 256 + threadIdx.x;
-    int st = 0, ln = 0, nodes = 0;
+    int st = 0, ln = 0, nodes = 0, ne = 0;
     if (p < npts) {
         int s0 = -1, ps = 0;
         const int8_t cd = code[p];
@@ -340,34 +344,39 @@ __global__ __launch_bounds__(256) void seg_chain_flags(int npts, const int8_t *c
             }
         }
         nodes = cd < 0;
+        if (nodes)
+            for (int k = 0; k < nd; ++k) ne += nbr[(size_t)p * nd + k] >= 0;
         c.start[p] = s0;
         c.pos[p] = ps;
         c.len[p] = ln;
+        c.nent[p] = ne;
     }
-    int x = st, y = ln, z = nodes;
+    int x = st, y = ln, z = nodes, w = ne;
     for (int o = 32; o > 0; o >>= 1) {
         x += __shfl_xor(x, o);
         y += __shfl_xor(y, o);
         z += __shfl_xor(z, o);
+        w += __shfl_xor(w, o);
     }
     if ((threadIdx.x & 63) == 0) {
         s_a[threadIdx.x >> 6] = x;
         s_b[threadIdx.x >> 6] = y;
         s_c[threadIdx.x >> 6] = z;
+        s_d[threadIdx.x >> 6] = w;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         c.wch[blockIdx.x] = s_a[0] + s_a[1] + s_a[2] + s_a[3];
         c.wlen[blockIdx.x] = s_b[0] + s_b[1] + s_b[2] + s_b[3];
         c.wnode[blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+        c.went[blockIdx.x] = s_d[0] + s_d[1] + s_d[2] + s_d[3];
     }
 }
 
 // chain ids and first positions in the order of e0 and the chain records;
-// the nodes in rank order with their coordinates and neighbour ranks
-__global__ __launch_bounds__(256) void seg_chain_base(int npts, int nd, const int64_t *idx,
-                                                      const int8_t *code, const int *nbr, Jump j,
-                                                      ChainTmp c, GraphOut o)
+// the nodes in rank order with their coordinates and first entries
+__global__ __launch_bounds__(256) void seg_chain_base(int npts, const int64_t *idx, const int8_t *code,
+                                                      Jump j, ChainTmp c, GraphOut o)
 {
     __shared__ int s_w[4];
     const int p = blockIdx.x * 256 + threadIdx.x;
@@ -377,10 +386,12 @@ __global__ __launch_bounds__(256) void seg_chain_base(int npts, int nd, const in
     const int id = c.wch[blockIdx.x] + wg_exclusive(st ? 1 : 0, s_w);
     const int off = c.wlen[blockIdx.x] + wg_exclusive(ln, s_w);
     const int nid = c.wnode[blockIdx.x] + wg_exclusive(node ? 1 : 0, s_w);
+    const int eo = c.went[blockIdx.x] + wg_exclusive(node ? c.nent[p] : 0, s_w);
     if (node) {
         o.node_rank[nid] = p;
         o.node_xyz[nid] = idx[p];
-        for (int k = 0; k < nd; ++k) o.node_nbr[(size_t)nid * nd + k] = nbr[(size_t)p * nd + k];
+        o.node_off[nid] = eo;
+        c.eoff[p] = eo;
     }
     if (!st) return;
     c.cid[p] = id;
@@ -411,20 +422,26 @@ __global__ __launch_bounds__(256) void seg_chain_base(int npts, int nd, const in
     o.rec[id] = r;
 }
 
-// every chain point: its chain id, and its coordinates at its place in the
-// chain's run
-__global__ __launch_bounds__(256) void seg_chain_scatter(int npts, const int64_t *idx, ChainTmp c,
+// every chain point: its coordinates at its place in the chain's run; every
+// node: its entries (neighbours that are points, with their chains)
+__global__ __launch_bounds__(256) void seg_chain_scatter(int npts, int nd, const int64_t *idx,
+                                                         const int8_t *code, const int *nbr, ChainTmp c,
                                                          GraphOut o)
 {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= npts) return;
     const int s0 = c.start[p];
-    if (s0 < 0) {
-        o.chain[p] = -1;
-        return;
+    if (s0 >= 0) o.chain_xyz[c.coff[s0] + c.pos[p]] = idx[p];
+    if (code[p] < 0) {
+        int e = c.eoff[p];
+        for (int k = 0; k < nd; ++k) {
+            const int r = nbr[(size_t)p * nd + k];
+            if (r < 0) continue;
+            const int rs = c.start[r];
+            const int64_t ch = rs >= 0 ? (int64_t)c.cid[rs] : -1;
+            o.node_ent[e++] = (int64_t)(uint32_t)r | (int64_t)k << 32 | (ch + 1) << 37;
+        }
     }
-    o.chain[p] = c.cid[s0];
-    o.chain_xyz[c.coff[s0] + c.pos[p]] = idx[p];
 }
 
 // the tracer's foreground of a pixel array: nonzero -> 1
@@ -547,43 +564,44 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     const int n = g.npts;
     // the pre-pass's per-point arrays (device only)
     const size_t a_idx = round_up(sizeof(int64_t) * n, 256), a_code = round_up((size_t)n, 256),
-                 a_nbr = round_up(sizeof(int) * (size_t)n * g.nd, 256), a_int = round_up(sizeof(int) * (size_t)n, 256);
-    // what the host reads, each region sized for n points (only the used part
-    // of each is copied)
-    const size_t a_nxyz = round_up(sizeof(int64_t) * n, 256), a_rec = round_up(sizeof(SegChain) * (size_t)n, 256);
+                 a_nbr = round_up(sizeof(int) * (size_t)n * g.nd, 256), a_int = round_up(sizeof(int) * ((size_t)n + 1), 256);
+    // what the host reads, each region sized for the most there can be (only
+    // the used part of each is copied)
+    const size_t a_nxyz = round_up(sizeof(int64_t) * n, 256), a_ent = round_up(sizeof(int64_t) * (size_t)n * g.nd, 256),
+                 a_rec = round_up(sizeof(SegChain) * (size_t)n, 256);
     Workspace &wo = thread_workspace(6);
-    BGT_CHECK(workspace_reserve(wo, a_idx + a_code + a_nbr + 3 * a_int + 2 * a_nxyz + a_nbr + a_rec));
+    BGT_CHECK(workspace_reserve(wo, a_idx + a_nbr + a_code + 2 * a_int + 2 * a_nxyz + a_ent + a_rec));
     char *q = (char *)wo.ptr;
     int64_t *d_idx = (int64_t *)q;
     q += a_idx;
     int *d_nbr = (int *)q;
     q += a_nbr;
-    int8_t *d_code = (int8_t *)q;  // the host copy starts here: code, chain
+    int8_t *d_code = (int8_t *)q;
     q += a_code;
     GraphOut go;
-    go.chain = (int *)q;
-    q += a_int;
     go.node_rank = (int *)q;
+    q += a_int;
+    go.node_off = (int *)q;
     q += a_int;
     go.node_xyz = (int64_t *)q;
     q += a_nxyz;
-    go.node_nbr = (int *)q;
-    q += a_nbr;
-    go.rec = (SegChain *)q;
-    q += a_rec;
     go.chain_xyz = (int64_t *)q;
+    q += a_nxyz;
+    go.node_ent = (int64_t *)q;
+    q += a_ent;
+    go.rec = (SegChain *)q;
     hipLaunchKernelGGL(seg_place, dim3((unsigned)nwg), dim3(256), 0, s, f, nchunks, bits, base, d_idx);
     hipLaunchKernelGGL(seg_emit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, n, bits, base,
                        d_idx, d_code, d_nbr);
     BGT_HIP(hipGetLastError());
 
-    // chains: link, ceil(log5 n) jump launches, chain starts and lengths and
-    // nodes, their prefix sums, records, nodes and runs
+    // chains: link, ceil(log5 n) jump launches; chain starts and lengths,
+    // nodes and their entries, their prefix sums; records, nodes, runs
     const int pwg = (n + 255) / 256, hwg = (2 * n + 255) / 256;
     const size_t half = round_up(sizeof(int) * 2 * (size_t)n, 256), one = round_up(sizeof(int) * (size_t)n, 256),
                  wsz = round_up(sizeof(int) * (size_t)pwg, 256);
     Workspace &wt = thread_workspace(7);
-    BGT_CHECK(workspace_reserve(wt, 8 * half + 5 * one + 3 * wsz + 256));
+    BGT_CHECK(workspace_reserve(wt, 8 * half + 7 * one + 4 * wsz + 256));
     char *t = (char *)wt.ptr;
     Jump jb[2];
     for (Jump &j : jb) {
@@ -596,50 +614,58 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     ct.start = (int *)t; t += one;
     ct.pos = (int *)t; t += one;
     ct.len = (int *)t; t += one;
+    ct.nent = (int *)t; t += one;
     ct.cid = (int *)t; t += one;
     ct.coff = (int *)t; t += one;
+    ct.eoff = (int *)t; t += one;
     ct.wch = (int *)t; t += wsz;
     ct.wlen = (int *)t; t += wsz;
     ct.wnode = (int *)t; t += wsz;
-    long *d_tot = (long *)t;  // [0] chains, [1] chain points, [2] nodes
+    ct.went = (int *)t; t += wsz;
+    long *d_tot = (long *)t;  // [0] chains, [1] chain points, [2] nodes, [3] node entries
     hipLaunchKernelGGL(seg_link, dim3((unsigned)pwg), dim3(256), 0, s, n, g.nd, d_code, d_nbr, jb[0]);
     int cur = 0;
     for (long reach = 1; reach < n; reach *= kHops + 1, cur ^= 1)
         hipLaunchKernelGGL(seg_jump, dim3((unsigned)hwg), dim3(256), 0, s, 2 * n, jb[cur], jb[cur ^ 1]);
-    hipLaunchKernelGGL(seg_chain_flags, dim3((unsigned)pwg), dim3(256), 0, s, n, d_code, jb[cur], ct);
+    hipLaunchKernelGGL(seg_chain_flags, dim3((unsigned)pwg), dim3(256), 0, s, n, g.nd, d_code, d_nbr, jb[cur], ct);
     hipLaunchKernelGGL(seg_scan, dim3(1), dim3(1024), 0, s, ct.wch, pwg, d_tot);
     hipLaunchKernelGGL(seg_scan, dim3(1), dim3(1024), 0, s, ct.wlen, pwg, d_tot + 1);
     hipLaunchKernelGGL(seg_scan, dim3(1), dim3(1024), 0, s, ct.wnode, pwg, d_tot + 2);
-    hipLaunchKernelGGL(seg_chain_base, dim3((unsigned)pwg), dim3(256), 0, s, n, g.nd, d_idx, d_code, d_nbr,
-                       jb[cur], ct, go);
-    hipLaunchKernelGGL(seg_chain_scatter, dim3((unsigned)pwg), dim3(256), 0, s, n, d_idx, ct, go);
+    hipLaunchKernelGGL(seg_scan, dim3(1), dim3(1024), 0, s, ct.went, pwg, d_tot + 3);
+    hipLaunchKernelGGL(seg_chain_base, dim3((unsigned)pwg), dim3(256), 0, s, n, d_idx, d_code, jb[cur], ct, go);
+    hipLaunchKernelGGL(seg_chain_scatter, dim3((unsigned)pwg), dim3(256), 0, s, n, g.nd, d_idx, d_code, d_nbr,
+                       ct, go);
     BGT_HIP(hipGetLastError());
-    BGT_HIP(hipMemcpyAsync(st.ptr, d_tot, 3 * sizeof(long), hipMemcpyDeviceToHost, s));
+    BGT_HIP(hipMemcpyAsync(st.ptr, d_tot, 4 * sizeof(long), hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
-    long tot[3];
+    long tot[4];
     std::memcpy(tot, st.ptr, sizeof(tot));
-    if (tot[0] < 0 || tot[0] > n || tot[1] < tot[0] || tot[1] > n || tot[2] < 0 || tot[2] > n)
-        return fail(BGT_ERR_HIP, "segment pre-pass: %ld chains of %ld points, %ld nodes among %d points",
-                    tot[0], tot[1], tot[2], n);
-    // one pinned buffer: code and chain, the nodes' ranks, coordinates and
-    // neighbours, the chain records, the chains' coordinates
+    if (tot[0] < 0 || tot[0] > n || tot[1] < tot[0] || tot[1] > n || tot[2] < 0 || tot[2] > n || tot[3] < 0 ||
+        tot[3] > (long)n * g.nd)
+        return fail(BGT_ERR_HIP, "segment pre-pass: %ld chains of %ld points, %ld nodes of %ld entries among %d points",
+                    tot[0], tot[1], tot[2], tot[3], n);
+    // one pinned buffer: the codes, the nodes' ranks, first entries (and the
+    // total after the last), coordinates and entries, the chain records, the
+    // chains' coordinates
     const size_t nn = (size_t)tot[2];
     struct Part {
         const void *src;
         size_t bytes;
-    } parts[] = {{d_code, a_code + a_int},  // code and chain lie one after the other
+    } parts[] = {{d_code, (size_t)n},
                  {go.node_rank, sizeof(int) * nn},
+                 {go.node_off, sizeof(int) * nn},
                  {go.node_xyz, sizeof(int64_t) * nn},
-                 {go.node_nbr, sizeof(int) * nn * g.nd},
+                 {go.node_ent, sizeof(int64_t) * (size_t)tot[3]},
                  {go.rec, sizeof(SegChain) * (size_t)tot[0]},
                  {go.chain_xyz, sizeof(int64_t) * (size_t)tot[1]}};
-    size_t at[6], bytes = 0;
-    for (int k = 0; k < 6; ++k) {
+    constexpr int kParts = 7;
+    size_t at[kParts], bytes = 0;
+    for (int k = 0; k < kParts; ++k) {
         at[k] = bytes;
-        bytes += round_up(parts[k].bytes, 16);
+        bytes += round_up(parts[k].bytes + (k == 2 ? sizeof(int) : 0), 16);  // room for node_off[nn]
     }
     BGT_CHECK(st.reserve(bytes));
-    for (int k = 0; k < 6; ++k)
+    for (int k = 0; k < kParts; ++k)
         if (parts[k].bytes)
             BGT_HIP(hipMemcpyAsync(st.ptr + at[k], parts[k].src, parts[k].bytes, hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
@@ -649,16 +675,18 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     thread_local std::vector<char> hv;
     if (hv.size() < bytes) hv.resize(bytes);
     std::memcpy(hv.data(), st.ptr, bytes);
-    const char *hq = hv.data();
+    char *hq = hv.data();
+    const int total_ent = (int)tot[3];
+    std::memcpy(hq + at[2] + sizeof(int) * nn, &total_ent, sizeof(int));  // node_off[nn]
     g.code = (const int8_t *)(hq + at[0]);
-    g.chain = (const int *)(hq + at[0] + a_code);
     g.nnodes = (int)nn;
     g.node_rank = (const int *)(hq + at[1]);
-    g.node_xyz = (const int64_t *)(hq + at[2]);
-    g.node_nbr = (const int *)(hq + at[3]);
+    g.node_off = (const int *)(hq + at[2]);
+    g.node_xyz = (const int64_t *)(hq + at[3]);
+    g.node_ent = (const int64_t *)(hq + at[4]);
     g.nchains = (int)tot[0];
-    g.chains = (const SegChain *)(hq + at[4]);
-    g.chain_xyz = (const int64_t *)(hq + at[5]);
+    g.chains = (const SegChain *)(hq + at[5]);
+    g.chain_xyz = (const int64_t *)(hq + at[6]);
     return BGT_OK;
 }
 

@@ -139,37 +139,59 @@ void trace(const uint8_t *mask, int w, int h, int d, bool three_d, Flat &out)
 // point of it is looked at again.  A path point that is not such an end can
 // only be reached by stepping through a chain, which never happens, and is
 // reported as an inconsistent graph.
-int trace_graph(const SegGraph &g, bool three_d, Flat &out)
+//
+// A node's neighbours that are points come as one list (node_ent), so the
+// scan for the next live neighbour skips the neighbours that are not points
+// without looking at them.  The segments are written straight into the
+// output arrays, allocated for the most the walk can emit: a node with code
+// -n starts at most n segments, each of two points besides the chain it
+// passes through, and every chain is passed through at most once, so there
+// are at most S = sum over nodes of n segments and 2 S + npts points.
+template <bool D3>
+int trace_graph(const SegGraph &g, bgt_segments *out)
 {
-    const int nd = g.nd;
     // offsets of the nd neighbours in the packed coordinates, in the
     // reference's order (a point's neighbours lie inside the frame, so no
     // field borrows from another)
     int64_t delta[26];
     {
         int j = 0;
-        for (int dz = three_d ? -1 : 0; dz <= (three_d ? 1 : 0); ++dz)
+        for (int dz = D3 ? -1 : 0; dz <= (D3 ? 1 : 0); ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx)
                     if (dz || dy || dx) delta[j++] = (int64_t)dx + (int64_t)dy * 65536 + (int64_t)dz * 4294967296LL;
     }
+    const int8_t *orig = g.code;
+    size_t max_seg = 0;
+    for (int k = 0; k < g.nnodes; ++k) max_seg += (size_t)-(int)orig[g.node_rank[k]];
+    const size_t max_pts = 2 * max_seg + (size_t)g.npts;
+    if (max_pts > 0xffffffffu) return fail(BGT_ERR_UNSUPPORTED, "segment list beyond 2^32 points");
+    int16_t *ox = (int16_t *)std::malloc(sizeof(int16_t) * (max_pts + 1));
+    int16_t *oy = (int16_t *)std::malloc(sizeof(int16_t) * (max_pts + 1));
+    int16_t *oz = D3 ? (int16_t *)std::malloc(sizeof(int16_t) * (max_pts + 1)) : nullptr;
+    uint32_t *ost = (uint32_t *)std::malloc(sizeof(uint32_t) * (max_seg + 1));
+    uint8_t *oend = (uint8_t *)std::malloc(max_seg + 1);
+    out->x = ox;
+    out->y = oy;
+    out->z = oz;
+    out->seg_start = ost;
+    out->is_end = oend;
+    if (!ox || !oy || (D3 && !oz) || !ost || !oend) return fail(BGT_ERR_NOMEM, "segment list allocation failed");
     // this thread's working copy of the codes, kept between calls
     thread_local std::vector<int8_t> code;
     code.assign(g.code, g.code + g.npts);
-    const int8_t *orig = g.code;
-    // a segment list holds each point about once: reserve for that
-    out.x.reserve((size_t)g.npts + 16);
-    out.y.reserve((size_t)g.npts + 16);
-    if (three_d) out.z.reserve((size_t)g.npts + 16);
+    size_t np = 0, ns = 0;
+    ost[0] = 0;
     auto emit = [&](int64_t v) {
         const uint64_t q = (uint64_t)v;  // x | y << 16 | z << 32
-        out.x.push_back((int16_t)(q & 0xffffu));
-        out.y.push_back((int16_t)((q >> 16) & 0xffffu));
-        if (three_d) out.z.push_back((int16_t)(q >> 32));
+        ox[np] = (int16_t)(q & 0xffffu);
+        oy[np] = (int16_t)((q >> 16) & 0xffffu);
+        if (D3) oz[np] = (int16_t)(q >> 32);
+        ++np;
     };
     auto close = [&](bool end) {
-        out.is_end.push_back(end ? 1 : 0);
-        out.start.push_back((uint32_t)out.x.size());
+        oend[ns] = end ? 1 : 0;
+        ost[++ns] = (uint32_t)np;
     };
     auto live = [&](int r) { return r >= 0 && code[r] != 0; };
     for (int k = 0; k < g.nnodes; ++k) {
@@ -177,19 +199,23 @@ int trace_graph(const SegGraph &g, bool three_d, Flat &out)
         const int8_t nodes = code[idx];
         if (nodes >= 0) continue;
         code[idx] = 0;
-        const int *nb = &g.node_nbr[(size_t)k * nd];
+        const int64_t *ent = g.node_ent + g.node_off[k];
+        const int ne = g.node_off[k + 1] - g.node_off[k];
         const int64_t here = g.node_xyz[k];
-        int ni = -1;
+        int ei = -1;
         for (int8_t si = 0; si > nodes; --si) {
             bool end = std::abs((int)orig[idx]) < 2;
             emit(here);
-            ++ni;
-            while (ni < nd && !live(nb[ni])) ++ni;
-            if (ni >= nd) { close(end); continue; }
-            int t = nb[ni];
-            int64_t txyz = here + delta[ni];
+            ++ei;
+            while (ei < ne && !code[(uint32_t)ent[ei]]) ++ei;
+            if (ei >= ne) {
+                close(end);
+                continue;
+            }
+            int t = (int)(uint32_t)ent[ei];
+            int64_t txyz = here + delta[(ent[ei] >> 32) & 31];
             if (code[t] == 2) {
-                const int ch = g.chain[t];
+                const int ch = (int)(ent[ei] >> 37) - 1;
                 if (ch < 0 || ch >= g.nchains)
                     return fail(BGT_ERR_HIP, "segment graph: path point %d on no chain", t);
                 const SegChain c = g.chains[ch];
@@ -235,6 +261,8 @@ int trace_graph(const SegGraph &g, bool three_d, Flat &out)
             close(end);
         }
     }
+    out->nseg = (int)ns;
+    out->npts = (int)np;
     return BGT_OK;
 }
 
@@ -299,9 +327,16 @@ int extract_segments_graph(const SegGraph &g, int pw, int ph, bool three_d, bgt_
     std::memset(out, 0, sizeof(*out));
     (void)pw;
     (void)ph;
-    Flat f;
-    BGT_CHECK(trace_graph(g, three_d, f));
-    return export_segments(f, three_d, out);
+    const int r = three_d ? trace_graph<true>(g, out) : trace_graph<false>(g, out);
+    if (r != BGT_OK) {
+        std::free(out->x);
+        std::free(out->y);
+        std::free(out->z);
+        std::free(out->seg_start);
+        std::free(out->is_end);
+        std::memset(out, 0, sizeof(*out));
+    }
+    return r;
 }
 
 int extract_segments(int dtype, const void *data, int w, int h, int d, bool three_d,

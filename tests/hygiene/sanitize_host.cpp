@@ -114,13 +114,14 @@ static void check_segments(const std::vector<uint8_t> &m, int w, int h, int d, b
 // The device pre-pass's graph (begputhinning_amd/csrc/seggraph.hip) built on
 // the host for a mask whose frame is the mask itself: the set voxels off the
 // frame's border in raster order with their codes (neighbours counted over
-// the whole mask); the nodes (negative codes) with their packed coordinates
-// and neighbour ranks; and the chains of path points, each walked here from
-// its end of lower rank (the device ranks them by pointer jumping instead).
+// the whole mask); the chains of path points, each walked here from its end
+// of lower rank (the device ranks them by pointer jumping instead); and the
+// nodes (negative codes) with their packed coordinates and their neighbours
+// that are points (rank, slot, chain + 1).
 struct HostGraph {
-    std::vector<int64_t> idx, node_xyz, chain_xyz;
+    std::vector<int64_t> idx, node_xyz, chain_xyz, node_ent;
     std::vector<int8_t> code;
-    std::vector<int> nbr, chain, node_rank, node_nbr;
+    std::vector<int> nbr, chain, node_rank, node_off;
     std::vector<bgt::SegChain> chains;
     bgt::SegGraph g;
 };
@@ -158,15 +159,6 @@ static void host_graph(const std::vector<uint8_t> &m, int w, int h, int d, bool 
                 if (nn == 0) nn = 1;
                 G.code[r] = (int8_t)(nn == 2 ? 2 : -nn);
             }
-    G.node_rank.clear();
-    G.node_xyz.clear();
-    G.node_nbr.clear();
-    for (int r = 0; r < n; ++r)
-        if (G.code[r] < 0) {
-            G.node_rank.push_back(r);
-            G.node_xyz.push_back(G.idx[r]);
-            G.node_nbr.insert(G.node_nbr.end(), G.nbr.begin() + (size_t)r * nd, G.nbr.begin() + (size_t)(r + 1) * nd);
-        }
     // the (at most two) point neighbours of a path point, in neighbour order
     auto slots = [&](int p, int *s) {
         int k = 0;
@@ -219,15 +211,30 @@ static void host_graph(const std::vector<uint8_t> &m, int w, int h, int d, bool 
         c.x1xyz = c.x1 >= 0 ? G.idx[c.x1] : 0;
         G.chains.push_back(c);
     }
+    G.node_rank.clear();
+    G.node_xyz.clear();
+    G.node_off.assign(1, 0);
+    G.node_ent.clear();
+    for (int r = 0; r < n; ++r)
+        if (G.code[r] < 0) {
+            G.node_rank.push_back(r);
+            G.node_xyz.push_back(G.idx[r]);
+            for (int j = 0; j < nd; ++j) {
+                const int v = G.nbr[(size_t)r * nd + j];
+                if (v >= 0)
+                    G.node_ent.push_back((int64_t)(uint32_t)v | (int64_t)j << 32 | (int64_t)(G.chain[v] + 1) << 37);
+            }
+            G.node_off.push_back((int)G.node_ent.size());
+        }
     G.g = bgt::SegGraph{};
     G.g.npts = n;
     G.g.nd = nd;
     G.g.code = G.code.data();
-    G.g.chain = G.chain.data();
     G.g.nnodes = (int)G.node_rank.size();
     G.g.node_rank = G.node_rank.data();
     G.g.node_xyz = G.node_xyz.data();
-    G.g.node_nbr = G.node_nbr.data();
+    G.g.node_off = G.node_off.data();
+    G.g.node_ent = G.node_ent.data();
     G.g.nchains = (int)G.chains.size();
     G.g.chains = G.chains.data();
     G.g.chain_xyz = G.chain_xyz.data();
