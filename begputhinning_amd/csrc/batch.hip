@@ -490,6 +490,9 @@ struct Batch {
     unsigned long long *h_rep = nullptr;  // pinned, device-mapped pass reports (Close2D)
     char *h_stage = nullptr;   // pinned staging of the tables uploaded by prepare
     size_t h_stage_bytes = 0;
+    // outgrown pinned buffers, freed with the batch: hipHostFree waits for the
+    // device, which stalls the other threads' work on it
+    std::vector<void *> h_retired;
     int *h_states = nullptr;   // pinned copy of the item states after a run
     size_t h_states_ints = 0;
     hipEvent_t ev_poll3[2] = {nullptr, nullptr};
@@ -522,6 +525,7 @@ struct Batch {
         if (h_rep) (void)hipHostFree(h_rep);
         if (h_stage) (void)hipHostFree(h_stage);
         if (h_states) (void)hipHostFree(h_states);
+        for (void *p : h_retired) (void)hipHostFree(p);
     }
     void release()
     {
@@ -772,12 +776,13 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
         }
     }
     if (host_stage.size() > b.h_stage_bytes) {
-        if (b.h_stage) (void)hipHostFree(b.h_stage);
+        const size_t want = grow_bytes(host_stage.size(), b.h_stage_bytes);
+        if (b.h_stage) b.h_retired.push_back(b.h_stage);
         b.h_stage = nullptr;
         b.h_stage_bytes = 0;
-        if (hipHostMalloc((void **)&b.h_stage, host_stage.size(), hipHostMallocDefault) != hipSuccess)
-            return fail(BGT_ERR_NOMEM, "pinned staging of %zu bytes failed", host_stage.size());
-        b.h_stage_bytes = host_stage.size();
+        if (hipHostMalloc((void **)&b.h_stage, want, hipHostMallocDefault) != hipSuccess)
+            return fail(BGT_ERR_NOMEM, "pinned staging of %zu bytes failed", want);
+        b.h_stage_bytes = want;
     }
     if (!host_stage.empty()) std::memcpy(b.h_stage, host_stage.data(), host_stage.size());
     for (const Staged &t : staged)
@@ -1263,12 +1268,13 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     }
     const size_t nst = (size_t)(n2 + n3) * ST_INTS;
     if (nst > b.h_states_ints) {
-        if (b.h_states) (void)hipHostFree(b.h_states);
+        const size_t want = grow_bytes(nst * sizeof(int), b.h_states_ints * sizeof(int)) / sizeof(int);
+        if (b.h_states) b.h_retired.push_back(b.h_states);
         b.h_states = nullptr;
         b.h_states_ints = 0;
-        if (hipHostMalloc((void **)&b.h_states, nst * sizeof(int), hipHostMallocDefault) != hipSuccess)
+        if (hipHostMalloc((void **)&b.h_states, want * sizeof(int), hipHostMallocDefault) != hipSuccess)
             return fail(BGT_ERR_NOMEM, "pinned state copy failed");
-        b.h_states_ints = nst;
+        b.h_states_ints = want;
     }
     if (n2)
         BGT_HIP(hipMemcpyAsync(b.h_states, b.d_states2, (size_t)n2 * ST_INTS * sizeof(int),

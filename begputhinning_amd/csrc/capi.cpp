@@ -50,15 +50,12 @@ int workspace_reserve(Workspace &ws, size_t bytes)
     int dev = 0;
     BGT_HIP(hipGetDevice(&dev));
     if (ws.ptr && ws.bytes >= bytes && ws.device == dev) return BGT_OK;
-    if (ws.ptr) {
-        int cur = dev;
-        if (ws.device != dev) BGT_HIP(hipSetDevice(ws.device));
-        (void)hipFree(ws.ptr);
-        if (ws.device != dev) BGT_HIP(hipSetDevice(cur));
+    const size_t want = grow_bytes(bytes, ws.device == dev ? ws.bytes : 0);
+    if (ws.ptr) {  // work queued before may still use it: freed with the workspace
+        ws.retired.emplace_back(ws.ptr, ws.device);
         ws.ptr = nullptr;
         ws.bytes = 0;
     }
-    size_t want = bytes + bytes / 4 + 4096;
     void *p = nullptr;
     hipError_t e = device_malloc(&p, want);
     if (e != hipSuccess) {
@@ -90,12 +87,17 @@ Workspace::~Workspace()
 {
     // at process exit the runtime may be going away: the main thread's memory
     // is returned with the process
-    if (!ptr || on_main_thread()) return;
-    int cur = -1;
-    if (hipGetDevice(&cur) == hipSuccess && cur != device) (void)hipSetDevice(device);
-    (void)hipFree(ptr);
-    if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+    if (on_main_thread()) return;
+    if (ptr) retired.emplace_back(ptr, device);
     ptr = nullptr;
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    for (const auto &r : retired) {
+        if (r.second != cur) (void)hipSetDevice(r.second);
+        (void)hipFree(r.first);
+        if (r.second != cur && cur >= 0) (void)hipSetDevice(cur);
+    }
+    retired.clear();
 }
 
 PinnedBuffer::~PinnedBuffer()

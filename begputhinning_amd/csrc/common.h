@@ -6,6 +6,8 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/bgt.h"
 
@@ -40,12 +42,26 @@ hipError_t device_malloc(void **p, size_t bytes);
 // A thread's workspaces and pinned poll buffers are released when the thread
 // exits (server workers, caller thread pools); the process's main thread
 // keeps them until the process ends.
+//
+// An outgrown buffer is retired, not freed: hipFree waits for the whole
+// device, which stalls every other thread's work on it for milliseconds under
+// load.  Each growth at least doubles the buffer, so the retired buffers
+// together hold less than the current one.
 struct Workspace {
     void *ptr = nullptr;
     size_t bytes = 0;
     int device = -1;
+    std::vector<std::pair<void *, int>> retired;  // (buffer, device)
     ~Workspace();
 };
+
+// The size to allocate for a grow-only buffer of `have` bytes that must hold
+// `need`: a quarter more than asked, and at least twice what it had.
+inline size_t grow_bytes(size_t need, size_t have)
+{
+    const size_t want = need + need / 4 + 4096;
+    return have && want < 2 * have ? 2 * have : want;
+}
 // Pinned host memory of one thread (hipHostMalloc), released at thread exit.
 struct PinnedBuffer {
     void *ptr = nullptr;

@@ -56,4 +56,9 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
 int segment_graph_host(int dtype, const void *data, int w, int h, int d, bool three_d, hipStream_t s,
                        SegGraph &g);
 
+// Reserves this thread's pre-pass workspaces and pinned staging for a frame
+// of w x h x d with up to npts skeleton points, so that calls up to that size
+// allocate nothing (serving tracers call it when they start).
+int segment_reserve(int w, int h, int d, bool three_d, long npts);
+
 }  // namespace bgt

@@ -26,9 +26,16 @@
 //   seg_link, seg_jump  -- list ranking of the chains by pointer jumping;
 //   seg_chain_flags, seg_scan, seg_chain_base -- chain ids and records in
 //                          the order of each chain's first end;
-//   seg_chain_scatter   -- every chain's points in order.
+//   seg_chain_scatter   -- every chain's points in order;
+//   seg_gather          -- the graph's parts into the thread's pinned staging.
 // The host's second pass then follows a chain as one run instead of point by
 // point.
+//
+// Nothing goes through the runtime's copy engine: the totals the host waits
+// for and the graph itself are written by kernels straight into mapped pinned
+// memory.  Under load from other threads (serving), a device-to-host
+// hipMemcpyAsync can block in the runtime for milliseconds before it is even
+// queued; a kernel launch does not.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -455,24 +462,56 @@ __global__ __launch_bounds__(256) void nonzero_u8(const T *__restrict__ in, uint
 // Grow-only pinned host staging of one thread's pre-pass results (device to
 // host copies into pageable memory go through the runtime's own staging),
 // released at thread exit.
+// the graph's parts, each from a 256-byte aligned device region to a 16-byte
+// aligned place in mapped pinned memory; blockIdx.y picks the part
+struct GatherParts {
+    const char *src[8];
+    char *dst[8];
+    size_t bytes[8];
+};
+
+__global__ __launch_bounds__(256) void seg_gather(GatherParts g)
+{
+    const int k = blockIdx.y;
+    const size_t nb = g.bytes[k], n16 = nb / 16;
+    const uint4 *s = reinterpret_cast<const uint4 *>(g.src[k]);
+    uint4 *d = reinterpret_cast<uint4 *>(g.dst[k]);
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (size_t)gridDim.x * 256) d[i] = s[i];
+    if (blockIdx.x == 0 && threadIdx.x < nb - n16 * 16) g.dst[k][n16 * 16 + threadIdx.x] = g.src[k][n16 * 16 + threadIdx.x];
+}
+
 struct PinnedStage {
-    char *ptr = nullptr;
+    char *ptr = nullptr;   // host address
+    char *dptr = nullptr;  // the same memory as kernels address it
     size_t bytes = 0;
+    // outgrown buffers, kept until the thread exits: hipHostFree waits for
+    // the device (see Workspace)
+    std::vector<char *> retired;
     ~PinnedStage()
     {
-        if (ptr && !on_main_thread()) (void)hipHostFree(ptr);
+        if (on_main_thread()) return;
+        if (ptr) (void)hipHostFree(ptr);
+        for (char *p : retired) (void)hipHostFree(p);
     }
     int reserve(size_t n)
     {
         if (ptr && bytes >= n) return BGT_OK;
-        if (ptr) (void)hipHostFree(ptr);
-        ptr = nullptr;
+        const size_t want = grow_bytes(n, bytes);
+        if (ptr) retired.push_back(ptr);
+        ptr = dptr = nullptr;
         bytes = 0;
-        const size_t want = n + n / 4 + 4096;
-        if (hipHostMalloc((void **)&ptr, want, hipHostMallocDefault) != hipSuccess) {
+        // coherent: what kernels write here is in host memory when the
+        // stream's work is done, whatever scope the runtime's fences take
+        if (hipHostMalloc((void **)&ptr, want, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
             (void)hipGetLastError();
             ptr = nullptr;
             return fail(BGT_ERR_NOMEM, "pinned staging of %zu bytes failed", want);
+        }
+        if (hipHostGetDevicePointer((void **)&dptr, ptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            retired.push_back(ptr);
+            ptr = dptr = nullptr;
+            return fail(BGT_ERR_HIP, "pinned staging has no device address");
         }
         bytes = want;
         return BGT_OK;
@@ -484,6 +523,42 @@ PinnedStage &thread_stage()
     thread_local PinnedStage st;
     return st;
 }
+
+// Device bytes of the pre-pass's three workspaces: the point count over a
+// frame of nchunks 64-pixel chunks (bits, counts, bases, workgroup sums,
+// total); the per-point arrays and what the host reads, for n points; the
+// chain ranking, for n points.
+struct CountLayout {
+    size_t bits, ints, wsum;
+    explicit CountLayout(long nchunks)
+        : bits(round_up(sizeof(uint64_t) * nchunks, 256)), ints(round_up(sizeof(int) * nchunks, 256)),
+          wsum(round_up(sizeof(int) * ((nchunks + 255) / 256), 256))
+    {
+    }
+    size_t total() const { return bits + 2 * ints + wsum + 256; }
+};
+
+struct GraphLayout {
+    size_t idx, code, nbr, ints, xyz, ent, rec;
+    GraphLayout(long n, int nd)
+        : idx(round_up(sizeof(int64_t) * n, 256)), code(round_up((size_t)n, 256)),
+          nbr(round_up(sizeof(int) * (size_t)n * nd, 256)), ints(round_up(sizeof(int) * ((size_t)n + 1), 256)),
+          xyz(round_up(sizeof(int64_t) * n, 256)), ent(round_up(sizeof(int64_t) * (size_t)n * nd, 256)),
+          rec(round_up(sizeof(SegChain) * (size_t)n, 256))
+    {
+    }
+    size_t total() const { return idx + nbr + code + 2 * ints + 2 * xyz + ent + rec; }
+};
+
+struct ChainLayout {
+    size_t half, one, wsz;
+    explicit ChainLayout(long n)
+        : half(round_up(sizeof(int) * 2 * (size_t)n, 256)), one(round_up(sizeof(int) * (size_t)n, 256)),
+          wsz(round_up(sizeof(int) * (size_t)((n + 255) / 256), 256))
+    {
+    }
+    size_t total() const { return 8 * half + 7 * one + 4 * wsz + 256; }
+};
 
 }  // namespace
 
@@ -520,6 +595,17 @@ int segment_graph_host(int dtype, const void *data, int w, int h, int d, bool th
     return segment_graph_device(mask, w, h, d, w, h, d, three_d, s, g);
 }
 
+int segment_reserve(int w, int h, int d, bool three_d, long npts)
+{
+    if (w <= 0 || h <= 0 || d <= 0 || npts < 0) return fail(BGT_ERR_INVALID, "bad size");
+    BGT_CHECK(workspace_reserve(thread_workspace(5), CountLayout((long)d * h * ((w + 63) / 64)).total()));
+    if (!npts) return BGT_OK;
+    BGT_CHECK(workspace_reserve(thread_workspace(6), GraphLayout(npts, three_d ? 26 : 8).total()));
+    BGT_CHECK(workspace_reserve(thread_workspace(7), ChainLayout(npts).total()));
+    // a graph takes about 16 bytes per point to the host
+    return thread_stage().reserve((size_t)npts * 32 + 4096);
+}
+
 int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int ph, int pd,
                          bool three_d, hipStream_t s, SegGraph &g)
 {
@@ -534,27 +620,24 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     const long nwg = (nchunks + 255) / 256;
     if (nwg > 0x7fffffffL) return fail(BGT_ERR_UNSUPPORTED, "skeleton too large for the pre-pass");
     // scratch: bits, counts, bases, workgroup sums, total
-    const size_t need = round_up(sizeof(uint64_t) * nchunks, 256) + 2 * round_up(sizeof(int) * nchunks, 256) +
-                        round_up(sizeof(int) * nwg, 256) + 256;
+    const CountLayout cl(nchunks);
     Workspace &ws = thread_workspace(5);
-    BGT_CHECK(workspace_reserve(ws, need));
+    BGT_CHECK(workspace_reserve(ws, cl.total()));
     char *p = (char *)ws.ptr;
     uint64_t *bits = (uint64_t *)p;
-    p += round_up(sizeof(uint64_t) * nchunks, 256);
+    p += cl.bits;
     int *cnt = (int *)p;
-    p += round_up(sizeof(int) * nchunks, 256);
+    p += cl.ints;
     int *base = (int *)p;
-    p += round_up(sizeof(int) * nchunks, 256);
+    p += cl.ints;
     int *wsum = (int *)p;
-    p += round_up(sizeof(int) * nwg, 256);
-    long *d_total = (long *)p;
-    hipLaunchKernelGGL(seg_count, dim3((unsigned)nwg), dim3(256), 0, s, f, nchunks, bits, cnt, wsum);
-    hipLaunchKernelGGL(seg_scan, dim3(1), dim3(1024), 0, s, wsum, (int)nwg, d_total);
-    hipLaunchKernelGGL(seg_base, dim3((unsigned)nwg), dim3(256), 0, s, nchunks, cnt, wsum, base);
-    BGT_HIP(hipGetLastError());
+    // the point count goes straight to the pinned staging
     PinnedStage &st = thread_stage();
     BGT_CHECK(st.reserve(4096));
-    BGT_HIP(hipMemcpyAsync(st.ptr, d_total, sizeof(long), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(seg_count, dim3((unsigned)nwg), dim3(256), 0, s, f, nchunks, bits, cnt, wsum);
+    hipLaunchKernelGGL(seg_scan, dim3(1), dim3(1024), 0, s, wsum, (int)nwg, (long *)st.dptr);
+    hipLaunchKernelGGL(seg_base, dim3((unsigned)nwg), dim3(256), 0, s, nchunks, cnt, wsum, base);
+    BGT_HIP(hipGetLastError());
     BGT_HIP(hipStreamSynchronize(s));
     long total = 0;
     std::memcpy(&total, st.ptr, sizeof(long));
@@ -562,33 +645,30 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     g.npts = (int)total;
     if (!g.npts) return BGT_OK;
     const int n = g.npts;
-    // the pre-pass's per-point arrays (device only)
-    const size_t a_idx = round_up(sizeof(int64_t) * n, 256), a_code = round_up((size_t)n, 256),
-                 a_nbr = round_up(sizeof(int) * (size_t)n * g.nd, 256), a_int = round_up(sizeof(int) * ((size_t)n + 1), 256);
-    // what the host reads, each region sized for the most there can be (only
-    // the used part of each is copied)
-    const size_t a_nxyz = round_up(sizeof(int64_t) * n, 256), a_ent = round_up(sizeof(int64_t) * (size_t)n * g.nd, 256),
-                 a_rec = round_up(sizeof(SegChain) * (size_t)n, 256);
+    // the pre-pass's per-point arrays (device only), then what the host
+    // reads, each region sized for the most there can be (only the used part
+    // of each is copied)
+    const GraphLayout gl(n, g.nd);
     Workspace &wo = thread_workspace(6);
-    BGT_CHECK(workspace_reserve(wo, a_idx + a_nbr + a_code + 2 * a_int + 2 * a_nxyz + a_ent + a_rec));
+    BGT_CHECK(workspace_reserve(wo, gl.total()));
     char *q = (char *)wo.ptr;
     int64_t *d_idx = (int64_t *)q;
-    q += a_idx;
+    q += gl.idx;
     int *d_nbr = (int *)q;
-    q += a_nbr;
+    q += gl.nbr;
     int8_t *d_code = (int8_t *)q;
-    q += a_code;
+    q += gl.code;
     GraphOut go;
     go.node_rank = (int *)q;
-    q += a_int;
+    q += gl.ints;
     go.node_off = (int *)q;
-    q += a_int;
+    q += gl.ints;
     go.node_xyz = (int64_t *)q;
-    q += a_nxyz;
+    q += gl.xyz;
     go.chain_xyz = (int64_t *)q;
-    q += a_nxyz;
+    q += gl.xyz;
     go.node_ent = (int64_t *)q;
-    q += a_ent;
+    q += gl.ent;
     go.rec = (SegChain *)q;
     hipLaunchKernelGGL(seg_place, dim3((unsigned)nwg), dim3(256), 0, s, f, nchunks, bits, base, d_idx);
     hipLaunchKernelGGL(seg_emit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, n, bits, base,
@@ -598,10 +678,10 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     // chains: link, ceil(log5 n) jump launches; chain starts and lengths,
     // nodes and their entries, their prefix sums; records, nodes, runs
     const int pwg = (n + 255) / 256, hwg = (2 * n + 255) / 256;
-    const size_t half = round_up(sizeof(int) * 2 * (size_t)n, 256), one = round_up(sizeof(int) * (size_t)n, 256),
-                 wsz = round_up(sizeof(int) * (size_t)pwg, 256);
+    const ChainLayout chl(n);
+    const size_t half = chl.half, one = chl.one, wsz = chl.wsz;
     Workspace &wt = thread_workspace(7);
-    BGT_CHECK(workspace_reserve(wt, 8 * half + 7 * one + 4 * wsz + 256));
+    BGT_CHECK(workspace_reserve(wt, chl.total()));
     char *t = (char *)wt.ptr;
     Jump jb[2];
     for (Jump &j : jb) {
@@ -622,7 +702,8 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     ct.wlen = (int *)t; t += wsz;
     ct.wnode = (int *)t; t += wsz;
     ct.went = (int *)t; t += wsz;
-    long *d_tot = (long *)t;  // [0] chains, [1] chain points, [2] nodes, [3] node entries
+    // to the pinned staging: [0] chains, [1] chain points, [2] nodes, [3] node entries
+    long *d_tot = (long *)st.dptr;
     hipLaunchKernelGGL(seg_link, dim3((unsigned)pwg), dim3(256), 0, s, n, g.nd, d_code, d_nbr, jb[0]);
     int cur = 0;
     for (long reach = 1; reach < n; reach *= kHops + 1, cur ^= 1)
@@ -636,7 +717,6 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
     hipLaunchKernelGGL(seg_chain_scatter, dim3((unsigned)pwg), dim3(256), 0, s, n, g.nd, d_idx, d_code, d_nbr,
                        ct, go);
     BGT_HIP(hipGetLastError());
-    BGT_HIP(hipMemcpyAsync(st.ptr, d_tot, 4 * sizeof(long), hipMemcpyDeviceToHost, s));
     BGT_HIP(hipStreamSynchronize(s));
     long tot[4];
     std::memcpy(tot, st.ptr, sizeof(tot));
@@ -665,9 +745,17 @@ int segment_graph_device(const uint8_t *d_skel, int w, int h, int d, int pw, int
         bytes += round_up(parts[k].bytes + (k == 2 ? sizeof(int) : 0), 16);  // room for node_off[nn]
     }
     BGT_CHECK(st.reserve(bytes));
-    for (int k = 0; k < kParts; ++k)
-        if (parts[k].bytes)
-            BGT_HIP(hipMemcpyAsync(st.ptr + at[k], parts[k].src, parts[k].bytes, hipMemcpyDeviceToHost, s));
+    GatherParts gp{};
+    size_t most = 0;
+    for (int k = 0; k < kParts; ++k) {
+        gp.src[k] = (const char *)parts[k].src;
+        gp.dst[k] = st.dptr + at[k];
+        gp.bytes[k] = parts[k].bytes;
+        most = std::max(most, parts[k].bytes);
+    }
+    const unsigned gx = (unsigned)std::min<size_t>(std::max<size_t>((most + 4095) / 4096, 1), 1024);
+    hipLaunchKernelGGL(seg_gather, dim3(gx, kParts), dim3(256), 0, s, gp);
+    BGT_HIP(hipGetLastError());
     BGT_HIP(hipStreamSynchronize(s));
     // the host trace reads these at random: from ordinary pageable memory
     // (this thread's copy, kept between calls) it runs faster than from the

@@ -108,6 +108,8 @@ struct Server {
     std::deque<Request *> tq;
     std::condition_variable cv_trace;
     std::vector<std::thread> tracers;
+    const uint8_t *warm = nullptr;  // warm-up skeletons while tracers start
+    int tracers_warm = 0;
 
     void worker(int cls);
     void tracer();
@@ -127,6 +129,22 @@ void Server::completed_locked(Request *r)
     }
 }
 
+// The warm-up skeletons: a 16 x 16 frame (a line of 11 pixels with a branch
+// of 5 meeting it) and, after it, an 8 x 8 x 8 volume (a line of 6 voxels
+// with a branch of 3).
+const int kWarm2 = 16 * 16, kWarm3 = 8 * 8 * 8;
+
+std::vector<uint8_t> warm_skeletons()
+{
+    std::vector<uint8_t> v(kWarm2 + kWarm3, 0);
+    for (int x = 2; x <= 12; ++x) v[8 * 16 + x] = 1;
+    for (int y = 3; y <= 7; ++y) v[y * 16 + 7] = 1;
+    uint8_t *c = v.data() + kWarm2;
+    for (int x = 1; x <= 6; ++x) c[(4 * 8 + 4) * 8 + x] = 1;
+    for (int z = 1; z <= 3; ++z) c[(z * 8 + 4) * 8 + 3] = 1;
+    return v;
+}
+
 void Server::tracer()
 {
     // below the engine's host threads, which must react to every pass: a
@@ -135,6 +153,29 @@ void Server::tracer()
     (void)hipSetDevice(device);
     hipStream_t s = nullptr;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    // Before the first request: this thread's workspaces (for up to 2^17
+    // points of a 256^3 volume, which also covers a 1024 x 1024 frame's) and
+    // the pre-pass kernels' first launches, which otherwise cost the first
+    // requests milliseconds.
+    // Failures here are left to the requests to report.
+    if (s) {
+        (void)segment_reserve(256, 256, 256, true, 1L << 17);
+        for (int k = 0; warm && k < 2; ++k) {
+            const bool three = k == 1;
+            const int e = three ? 8 : 16;
+            SegGraph gr;
+            bgt_segments g{};
+            if (segment_graph_device(warm + (three ? kWarm2 : 0), e, e, three ? e : 1, e, e, three ? e : 1,
+                                     three, s, gr) == BGT_OK)
+                (void)extract_segments_graph(gr, e, e, three, &g);
+            free_segments(g);
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(m);
+        ++tracers_warm;
+    }
+    cv_done.notify_all();
     for (;;) {
         Request *r = nullptr;
         {
@@ -203,6 +244,7 @@ void Server::worker(int cls)
     (void)batch_set_config(*b, c2, c3);
     void *arena = nullptr;
     size_t arena_bytes = 0;
+    std::vector<void *> retired;
     std::vector<Request *> take;
     std::vector<int> iters;
     for (;;) {
@@ -244,10 +286,12 @@ void Server::worker(int cls)
         if (rc == BGT_OK) rc = batch_plan_bytes(*b, &need);
         if (rc == BGT_OK && need > arena_bytes) {
             batch_set_arena(*b, nullptr, 0);
-            if (arena) (void)hipFree(arena);
+            // the outgrown arena is freed when the worker leaves: hipFree
+            // waits for the whole device, stalling the other workers
+            if (arena) retired.push_back(arena);
+            const size_t grow = grow_bytes(need, arena_bytes);
             arena = nullptr;
             arena_bytes = 0;
-            const size_t grow = need + need / 4;
             if (device_malloc(&arena, grow) == hipSuccess) {
                 arena_bytes = grow;
             } else {
@@ -266,6 +310,7 @@ void Server::worker(int cls)
     batch_set_arena(*b, nullptr, 0);
     b.reset();
     if (arena) (void)hipFree(arena);
+    for (void *p : retired) (void)hipFree(p);
     if (s) (void)hipStreamDestroy(s);
 }
 
@@ -353,11 +398,30 @@ void server_destroy(Server *sv)
 int server_enable_segments(Server *sv, int threads)
 {
     if (!sv || threads < 1 || threads > 64) return fail(BGT_ERR_INVALID, "bad tracer count");
-    std::lock_guard<std::mutex> lk(sv->m);
+    std::unique_lock<std::mutex> lk(sv->m);
     if (sv->tracing) return fail(BGT_ERR_INVALID, "segment tracing is already on");
     if (!sv->reqs.empty()) return fail(BGT_ERR_INVALID, "turn segment tracing on before submitting");
+    // the warm-up skeletons on the server's device (without them the tracers
+    // only reserve their workspaces)
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(sv->device);
+    void *warm = nullptr;
+    const std::vector<uint8_t> pattern = warm_skeletons();
+    if (device_malloc(&warm, pattern.size()) != hipSuccess ||
+        hipMemcpy(warm, pattern.data(), pattern.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        if (warm) (void)hipFree(warm);
+        warm = nullptr;
+    }
+    if (cur >= 0) (void)hipSetDevice(cur);
+    sv->warm = (const uint8_t *)warm;
     sv->tracing = threads;
     for (int k = 0; k < threads; ++k) sv->tracers.emplace_back(&Server::tracer, sv);
+    sv->cv_done.wait(lk, [&] { return sv->tracers_warm == threads; });
+    sv->warm = nullptr;
+    lk.unlock();
+    if (warm) (void)hipFree(warm);
     return BGT_OK;
 }
 

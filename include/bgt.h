@@ -278,7 +278,10 @@ int bgt_server_poll(bgt_server s, uint64_t ticket, int *done);
  * then the host trace -- the result that of bgt_extract_segments2d/3d_device
  * on the output (a frame of the output's size; 2D outputs need contiguous
  * rows).  Call before the first submit.  bgt_server_wait then also waits for
- * the trace (and drops the list). */
+ * the trace (and drops the list).  Each tracer thread reserves its
+ * workspaces (about 80 MB of device memory and 5 MB of pinned host memory,
+ * enough for 2^17 points) and traces two tiny skeletons before this returns,
+ * so that the first requests do not pay for either. */
 int bgt_server_enable_segments(bgt_server s, int threads);
 /* Blocks until the request's segment list is ready; hands it over in *out
  * (release with bgt_segments_free).  latency_ms: submit-to-skeleton,
