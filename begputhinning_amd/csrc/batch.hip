@@ -487,6 +487,7 @@ struct Batch {
     long ntiles2 = 0;
 
     int *h_live = nullptr;  // pinned, two slots of 32: [8..] 3D control words; [64..66] 2D chain words
+    int *dh_live = nullptr;  // its device address (kernels write the polled words)
     unsigned long long *h_rep = nullptr;  // pinned, device-mapped pass reports (Close2D)
     char *h_stage = nullptr;   // pinned staging of the tables uploaded by prepare
     size_t h_stage_bytes = 0;
@@ -494,6 +495,7 @@ struct Batch {
     // device, which stalls the other threads' work on it
     std::vector<void *> h_retired;
     int *h_states = nullptr;   // pinned copy of the item states after a run
+    int *dh_states = nullptr;  // its device address
     size_t h_states_ints = 0;
     hipEvent_t ev_poll3[2] = {nullptr, nullptr};
     hipStream_t side = nullptr;
@@ -790,8 +792,7 @@ static int prepare(Batch &b, hipStream_t s, bool size_only = false)
     if (n2)
         b.grid2 = queue_pass_grid2d(b.c2, b.V2, (int)std::min<long>(b.ntiles2 * kParts, 1L << 30),
                                     blocks2(b));
-    if (!b.h_live && hipHostMalloc((void **)&b.h_live, 96 * sizeof(int), hipHostMallocDefault) != hipSuccess)
-        return fail(BGT_ERR_NOMEM, "pinned allocation failed");
+    if (!b.h_live) BGT_CHECK(host_alloc_mapped(96 * sizeof(int), (void **)&b.h_live, (void **)&b.dh_live));
     if (!b.h_rep) {
         if (hipHostMalloc((void **)&b.h_rep, 16 * sizeof(unsigned long long),
                           hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
@@ -1128,7 +1129,6 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     c3.live = n3 > 0;
     auto enqueue3 = [&](Chain &c) -> int {
         const int slot = (c.head + c.inflight) & 1;
-        int *hs = b.h_live + slot * 32;
         c.tiles[slot] = b.on_tiles3;
         if (b.on_tiles3) {
             // a chunk of iterations, one launch each; iteration t closes the
@@ -1138,7 +1138,7 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
             BGT_CHECK(timed_mark(b, s3, 1, true, &mk));
             for (int i = 0; i < kChunk; ++i) BGT_CHECK(tile3d_iteration(b.t3, b.titer3++, grid, s3));
             BGT_CHECK(timed_mark(b, s3, 1, false, &mk));
-            BGT_HIP(hipMemcpyAsync(hs + 8, b.t3.ctl, TC_INTS * sizeof(int), hipMemcpyDeviceToHost, s3));
+            BGT_CHECK(copy_ints_to_host(b.t3.ctl, b.dh_live + slot * 32 + 8, TC_INTS, s3));
             BGT_HIP(hipEventRecord(b.ev_poll3[slot], s3));
             ++c.inflight;
             if (++c.chunks > max_chunks) return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
@@ -1156,7 +1156,7 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
         } else {
             BGT_CHECK(launch_chunk(b, true, use_graph, s3));
         }
-        BGT_HIP(hipMemcpyAsync(hs + 8, b.e3.ctl, SC_INTS * sizeof(int), hipMemcpyDeviceToHost, s3));
+        BGT_CHECK(copy_ints_to_host(b.e3.ctl, b.dh_live + slot * 32 + 8, SC_INTS, s3));
         BGT_HIP(hipEventRecord(b.ev_poll3[slot], s3));
         ++c.inflight;
         if (++c.chunks > max_chunks) return fail(BGT_ERR_NOT_CONVERGED, "batch did not converge");
@@ -1270,20 +1270,18 @@ int batch_run(Batch &b, hipStream_t s, bool use_graph)
     if (nst > b.h_states_ints) {
         const size_t want = grow_bytes(nst * sizeof(int), b.h_states_ints * sizeof(int)) / sizeof(int);
         if (b.h_states) b.h_retired.push_back(b.h_states);
-        b.h_states = nullptr;
+        b.h_states = b.dh_states = nullptr;
         b.h_states_ints = 0;
-        if (hipHostMalloc((void **)&b.h_states, want * sizeof(int), hipHostMallocDefault) != hipSuccess)
-            return fail(BGT_ERR_NOMEM, "pinned state copy failed");
+        BGT_CHECK(host_alloc_mapped(want * sizeof(int), (void **)&b.h_states, (void **)&b.dh_states));
         b.h_states_ints = want;
     }
     if (n2)
-        BGT_HIP(hipMemcpyAsync(b.h_states, b.d_states2, (size_t)n2 * ST_INTS * sizeof(int),
-                               hipMemcpyDeviceToHost, s));
+        BGT_CHECK(copy_ints_to_host(b.d_states2, b.dh_states, n2 * ST_INTS, s));
     if (n3)
-        BGT_HIP(hipMemcpyAsync(b.h_states + (size_t)n2 * ST_INTS, b.on_tiles3 ? b.t3.states : b.e3.states,
-                               (size_t)n3 * ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
+        BGT_CHECK(copy_ints_to_host(b.on_tiles3 ? b.t3.states : b.e3.states, b.dh_states + (size_t)n2 * ST_INTS,
+                                    n3 * ST_INTS, s));
     if (n2 && b.c2.persistent)
-        BGT_HIP(hipMemcpyAsync(b.h_live + 64, b.d_ctl + 8, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+        BGT_CHECK(copy_ints_to_host(b.d_ctl + 8, b.dh_live + 64, 3, s));
     BGT_HIP(hipStreamSynchronize(s));
     if (n2 && b.c2.persistent && b.h_live[66])
         return fail(BGT_ERR_HIP, "persistent 2D chain: a device-wide barrier waited past its limit");

@@ -63,10 +63,21 @@ inline size_t grow_bytes(size_t need, size_t have)
     return have && want < 2 * have ? 2 * have : want;
 }
 // Pinned host memory of one thread (hipHostMalloc), released at thread exit.
+// Mapped and coherent (pinned_mapped): kernels write it directly.
 struct PinnedBuffer {
     void *ptr = nullptr;
+    void *dptr = nullptr;  // its device address
     ~PinnedBuffer();
 };
+// Gives b `bytes` of mapped, coherent pinned memory, unless it has some.
+int pinned_mapped(PinnedBuffer &b, size_t bytes);
+// Allocates mapped, coherent pinned memory: *host, and its device address.
+int host_alloc_mapped(size_t bytes, void **host, void **dev);
+// Copies n ints of device memory to mapped pinned memory, given by its device
+// address, with a one-workgroup kernel on stream s.  Under load from other
+// threads a device-to-host hipMemcpyAsync can block in the runtime for
+// milliseconds before it is even queued; a kernel launch does not.
+int copy_ints_to_host(const int *src, int *dst, int n, hipStream_t s);
 bool on_main_thread();
 int workspace_reserve(Workspace &ws, size_t bytes);
 Workspace &thread_workspace(int slot);

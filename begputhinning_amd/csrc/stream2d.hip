@@ -122,7 +122,8 @@ struct Stream2D {
     int grid = 0;
     unsigned long long *h_rep = nullptr;  // host-mapped pass reports
     unsigned *h_closed = nullptr;         // host-mapped per-slot closed flags
-    int *h_states = nullptr;              // pinned state copies after unpack
+    int *h_states = nullptr;              // pinned state copies after unpack (mapped)
+    int *dh_states = nullptr;             // their device address
     std::vector<Slot> slots;
 
     std::mutex m;
@@ -268,8 +269,8 @@ void Stream2D::run()
                     waited = true;
                 }
                 launch_unpack_item2d(d_items + k, sl.r.out, sl.r.w, sl.r.h, sl.r.out_pitch, su);
-                (void)hipMemcpyAsync(h_states + (size_t)k * ST_INTS, d_states + (size_t)k * ST_INTS,
-                                     ST_INTS * sizeof(int), hipMemcpyDeviceToHost, su);
+                (void)copy_ints_to_host(d_states + (size_t)k * ST_INTS, dh_states + (size_t)k * ST_INTS, ST_INTS,
+                                        su);
                 (void)hipEventRecord(sl.ev, su);
                 sl.state = UNPACKING;
                 progress = true;
@@ -351,8 +352,8 @@ int stream2d_create(const Stream2DConfig &cfg, const Config2D &c2, Stream2DDone 
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
         hipHostMalloc((void **)&e->h_closed, ns * sizeof(unsigned),
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
-        hipHostMalloc((void **)&e->h_states, (size_t)ns * ST_INTS * sizeof(int), hipHostMallocDefault) !=
-            hipSuccess) {
+        host_alloc_mapped((size_t)ns * ST_INTS * sizeof(int), (void **)&e->h_states, (void **)&e->dh_states) !=
+            BGT_OK) {
         (void)hipGetLastError();
         if (e->h_rep) (void)hipHostFree(e->h_rep);
         if (e->h_closed) (void)hipHostFree(e->h_closed);

@@ -366,20 +366,56 @@ int setup_item2d(const Plan2D &pl, void *ws, hipStream_t s, Item2D **d_item, uns
     return BGT_OK;
 }
 
-// Per-thread pinned word for the convergence poll.
-static int *poll_buffer()
+__global__ __launch_bounds__(256) void copy_ints(const int *__restrict__ src, int *__restrict__ dst, int n)
+{
+    for (int i = threadIdx.x; i < n; i += 256) dst[i] = src[i];
+}
+
+int copy_ints_to_host(const int *src, int *dst, int n, hipStream_t s)
+{
+    if (n <= 0) return BGT_OK;
+    hipLaunchKernelGGL(copy_ints, dim3(1), dim3(256), 0, s, src, dst, n);
+    BGT_HIP(hipGetLastError());
+    return BGT_OK;
+}
+
+int host_alloc_mapped(size_t bytes, void **host, void **dev)
+{
+    *host = *dev = nullptr;
+    void *h = nullptr, *d = nullptr;
+    if (hipHostMalloc(&h, bytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(BGT_ERR_NOMEM, "pinned allocation of %zu bytes failed", bytes);
+    }
+    if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipHostFree(h);
+        return fail(BGT_ERR_HIP, "pinned memory has no device address");
+    }
+    *host = h;
+    *dev = d;
+    return BGT_OK;
+}
+
+int pinned_mapped(PinnedBuffer &b, size_t bytes)
+{
+    if (b.ptr) return BGT_OK;
+    return host_alloc_mapped(bytes, &b.ptr, &b.dptr);
+}
+
+// Per-thread pinned words for the convergence poll.
+static PinnedBuffer *poll_buffer()
 {
     thread_local PinnedBuffer b;  // released when the thread exits
-    if (!b.ptr) {
-        if (hipHostMalloc(&b.ptr, 64, hipHostMallocDefault) != hipSuccess) b.ptr = nullptr;
-    }
-    return (int *)b.ptr;
+    return pinned_mapped(b, 64) == BGT_OK ? &b : nullptr;
 }
 
 int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned *d_ctr,
                  const int *d_state, hipStream_t s, int *iterations, int *host_poll)
 {
-    int *hp = host_poll ? host_poll : poll_buffer();
+    // host_poll (callers that cannot allocate) takes a runtime copy
+    PinnedBuffer *pb = host_poll ? nullptr : poll_buffer();
+    int *hp = host_poll ? host_poll : pb ? (int *)pb->ptr : nullptr;
     if (!hp) return fail(BGT_ERR_NOMEM, "pinned poll buffer allocation failed");
     const long max_pass = (long)pl.w * pl.h / cfg.iters_per_pass + 2;  // each changing iteration deletes >= 1 px
     long launched = 0;
@@ -391,7 +427,10 @@ int run_passes2d(const Config2D &cfg, const Plan2D &pl, Item2D *d_item, unsigned
         for (int i = 0; i < chunk; ++i)
             BGT_CHECK(launch_image_pass2d(cfg, pl.V, pl.nbands, kp, d_item, d_ctr, s));
         launched += chunk;
-        BGT_HIP(hipMemcpyAsync(hp, d_state, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
+        if (pb)
+            BGT_CHECK(copy_ints_to_host(d_state, (int *)pb->dptr, ST_INTS, s));
+        else
+            BGT_HIP(hipMemcpyAsync(hp, d_state, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
         BGT_HIP(hipStreamSynchronize(s));
         if (hp[ST_DONE]) break;
         if (launched > max_pass) return fail(BGT_ERR_NOT_CONVERGED, "2D thinning did not converge");

@@ -85,13 +85,12 @@ int check_config3d(const Config3D &c)
     return BGT_OK;
 }
 
-static int *poll_buffer3()
+// Per-thread pinned words for the convergence poll (mapped: a kernel writes
+// them).
+static PinnedBuffer *poll_buffer3()
 {
     thread_local PinnedBuffer b;  // released when the thread exits
-    if (!b.ptr) {
-        if (hipHostMalloc(&b.ptr, 256, hipHostMallocDefault) != hipSuccess) b.ptr = nullptr;
-    }
-    return (int *)b.ptr;
+    return pinned_mapped(b, 256) == BGT_OK ? &b : nullptr;
 }
 
 // Engine for one volume in a workspace: [item|geo|io] [states+ctl] [changed]
@@ -196,13 +195,13 @@ static int one_volume_setup(int w, int h, int d, const uint8_t *in, uint8_t *out
 // engine), polling item 0 after chunks of 2, 4, ... up to max_chunk.  The
 // launch of iteration t closes the item when iteration t - 1 changed nothing.
 static int tiles_to_convergence(const Tile3D &e, int t, long max_iter, const Config3D &cfg,
-                                hipStream_t s, int *hp)
+                                hipStream_t s, int *hp, int *hd)
 {
     const int grid = tile3d_grid(cfg.blocks_per_cu);
     int chunk = std::min(2, cfg.max_chunk);
     for (;;) {
         for (int i = 0; i < chunk; ++i) BGT_CHECK(tile3d_iteration(e, t++, grid, s));
-        BGT_HIP(hipMemcpyAsync(hp, e.states, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
+        BGT_CHECK(copy_ints_to_host(e.states, hd, ST_INTS, s));
         BGT_HIP(hipStreamSynchronize(s));
         if (hp[ST_DONE]) return BGT_OK;
         if (t > max_iter) return fail(BGT_ERR_NOT_CONVERGED, "3D thinning did not converge");
@@ -216,8 +215,9 @@ static int one_volume_run(OneVolume &ov, const Config3D &cfg, hipStream_t s, int
 {
     const Sparse3D &e = ov.e;
     BGT_CHECK(sparse3d_begin(e, true, s));
-    int *hp = poll_buffer3();
-    if (!hp) return fail(BGT_ERR_NOMEM, "pinned poll buffer allocation failed");
+    PinnedBuffer *pb = poll_buffer3();
+    if (!pb) return fail(BGT_ERR_NOMEM, "pinned poll buffer allocation failed");
+    int *hp = (int *)pb->ptr, *hd = (int *)pb->dptr;
     const long max_iter = (long)ov.item.w * ov.item.h * ov.item.d + 2;
     const int grid = sparse3d_grid(cfg.blocks_per_cu);
     long done_iters = 0;
@@ -228,8 +228,7 @@ static int one_volume_run(OneVolume &ov, const Config3D &cfg, hipStream_t s, int
             cur ^= 1;
         }
         done_iters += chunk;
-        BGT_HIP(hipMemcpyAsync(hp, e.states, (ST_INTS + SC_INTS) * sizeof(int),
-                               hipMemcpyDeviceToHost, s));
+        BGT_CHECK(copy_ints_to_host(e.states, hd, ST_INTS + SC_INTS, s));
         BGT_HIP(hipStreamSynchronize(s));
         if (hp[ST_DONE]) break;
         if (done_iters > max_iter) return fail(BGT_ERR_NOT_CONVERGED, "3D thinning did not converge");
@@ -238,14 +237,13 @@ static int one_volume_run(OneVolume &ov, const Config3D &cfg, hipStream_t s, int
         if (ov.tiles && cfg.tail_words > 0 && abound <= cfg.tail_words && !(done_iters & 1)) {
             // few active words left: the tile engine, one launch per iteration
             BGT_CHECK(tile3d_from_words(ov.t, e, (int)done_iters, bound, s));
-            BGT_CHECK(tiles_to_convergence(ov.t, (int)done_iters, max_iter, cfg, s, hp));
+            BGT_CHECK(tiles_to_convergence(ov.t, (int)done_iters, max_iter, cfg, s, hp, hd));
             break;
         }
         if (!ov.tiles && cfg.tail_words > 0 && abound <= cfg.tail_words) {
             // few active words left: the remaining iterations in one launch
             BGT_CHECK(sparse3d_tail(e, s));
-            BGT_HIP(hipMemcpyAsync(hp, e.states, (ST_INTS + SC_INTS) * sizeof(int),
-                                   hipMemcpyDeviceToHost, s));
+            BGT_CHECK(copy_ints_to_host(e.states, hd, ST_INTS + SC_INTS, s));
             BGT_HIP(hipStreamSynchronize(s));
             if (!hp[ST_DONE]) return fail(BGT_ERR_NOT_CONVERGED, "3D tail did not converge");
             break;
@@ -317,14 +315,15 @@ static int one_volume_tiles_run(OneVolumeTiles &ov, const Config3D &cfg, hipStre
 {
     const Tile3D &e = ov.e;
     BGT_CHECK(tile3d_begin(e, true, s));
-    int *hp = poll_buffer3();
-    if (!hp) return fail(BGT_ERR_NOMEM, "pinned poll buffer allocation failed");
+    PinnedBuffer *pb = poll_buffer3();
+    if (!pb) return fail(BGT_ERR_NOMEM, "pinned poll buffer allocation failed");
+    int *hp = (int *)pb->ptr, *hd = (int *)pb->dptr;
     const long max_iter = (long)ov.item.w * ov.item.h * ov.item.d + 3;
     const int grid = tile3d_grid(cfg.blocks_per_cu);
     int t = 0, chunk = std::min(2, cfg.max_chunk);
     for (;;) {
         for (int i = 0; i < chunk; ++i) BGT_CHECK(tile3d_iteration(e, t++, grid, s));
-        BGT_HIP(hipMemcpyAsync(hp, e.states, ST_INTS * sizeof(int), hipMemcpyDeviceToHost, s));
+        BGT_CHECK(copy_ints_to_host(e.states, hd, ST_INTS, s));
         BGT_HIP(hipStreamSynchronize(s));
         if (hp[ST_DONE]) break;
         if (t > max_iter) return fail(BGT_ERR_NOT_CONVERGED, "3D thinning did not converge");
