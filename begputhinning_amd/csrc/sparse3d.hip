@@ -17,7 +17,8 @@
 // An iteration is six launches, U (with the protection pre-test of all six
 // directions, 3D/CUDA_ThinningKernel3D_Linear.cuh:690-702), N, E, D, S, W,
 // alternating arrays 0 -> 1 -> 0 ..., so it ends in array 0, and a compaction
-// launch that builds the next list and records per-item convergence.  The
+// launch that builds the next list and records per-item convergence.  W also
+// activates the blocks around the blocks its changes mark dirty.  The
 // subiterations run over the active list only (the words near the previous
 // iteration's changes, see sparse3d.h); the other words of the list cannot
 // change and hold the same value in both arrays.
@@ -264,6 +265,15 @@ __global__ __launch_bounds__(256) void sparse3d_unpack(const SItem3D *__restrict
     unpack_block64_wave_u8(f.out + (size_t)z * plane, plane, (uint32_t)it.w, c, src);
 }
 
+// Block b of item `it` as (bx, by, bz).
+__device__ __forceinline__ void block_xyz(const SItem3D &it, int b, int &bx, int &by, int &bz)
+{
+    const int l = b - it.bbase, byz = l / it.nh;
+    bx = l - byz * it.nh;
+    bz = byz / it.nby;
+    by = byz - bz * it.nby;
+}
+
 // The 3x3x3 word neighbourhood of interior word g, each word shifted so that
 // bit k holds the neighbour of voxel k (raster (dz, dy, dx) order).
 __device__ __forceinline__ void gather27(const uint32_t *__restrict__ src, uint32_t g, int2 gp,
@@ -310,16 +320,20 @@ __device__ __forceinline__ uint32_t sub_word(const uint32_t *__restrict__ src, u
 }
 
 // Subiteration D (0..5 = U, N, E, D, S, W) over the active list (list 0 in
-// the first iteration): src -> dst.  The last one stamps the blocks of the
-// words the iteration changed.
+// the first iteration): src -> dst.  The last one marks the blocks of the
+// words the iteration changed dirty and activates the 27 blocks around each
+// (the first entry to mark a block in this iteration stamps its
+// neighbourhood).
 template <int D>
 __global__ __launch_bounds__(256) void sparse3d_step(const uint32_t *__restrict__ src,
                                                      uint32_t *__restrict__ dst,
+                                                     const SItem3D *__restrict__ items,
                                                      const int2 *__restrict__ geo, SList act,
                                                      SList list0, uint32_t *__restrict__ prot,
                                                      uint32_t *__restrict__ first,
                                                      int *__restrict__ changed,
-                                                     const int *__restrict__ ctl, int *dstamp)
+                                                     const int *__restrict__ ctl, int *dstamp,
+                                                     int *bstamp)
 {
     // The list is cut into one contiguous range per XCD (workgroups are dealt
     // to the 8 XCDs round-robin; the grid is a multiple of 8 or smaller than
@@ -341,39 +355,24 @@ __global__ __launch_bounds__(256) void sparse3d_step(const uint32_t *__restrict_
             first[k] = fk;
         }
         if (del) changed[ii] = 1;
-        if (D == 5 && nv != first[k]) dstamp[list.blk[k]] = itn + 1;
-    }
-}
-
-// Block b is active in the next iteration when one of its 27 neighbour blocks
-// (itself included) was stamped dirty in this one.
-__global__ __launch_bounds__(256) void sparse3d_expand(const SItem3D *__restrict__ items,
-                                                       int nitems, const int *__restrict__ dstamp,
-                                                       int *__restrict__ bstamp, int nblocks,
-                                                       const int *__restrict__ ctl)
-{
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblocks) return;
-    const int stamp = ctl[SC_ITER] + 1;
-    int lo = 0, hi = nitems - 1;  // the item holding block b: the last with bbase <= b
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (items[mid].bbase <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const int nh = items[lo].nh, nby = items[lo].nby, nbz = items[lo].nbz;
-    const int l = b - items[lo].bbase, bx = l % nh, byz = l / nh, by = byz % nby, bz = byz / nby;
-    bool act = false;
-    for (int dz = -1; dz <= 1; ++dz) {
-        if (bz + dz < 0 || bz + dz >= nbz) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            if (by + dy < 0 || by + dy >= nby) continue;
-            const int row = b + (dz * nby + dy) * nh;
-            for (int dx = -1; dx <= 1; ++dx)
-                if (bx + dx >= 0 && bx + dx < nh) act |= dstamp[row + dx] == stamp;
+        if (D == 5 && nv != first[k]) {
+            const int b = list.blk[k], stamp = itn + 1;
+            if (dstamp[b] != stamp && atomicMax(&dstamp[b], stamp) < stamp) {
+                const SItem3D it = items[ii];
+                int bx, by, bz;
+                block_xyz(it, b, bx, by, bz);
+                for (int dz = -1; dz <= 1; ++dz) {
+                    if (bz + dz < 0 || bz + dz >= it.nbz) continue;
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        if (by + dy < 0 || by + dy >= it.nby) continue;
+                        const int row = b + (dz * it.nby + dy) * it.nh;
+                        for (int dx = -1; dx <= 1; ++dx)
+                            if (bx + dx >= 0 && bx + dx < it.nh) bstamp[row + dx] = stamp;
+                    }
+                }
+            }
         }
     }
-    if (act) bstamp[b] = stamp;
 }
 
 // Keep the entries whose word is non-empty and whose item changed in this
@@ -465,14 +464,6 @@ __global__ __launch_bounds__(256) void sparse3d_compact(const uint32_t *__restri
     }
 }
 
-// Block b of item `it` as (bx, by, bz).
-__device__ __forceinline__ void block_xyz(const SItem3D &it, int b, int &bx, int &by, int &bz)
-{
-    const int l = b - it.bbase, byz = l / it.nh;
-    bx = l - byz * it.nh;
-    bz = byz / it.nby;
-    by = byz - bz * it.nby;
-}
 
 // The remaining iterations of item blockIdx.x (see sparse3d_tail).  All
 // branches on shared values are taken after a barrier, so every thread
@@ -704,8 +695,8 @@ int sparse3d_iteration(const Sparse3D &e, int cur, int bound, int abound, int gr
     const dim3 g(steps), b(256);
     uint32_t *s0 = e.state[0], *s1 = e.state[1];
 #define BGT_STEP(D, SRC, DST)                                                                    \
-    hipLaunchKernelGGL(sparse3d_step<D>, g, b, 0, s, SRC, DST, e.geo, A, e.list[0], e.prot,      \
-                       e.first, e.changed, e.ctl, e.dstamp)
+    hipLaunchKernelGGL(sparse3d_step<D>, g, b, 0, s, SRC, DST, e.items, e.geo, A, e.list[0],     \
+                       e.prot, e.first, e.changed, e.ctl, e.dstamp, e.bstamp)
     BGT_STEP(0, s0, s1);
     BGT_STEP(1, s1, s0);
     BGT_STEP(2, s0, s1);
@@ -713,8 +704,6 @@ int sparse3d_iteration(const Sparse3D &e, int cur, int bound, int abound, int gr
     BGT_STEP(4, s0, s1);
     BGT_STEP(5, s1, s0);
 #undef BGT_STEP
-    hipLaunchKernelGGL(sparse3d_expand, dim3((unsigned)std::max(1, (e.nblocks + 255) / 256)), b, 0,
-                       s, e.items, e.nitems, e.dstamp, e.bstamp, e.nblocks, e.ctl);
     hipLaunchKernelGGL(sparse3d_compact, dim3(need), b, 0, s, s0, s1, e.changed, L, N, A, e.bstamp,
                        e.states, e.ctl, e.nitems);
     BGT_HIP(hipGetLastError());
