@@ -74,7 +74,8 @@ int pinned_mapped(PinnedBuffer &b, size_t bytes);
 // Allocates mapped, coherent pinned memory: *host, and its device address.
 int host_alloc_mapped(size_t bytes, void **host, void **dev);
 // Copies n ints of device memory to mapped pinned memory, given by its device
-// address, with a one-workgroup kernel on stream s.  Under load from other
+// address, with a kernel on stream s (one workgroup per 256 ints, at most
+// 256).  Under load from other
 // threads a device-to-host hipMemcpyAsync can block in the runtime for
 // milliseconds before it is even queued; a kernel launch does not.
 int copy_ints_to_host(const int *src, int *dst, int n, hipStream_t s);

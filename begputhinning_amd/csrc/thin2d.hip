@@ -368,13 +368,13 @@ int setup_item2d(const Plan2D &pl, void *ws, hipStream_t s, Item2D **d_item, uns
 
 __global__ __launch_bounds__(256) void copy_ints(const int *__restrict__ src, int *__restrict__ dst, int n)
 {
-    for (int i = threadIdx.x; i < n; i += 256) dst[i] = src[i];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) dst[i] = src[i];
 }
 
 int copy_ints_to_host(const int *src, int *dst, int n, hipStream_t s)
 {
     if (n <= 0) return BGT_OK;
-    hipLaunchKernelGGL(copy_ints, dim3(1), dim3(256), 0, s, src, dst, n);
+    hipLaunchKernelGGL(copy_ints, dim3((unsigned)std::min(256, (n + 255) / 256)), dim3(256), 0, s, src, dst, n);
     BGT_HIP(hipGetLastError());
     return BGT_OK;
 }
