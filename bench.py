@@ -358,8 +358,9 @@ def main():
                     default=[1000.0, 2000.0, 4000.0, 6000.0, 8000.0, 12000.0],
                     help="Poisson arrival rates of the serving leg (none or 0 = skip it)")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-tracers", type=int, default=8,
-                    help="the server's segment tracer threads")
+    ap.add_argument("--serve-tracers", type=int, default=6,
+                    help="the server's segment tracer threads (6 leave the engine threads "
+                         "enough host cores at high rates)")
     ap.add_argument("--only-serving", action="store_true",
                     help="run only the serving leg and print its figures")
     ap.add_argument("--only-frames", action="store_true",
